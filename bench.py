@@ -1,0 +1,224 @@
+"""Benchmark: H-HIGNN mobility matvec u = M(x) f on one MI355X.
+
+Workload (default): 1,000,000 spheres of radius 1 in 200 Gaussian blobs
+(sigma = 15), blob centres uniform in the cube sized for a 10% mean volume
+fraction, minimum centre distance 2.05, seed 2; gravity forcing (0,0,-1);
+leaf 256, ACA epsilon 1e-4, at most 32 ACA iterations; pair-function
+weights: seeded Xavier initialisation of the 3-128-512-9 MLP.
+
+A step is one matvec (HignnModel.dot) on a built tree with forces and
+velocities resident in device memory.  ``value`` is particle-updates/s of
+the chosen form (16-bit MLP on MFMA, FP32 accumulation, FP32 ACA); the plain
+FP32 path is timed in the same run, and both are checked against an FP64
+evaluation of the same block structure on a random subsample of targets.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
+PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # dense MFMA TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def work_counts(exp, ranks):
+    tree = exp["tree"].astype(np.int64)
+    sz = tree[:, 3] - tree[:, 2]
+    c, f = exp["close"], exp["far"]
+    e_near = int(np.sum(sz[c[:, 0]] * sz[c[:, 1]]))
+    e_far = int(np.sum(ranks.astype(np.int64) * (sz[f[:, 0]] + sz[f[:, 1]])))
+    return e_near, e_far
+
+
+def cpu_baseline(x, w, f, leaf, eps, max_iter, exp, budget_s):
+    """Oracle (C++ restatement, all host threads) timed on a bounded sample of
+    the same workload: CloseDot on every k-th close block and the full ACA of
+    every k-th far block, extrapolated to the whole matvec by pair-function
+    evaluation counts."""
+    from oracle import oracle as O
+    om = O.OracleModel(x, leaf, epsilon=eps, max_iter=max_iter)
+    om.update_coord()
+    nclose, nfar = len(exp["close"]), len(exp["far"])
+    # start small, grow the sample until it costs a few seconds
+    cs, fs = max(1, nclose // 8), max(1, nfar // 8)
+    t, wn, wf = om.sample_time(w, f, cs, fs)
+    target = budget_s
+    while t < 0.2 * target and (cs > 1 or fs > 1):
+        scale = max(2, int(target / max(t, 1e-3) * 0.5))
+        cs, fs = max(1, cs // scale), max(1, fs // scale)
+        t, wn, wf = om.sample_time(w, f, cs, fs)
+    return t, wn, wf, cs, fs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--prec", default="bf16", choices=["bf16", "fp16"])
+    ap.add_argument("--check-targets", type=int, default=2000)
+    ap.add_argument("--fp32-steps", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
+              "not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from hignn_amd import hignn, weights, workloads
+
+    t0 = time.time()
+    x = workloads.positions("c3_blobs_1m", n=args.n if args.n != 1_000_000 else None)
+    n = x.shape[0]
+    log(f"[bench] positions N={n} in {time.time() - t0:.1f}s")
+    f = workloads.gravity(n)
+    w = weights.xavier(0)
+    leaf, eps, max_iter = 256, 1e-4, 32
+
+    hignn.Init()
+    model = hignn.HignnModel(x, leaf)
+    model.load_two_body_weights(w)
+    model.set_epsilon(eps)
+    model.set_max_iter(max_iter)
+    t0 = time.time()
+    model.update_coord(x)
+    t_update = time.time() - t0
+    exp = model.tree_export()
+    log(f"[bench] update_coord {t_update:.2f}s: {len(exp['close'])} close, {len(exp['far'])} far")
+
+    dev = torch.device("cuda:0")
+    f_dev = torch.from_numpy(f).to(dev)
+    u_dev = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run(prec):
+        model.set_precision(prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        model.dot_device(u_dev.data_ptr(), f_dev.data_ptr(), stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    # chosen form: warmup, then K timed steps
+    for i in range(args.warmup):
+        dt = run(args.prec)
+        log(f"[bench] warmup {i} {args.prec}: {dt:.3f}s")
+    times, tcl, tfar = [], [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        dt = run(args.prec)
+        c, fa, _ = model.last_timings()
+        times.append(dt)
+        tcl.append(c)
+        tfar.append(fa)
+        log(f"[bench] step {i} {args.prec}: {dt:.3f}s (close {c:.1f} ms, far {fa:.1f} ms)")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    u_chosen = u_dev.cpu().numpy().copy()
+    ranks = model.far_ranks()
+    e_near, e_far = work_counts(exp, ranks)
+
+    # plain FP32 path in the same run
+    f32_times = []
+    for i in range(args.fp32_steps):
+        dt = run("fp32")
+        f32_times.append(dt)
+        c, fa, _ = model.last_timings()
+        log(f"[bench] fp32 step {i}: {dt:.3f}s (close {c:.1f} ms, far {fa:.1f} ms)")
+    u_fp32 = u_dev.cpu().numpy().copy()
+    ranks32 = model.far_ranks()
+
+    # accuracy: FP64 evaluation of the same blocks at random targets
+    err = err32 = None
+    if args.check_targets > 0:
+        g = np.random.default_rng(7)
+        tg = np.sort(g.choice(n, size=min(args.check_targets, n), replace=False))
+        t0 = time.time()
+        ref = model.check_fp64(f, tg, 0)
+        t_chk = time.time() - t0
+
+        def rel(u):
+            return float(np.sqrt(np.sum((u[tg] - ref) ** 2) / np.sum(ref ** 2)))
+
+        err, err32 = rel(u_chosen), rel(u_fp32)
+        log(f"[bench] fp64 check on {tg.size} targets ({t_chk:.1f}s): {args.prec} err {err:.3e}, "
+            f"fp32 err {err32:.3e}")
+
+    ms = 1000.0 * t_total / args.steps
+    value = n / (t_total / args.steps)
+    fp32_rate = n / float(np.median(f32_times)) if f32_times else None
+    # roofline of the dominant kernel: the near-field MLP kernel
+    t_close_s = float(np.median(tcl)) / 1000.0
+    achieved = MLP_FLOP * e_near / t_close_s / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[args.prec],
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
+            "kernel": f"near_h_kernel<{args.prec}>", "flop_per_launch": MLP_FLOP * e_near,
+            "launch_ms": round(1000 * t_close_s, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            t, wn, wf, cs, fs = cpu_baseline(x, w, f, leaf, eps, max_iter, exp, args.cpu_budget)
+            t_full = t * (e_near + e_far) / max(1, wn + wf)
+            cpu = {"value": round(n / t_full, 2), "unit": "particle-updates/s",
+                   "cores": os.cpu_count(), "kind": "port",
+                   "sample": f"oracle CloseDot on every {cs}th close block + ACA of every {fs}th "
+                             f"far block ({wn + wf} pair evaluations, {t:.1f}s), extrapolated "
+                             f"by evaluation count to the full matvec ({e_near + e_far})"}
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:  # the baseline must not sink the GPU result
+            log(f"[bench] cpu baseline failed: {e}")
+
+    line = {
+        "metric": "matvec particle-updates/s @N=1M: BF16+surrogate vs plain FP32, rel err<=1e-3",
+        "value": round(value, 2),
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.prec,
+        "data": "synthetic",
+        "config": {
+            "workload": f"gaussian blobs N={n} (200 blobs, sigma 15, 10% mean volume fraction, "
+                        f"seed 2), gravity; leaf {leaf}, eps {eps}, max_iter {max_iter}",
+            "far_entries": f"{args.prec} MLP (surrogate not used)",
+            "fp32_value": round(fp32_rate, 2) if fp32_rate else None,
+            "speedup_vs_fp32": round(value / fp32_rate, 3) if fp32_rate else None,
+            "rel_err_fp64": err, "rel_err_fp64_fp32_path": err32,
+            "check_targets": args.check_targets,
+            "e_near": e_near, "e_far": e_far,
+            "mean_far_rank": float(ranks.mean()) if ranks.size else 0.0,
+            "update_coord_s": round(t_update, 3),
+            "close_ms": round(float(np.median(tcl)), 2), "far_ms": round(float(np.median(tfar)), 2),
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+"""ctypes binding of the C ABI in include/hh.h (libhh.so).
+
+This is the binding a non-pybind caller would write; the tests call the
+library through it.  It raises RuntimeError with hh_last_error() on any
+non-zero status, and loads libhh.so from this package only: a missing
+library is an error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhh.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "hh.h")
+
+PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "fp64": 3}
+
+_lib = None
+c_i64, c_vp, c_int, c_dbl = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (python -m hignn_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hh_last_error.restype = ctypes.c_char_p
+        sig = {
+            "hh_init": [], "hh_finalize": [],
+            "hh_model_create": [c_vp, c_i64, c_int, c_vp],
+            "hh_model_load_two_body": [c_vp, c_vp, c_i64],
+            "hh_model_set_epsilon": [c_vp, c_dbl], "hh_model_set_eta": [c_vp, c_dbl],
+            "hh_model_set_max_iter": [c_vp, c_int],
+            "hh_model_set_mat_pool_size_factor": [c_vp, c_int],
+            "hh_model_set_max_far_dot_work_node_size": [c_vp, c_int],
+            "hh_model_set_max_relative_coord": [c_vp, c_i64],
+            "hh_model_set_post_check_flag": [c_vp, c_int],
+            "hh_model_set_use_symmetry_flag": [c_vp, c_int],
+            "hh_model_set_max_far_field_distance": [c_vp, c_dbl],
+            "hh_model_set_precision": [c_vp, c_int],
+            "hh_model_update_coord": [c_vp, c_vp, c_i64],
+            "hh_model_dot": [c_vp, c_vp, c_vp, c_i64],
+            "hh_model_dense_dot": [c_vp, c_vp, c_vp, c_i64],
+            "hh_model_dot_device": [c_vp, c_vp, c_vp, c_i64, c_vp],
+            "hh_model_tree_counts": [c_vp, c_vp],
+            "hh_model_tree_export": [c_vp] * 7,
+            "hh_model_far_ranks": [c_vp, c_vp],
+            "hh_model_check_fp64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp],
+            "hh_model_eval_pair": [c_vp, c_vp, c_i64, c_vp, c_int],
+            "hh_model_last_timings": [c_vp, c_vp],
+            "hh_tree_build": [c_vp, c_i64, c_int, c_int, c_i64, c_vp],
+            "hh_tree_counts": [c_vp, c_vp],
+            "hh_tree_export": [c_vp] * 7,
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        L.hh_model_destroy.argtypes = [c_vp]
+        L.hh_model_destroy.restype = None
+        L.hh_tree_free.argtypes = [c_vp]
+        L.hh_tree_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError(f"libhh error {rc}: {lib().hh_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(c_vp) if a is not None else None
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, 3))
+
+
+def _export(counts_fn, export_fn, handle, n):
+    c = np.zeros(4, dtype=np.int64)
+    _chk(counts_fn(handle, _p(c)))
+    T, nc, nf, nl = (int(v) for v in c)
+    out = dict(tree=np.empty((T, 5), np.uint64), perm=np.empty(n, np.uint64),
+               aux=np.empty((T, 6), np.float32), close=np.empty((nc, 2), np.int64),
+               far=np.empty((nf, 2), np.int64), leaves=np.empty(nl, np.int64))
+    _chk(export_fn(handle, *[_p(out[k]) for k in ("tree", "perm", "aux", "close", "far",
+                                                      "leaves")]))
+    return out
+
+
+def tree_build(x, leaf, use_symmetry=True, max_relative_coord=500000):
+    """Host-only tree + block lists (no device)."""
+    x = _f32(x)
+    h = c_vp()
+    _chk(lib().hh_tree_build(_p(x), x.shape[0], int(leaf), int(bool(use_symmetry)),
+                             int(max_relative_coord), ctypes.byref(h)))
+    try:
+        return _export(lib().hh_tree_counts, lib().hh_tree_export, h, x.shape[0])
+    finally:
+        lib().hh_tree_free(h)
+
+
+class Model:
+    """Thin object wrapper over an hh_model handle."""
+
+    def __init__(self, x, leaf):
+        _chk(lib().hh_init())
+        self.x = _f32(x)
+        self.n = self.x.shape[0]
+        h = c_vp()
+        _chk(lib().hh_model_create(_p(self.x), self.n, int(leaf), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hh_model_destroy(self.h)
+            self.h = None
+
+    def load_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        _chk(lib().hh_model_load_two_body(self.h, _p(w), w.size))
+
+    def set(self, epsilon=None, max_iter=None, use_symmetry=None, max_relative_coord=None,
+            precision=None):
+        L = lib()
+        if epsilon is not None:
+            _chk(L.hh_model_set_epsilon(self.h, float(epsilon)))
+        if max_iter is not None:
+            _chk(L.hh_model_set_max_iter(self.h, int(max_iter)))
+        if use_symmetry is not None:
+            _chk(L.hh_model_set_use_symmetry_flag(self.h, int(bool(use_symmetry))))
+        if max_relative_coord is not None:
+            _chk(L.hh_model_set_max_relative_coord(self.h, int(max_relative_coord)))
+        if precision is not None:
+            _chk(L.hh_model_set_precision(self.h, PREC[precision]))
+
+    def update_coord(self, x=None):
+        if x is not None:
+            self.x = _f32(x)
+        _chk(lib().hh_model_update_coord(self.h, _p(self.x), self.n))
+
+    def dot(self, f):
+        f = _f32(f)
+        u = np.empty((self.n, 3), np.float32)
+        _chk(lib().hh_model_dot(self.h, _p(u), _p(f), self.n))
+        return u
+
+    def dense_dot(self, f):
+        f = _f32(f)
+        u = np.empty((self.n, 3), np.float32)
+        _chk(lib().hh_model_dense_dot(self.h, _p(u), _p(f), self.n))
+        return u
+
+    def dot_device(self, u_ptr, f_ptr, stream=None):
+        _chk(lib().hh_model_dot_device(self.h, c_vp(u_ptr), c_vp(f_ptr), self.n,
+                                       c_vp(stream) if stream else None))
+
+    def export(self):
+        return _export(lib().hh_model_tree_counts, lib().hh_model_tree_export, self.h, self.n)
+
+    def far_ranks(self):
+        c = np.zeros(4, dtype=np.int64)
+        _chk(lib().hh_model_tree_counts(self.h, _p(c)))
+        r = np.empty(int(c[2]), np.int32)
+        _chk(lib().hh_model_far_ranks(self.h, _p(r)))
+        return r
+
+    def check_fp64(self, f, targets, convention=0):
+        f = _f32(f)
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        out = np.empty((t.size, 3), np.float64)
+        _chk(lib().hh_model_check_fp64(self.h, _p(f), _p(t), t.size, int(convention), _p(out)))
+        return out
+
+    def eval_pair(self, r, precision="fp32"):
+        r = _f32(r)
+        out = np.empty((r.shape[0], 9), np.float32)
+        _chk(lib().hh_model_eval_pair(self.h, _p(r), r.shape[0], _p(out), PREC[precision]))
+        return out
+
+    def timings(self):
+        t = np.zeros(3, np.float64)
+        _chk(lib().hh_model_last_timings(self.h, _p(t)))
+        return tuple(float(v) for v in t)
+
+
+def header_symbols():
+    """Function names declared in include/hh.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(hh_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,99 @@
+// Launch descriptors of the HIP kernels in hh_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hh_internal.h"
+
+namespace hh {
+
+// Device copies of the pair-function weights in MFMA fragment order.
+struct DevWeightsF {  // FP32 or FP64 elements
+  const void *w1b, *w2f, *b2, *w3f, *b3;
+};
+struct DevWeightsH {  // 16-bit fragments + FP32 layer 1 and biases
+  const float *w1b, *b2, *b3;
+  const void *w2f, *w3f;
+};
+
+struct NearLaunch {
+  int prec;
+  const float4 *xp;
+  const float4 *fp;
+  const int32_t *tlist;
+  const RowBlock *blocks;
+  const NearTask *tasks;
+  int64_t ntasks;
+  double *u;
+  DevWeightsF wf;
+  DevWeightsH wh;
+  int max_blocks;  // grid cap for the FP32/FP64 kernels
+  int cus;         // compute units (grid of the LDS-resident 16-bit kernel)
+};
+
+struct AcaMlpLaunch {
+  int prec;
+  const float4 *xp;
+  const FarBlock *blk;
+  const int32_t *active;
+  const int32_t *it;
+  const int32_t *ipiv;
+  const int32_t *jpiv;
+  const int64_t *ybase;
+  const void *tiles;  // (slot, offset) pairs
+  int64_t ntiles;
+  int max_iter;
+  int row_pass;
+  float *Y;
+  DevWeightsF wf;
+  DevWeightsH wh;
+  int max_blocks;
+  int cus;
+};
+
+struct AcaStepLaunch {
+  const FarBlock *blk;
+  const int32_t *active;
+  const int64_t *ybase;
+  const float *Y;
+  int32_t *it;
+  int32_t *ipiv;
+  int32_t *jpiv;
+  double *mu2;
+  double *nu2;
+  int32_t *stop;
+  int max_iter;
+  double eps2;
+  int nactive;
+  int pool_f64;
+  void *C;
+  void *Q;
+};
+
+struct AcaApplyLaunch {
+  const FarBlock *blk;
+  const int32_t *list;
+  const int32_t *it;
+  const float4 *fp;
+  double *u;
+  int use_symmetry;
+  int nlist;
+  int pool_f64;
+  const void *C;
+  const void *Q;
+};
+
+hipError_t launch_near(const NearLaunch &L, hipStream_t st);
+hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);
+hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st);
+hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st);
+hipError_t launch_pair(int prec, const float *r, int64_t P, float *out, DevWeightsF wf,
+                       DevWeightsH wh, int cus, hipStream_t st);
+hipError_t launch_gather_forces(const float *f, int64_t n, const int32_t *perm, float4 *fp,
+                                hipStream_t st);
+hipError_t launch_scatter_velocity(const double *u, int64_t n, const int32_t *perm, float *out,
+                                   hipStream_t st);
+
+}  // namespace hh

@@ -1,0 +1,1122 @@
+// HIP kernels of libhh for gfx950 (CDNA4, 64-wide waves, MFMA).
+//
+// The pair function (python/convert.py:11-20)
+//     y = W3 tanh(W2 tanh(W1 r + b1) + b2) + b3,  M = (y + I)/|r|,
+// is evaluated for 16 displacements per MFMA tile with the displacement index
+// on the MFMA column axis and the hidden features on the row axis, so that
+// each layer's output tile is directly the next layer's B operand (register
+// chaining, no round trip through memory):
+//   * FP32 / FP64 path: v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64.  Lane l
+//     (g = l/16, p = l%16) holds A[p][g], B[g][p], and D[4g+r][p] (r = 0..3).
+//     Layer 1 (K = 3) runs on the VALU in FP32/FP64 and produces feature
+//     4s+g of K-step s; layer 2's D register r of row tile mt is feature
+//     16mt+4g+r, which is the K-slot g of a layer-3 step whose A operand holds
+//     W3 columns 16mt+4g+r (host-side fragment layout).
+//   * 16-bit path: v_mfma_f32_16x16x32_{bf16,f16}.  Lane holds
+//     A[p][8g+j], B[8g+j][p] (j = 0..7), D[4g+r][p].  Layer 1 in FP32 on the
+//     VALU, rounded to 16 bit; two layer-2 row tiles (32 features) form one
+//     layer-3 K-step (K-slot 8g+j <-> feature 32q+4g+j for j<4 and
+//     32q+16+4g+j-4 for j>=4).  W2/W3 fragments are staged in LDS once per
+//     persistent workgroup.
+// The epilogue (+I, /|r|, |r| < 1e-5 -> I, M f) is lane-local: the lane that
+// holds output e = 3 row + col of a pair adds its share of M f directly, and
+// the per-pair sums are completed by cross-lane reductions, so the 3x3 block
+// never has to be gathered into one lane.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hh_internal.h"
+#include "hh_kernels.h"
+
+namespace hh {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mma(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mma(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mma(f16x8 a, f16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// tanh x = 1 - 2/(e^{2x} + 1), with e^{2x} = 2^{2x log2 e}: one v_exp_f32 and
+// one v_rcp_f32.  Saturates correctly at +-1 (e = inf -> 1, e = 0 -> -1);
+// absolute error a few ulp of 1.
+__device__ __forceinline__ float tanh_f(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.885390081777927f);
+  return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+}
+__device__ __forceinline__ double tanh_f(double x) { return tanh(x); }
+
+template <class R>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef f32x4 T;
+};
+template <>
+struct Vec4<double> {
+  typedef f64x4 T;
+};
+
+// ---------------------------------------------------------------------------
+// FP32 / FP64 MLP on NT tiles of 16 displacements.
+// ---------------------------------------------------------------------------
+template <class R>
+struct WF {
+  const R *w1b;    // [128][4] = W1 row k (3) + b1[k]
+  const R *w2f;    // [32 mt][32 s][64 lanes]: W2[16mt + p][4s + g]
+  const R *b2;     // [512]
+  const R *w3f;    // [32 mt][4 r][64 lanes]: W3[p][16mt + 4g + r] (0 for p >= 9)
+  const R *b3;     // [16] (0 beyond 9)
+};
+
+template <class R, int NT>
+__device__ __forceinline__ void mlp_tiles_f(const WF<R> &W, const R (&rx)[NT], const R (&ry)[NT],
+                                            const R (&rz)[NT],
+                                            typename Vec4<R>::T (&y)[NT]) {
+  typedef typename Vec4<R>::T V;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  R h1[NT][32];
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+    const R *w = W.w1b + 4 * (4 * s + g);
+    const R w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      R a = b;
+      a += w0 * rx[t];
+      a += w1 * ry[t];
+      a += w2 * rz[t];
+      h1[t][s] = tanh_f(a);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) y[t] = V{0, 0, 0, 0};
+  for (int mt = 0; mt < 32; mt++) {
+    const R *bb = W.b2 + 16 * mt + 4 * g;
+    V acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = V{bb[0], bb[1], bb[2], bb[3]};
+    const R *a2 = W.w2f + (size_t)mt * 32 * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < 32; s++) {
+      const R a = a2[s * 64];
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[t] = mma(a, h1[t][s], acc[t]);
+    }
+    const R *a3 = W.w3f + (size_t)mt * 4 * 64 + lane;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const R a = a3[r * 64];
+#pragma unroll
+      for (int t = 0; t < NT; t++) y[t] = mma(a, tanh_f(acc[t][r]), y[t]);
+    }
+  }
+  const R *b3 = W.b3 + 4 * g;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) y[t][r] += b3[r];
+}
+
+// ---------------------------------------------------------------------------
+// 16-bit MLP (BF16 or FP16 operands, FP32 accumulation).
+// ---------------------------------------------------------------------------
+template <class H>
+struct H16;
+template <>
+struct H16<bf16x8> {
+  typedef __bf16 E;
+};
+template <>
+struct H16<f16x8> {
+  typedef _Float16 E;
+};
+
+struct WH {
+  const float *w1b;  // [128][4]
+  const float *b2;   // [512]
+  const float *b3;   // [16]
+  const void *w2f;   // global: [32 mt][4 s][64 lanes] x 8 halves
+  const void *w3f;   // global: [16 q][64 lanes] x 8 halves
+};
+
+constexpr int kW2FragBytes = 32 * 4 * 64 * 16;  // 128 KiB
+constexpr int kW3FragBytes = 16 * 64 * 16;      // 16 KiB
+
+template <class H, int NT>
+__device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *lw3,
+                                            const float (&rx)[NT], const float (&ry)[NT],
+                                            const float (&rz)[NT], f32x4 (&y)[NT]) {
+  typedef typename H16<H>::E E;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  H h1[NT][4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float *w = W.w1b + 4 * (32 * s + 8 * g + j);
+      const float w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        float a = b;
+        a += w0 * rx[t];
+        a += w1 * ry[t];
+        a += w2 * rz[t];
+        h1[t][s][j] = (E)tanh_f(a);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) y[t] = f32x4{0, 0, 0, 0};
+  for (int q = 0; q < 16; q++) {
+    const float *bb0 = W.b2 + 32 * q + 4 * g;
+    const float *bb1 = bb0 + 16;
+    f32x4 acc0[NT], acc1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      acc0[t] = f32x4{bb0[0], bb0[1], bb0[2], bb0[3]};
+      acc1[t] = f32x4{bb1[0], bb1[1], bb1[2], bb1[3]};
+    }
+    const H *a0p = lw2 + (size_t)(2 * q) * 4 * 64 + lane;
+    const H *a1p = lw2 + (size_t)(2 * q + 1) * 4 * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const H a0 = a0p[s * 64];
+      const H a1 = a1p[s * 64];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        acc0[t] = mma(a0, h1[t][s], acc0[t]);
+        acc1[t] = mma(a1, h1[t][s], acc1[t]);
+      }
+    }
+    const H a3 = lw3[q * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      H b;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        b[r] = (E)tanh_f(acc0[t][r]);
+        b[4 + r] = (E)tanh_f(acc1[t][r]);
+      }
+      y[t] = mma(a3, b, y[t]);
+    }
+  }
+  const float *b3 = W.b3 + 4 * g;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) y[t][r] += b3[r];
+}
+
+// ---------------------------------------------------------------------------
+// Shared epilogue helpers.
+// ---------------------------------------------------------------------------
+// Value of block entry e (0..8) of pair with displacement r given the raw
+// network output yv: (y + delta)/|r|, or delta when |r| < 1e-5
+// (python/convert.py:14-19).  Computed in the network's precision.
+template <class R>
+__device__ __forceinline__ R block_entry(R yv, int e, R nr) {
+  const R delta = (e == 0 || e == 4 || e == 8) ? (R)1 : (R)0;
+  return nr < (R)1e-5 ? delta : (yv + delta) / nr;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// sum over the four lane groups g (lanes p, p+16, p+32, p+48)
+__device__ __forceinline__ double group_sum(double v) {
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Near field / dense row blocks: u_i += sum_j M(x_j - x_i) f_j  (CloseDot,
+// src/CloseDot.cpp:245-299; DenseDot, src/DenseDot.cpp:230-246).
+// One wave handles one task (4 consecutive targets of a row block) and walks
+// the block's sources 16 at a time; tile t of the MLP is target t.
+// ---------------------------------------------------------------------------
+template <class R>
+struct NearSrc {
+  const float4 *xp;    // tree-ordered positions (w unused)
+  const float4 *fp;    // tree-ordered forces
+  const int32_t *tlist;  // explicit targets (BF_EXPLICIT_I)
+  const RowBlock *blocks;
+  const NearTask *tasks;
+  int64_t ntasks;
+  double *u;           // (.,3) accumulators
+};
+
+template <class R>
+__device__ __forceinline__ void near_epilogue(const typename Vec4<R>::T (&y)[4], const R (&rx)[4],
+                                              const R (&ry)[4], const R (&rz)[4],
+                                              const bool (&ivalid)[4], bool jvalid,
+                                              const float4 &fj, const float4 (&fi)[4], int flags,
+                                              double (&v)[4][3], double (&w)[3]) {
+  const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (!(ivalid[t] && jvalid)) continue;
+    const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int e = 4 * g + rr;
+      if (e >= 9) continue;
+      const int row = e / 3, col = e - 3 * (e / 3);
+      const double val = (double)block_entry<R>((R)y[t][rr], e, nr);
+      const float fjc = col == 0 ? fj.x : (col == 1 ? fj.y : fj.z);
+      if (flags & BF_SYMM) {
+        const float fjr = row == 0 ? fj.x : (row == 1 ? fj.y : fj.z);
+        v[t][row] += 0.5 * val * (double)fjc;
+        v[t][col] += 0.5 * val * (double)fjr;
+      } else {
+        v[t][row] += val * (double)fjc;
+      }
+      if (flags & BF_REUSE) {
+        const float fic = col == 0 ? fi[t].x : (col == 1 ? fi[t].y : fi[t].z);
+        w[row] += val * (double)fic;
+      }
+    }
+  }
+}
+
+template <class R>
+__device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
+  const int lane = threadIdx.x & 63;
+  const int p = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t task = wave; task < S.ntasks; task += nwaves) {
+    const NearTask tk = S.tasks[task];
+    const RowBlock B = S.blocks[tk.block];
+    int ti[4];
+    bool iv[4];
+    float4 xi[4], fi[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      iv[t] = tk.ioff + t < B.m;
+      const int li = B.i0 + tk.ioff + (iv[t] ? t : 0);
+      ti[t] = li;
+      const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
+      xi[t] = S.xp[xi_idx];
+      fi[t] = S.fp[xi_idx];
+    }
+    double v[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int j0 = 0; j0 < B.n; j0 += 16) {
+      const bool jv = j0 + p < B.n;
+      const int j = B.j0 + j0 + (jv ? p : 0);
+      const float4 xj = S.xp[j];
+      const float4 fj = S.fp[j];
+      R rx[4], ry[4], rz[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        if (B.flags & BF_NEG) {
+          rx[t] = (R)(xi[t].x - xj.x), ry[t] = (R)(xi[t].y - xj.y), rz[t] = (R)(xi[t].z - xj.z);
+        } else {
+          rx[t] = (R)(xj.x - xi[t].x), ry[t] = (R)(xj.y - xi[t].y), rz[t] = (R)(xj.z - xi[t].z);
+        }
+      }
+      typename Vec4<R>::T y[4];
+      mlp_tiles_f<R, 4>(W, rx, ry, rz, y);
+      double w[3] = {0, 0, 0};
+      near_epilogue<R>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
+      if (B.flags & BF_REUSE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
+        if (lane < 16 && jv)
+#pragma unroll
+          for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double s = wave_sum(v[t][c]);
+        if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void near_fp32_kernel(NearSrc<float> S, WF<float> W) {
+  near_body_f<float>(S, W);
+}
+
+// FP64 variant, two tiles at a time is not needed for correctness; the same
+// four-target task shape is used with double registers.
+__global__ __launch_bounds__(256) void near_fp64_kernel(NearSrc<double> S, WF<double> W) {
+  near_body_f<double>(S, W);
+}
+
+// 16-bit near kernel: persistent workgroups stage W2/W3 fragments in LDS.
+template <class H>
+__global__ __launch_bounds__(256) void near_h_kernel(NearSrc<float> S, WH W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  H *lw2 = (H *)lds;
+  H *lw3 = (H *)(lds + kW2FragBytes);
+  {
+    const uint4 *src2 = (const uint4 *)W.w2f;
+    const uint4 *src3 = (const uint4 *)W.w3f;
+    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
+    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
+    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const int p = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t task = wave; task < S.ntasks; task += nwaves) {
+    const NearTask tk = S.tasks[task];
+    const RowBlock B = S.blocks[tk.block];
+    int ti[4];
+    bool iv[4];
+    float4 xi[4], fi[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      iv[t] = tk.ioff + t < B.m;
+      const int li = B.i0 + tk.ioff + (iv[t] ? t : 0);
+      ti[t] = li;
+      const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
+      xi[t] = S.xp[xi_idx];
+      fi[t] = S.fp[xi_idx];
+    }
+    double v[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int j0 = 0; j0 < B.n; j0 += 16) {
+      const bool jv = j0 + p < B.n;
+      const int j = B.j0 + j0 + (jv ? p : 0);
+      const float4 xj = S.xp[j];
+      const float4 fj = S.fp[j];
+      float rx[4], ry[4], rz[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        if (B.flags & BF_NEG) {
+          rx[t] = xi[t].x - xj.x, ry[t] = xi[t].y - xj.y, rz[t] = xi[t].z - xj.z;
+        } else {
+          rx[t] = xj.x - xi[t].x, ry[t] = xj.y - xi[t].y, rz[t] = xj.z - xi[t].z;
+        }
+      }
+      f32x4 y[4];
+      mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y);
+      double w[3] = {0, 0, 0};
+      near_epilogue<float>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
+      if (B.flags & BF_REUSE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
+        if (lane < 16 && jv)
+#pragma unroll
+          for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double s = wave_sum(v[t][c]);
+        if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ACA: batched block cross approximation (src/FarDot.cpp:186-1388).
+// ---------------------------------------------------------------------------
+struct AcaTile {
+  int32_t slot;  // index into the active list
+  int32_t off;   // first item (row or column index inside the block)
+};
+
+struct AcaMlpArgs {
+  const float4 *xp;
+  const FarBlock *blk;
+  const int32_t *active;  // slot -> block
+  const int32_t *it;      // per block: current iteration k
+  const int32_t *ipiv;    // per block [maxIter]
+  const int32_t *jpiv;    // per block [maxIter+1]
+  const int64_t *ybase;   // per slot: first row of Y
+  const AcaTile *tiles;
+  int64_t ntiles;
+  int max_iter;
+  int row_pass;           // 0: column pass (items i in I), 1: row pass (j in J)
+  float *Y;               // (items, 9) block entries M (not symmetrised)
+};
+
+template <class Core>
+__device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, p = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t q0 = 4 * wave; q0 < A.ntiles; q0 += 4 * nwaves) {
+    float rx[4], ry[4], rz[4];
+    bool valid[4];
+    int64_t yrow[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int64_t q = q0 + t < A.ntiles ? q0 + t : A.ntiles - 1;
+      const AcaTile tl = A.tiles[q];
+      const int b = A.active[tl.slot];
+      const FarBlock fb = A.blk[b];
+      const int k = A.it[b];
+      const int item = tl.off + p;
+      const int lim = A.row_pass ? fb.n : fb.m;
+      valid[t] = (q0 + t < A.ntiles) && item < lim;
+      const int itc = valid[t] ? item : 0;
+      float4 xa, xb;  // displacement xb - xa
+      if (!A.row_pass) {
+        xb = A.xp[fb.js + A.jpiv[(size_t)b * (A.max_iter + 1) + k]];
+        xa = A.xp[fb.is + itc];
+      } else {
+        xb = A.xp[fb.js + itc];
+        xa = A.xp[fb.is + A.ipiv[(size_t)b * A.max_iter + k]];
+      }
+      rx[t] = xb.x - xa.x, ry[t] = xb.y - xa.y, rz[t] = xb.z - xa.z;
+      yrow[t] = A.ybase[tl.slot] + itc;
+    }
+    f32x4 y[4];
+    core(rx, ry, rz, y);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (!valid[t]) continue;
+      const float nr = sqrtf(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int e = 4 * g + rr;
+        if (e < 9) A.Y[9 * yrow[t] + e] = block_entry<float>(y[t][rr], e, nr);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void aca_mlp_fp32_kernel(AcaMlpArgs A, WF<float> W) {
+  aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
+                      f32x4(&y)[4]) { mlp_tiles_f<float, 4>(W, rx, ry, rz, y); });
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  H *lw2 = (H *)lds;
+  H *lw3 = (H *)(lds + kW2FragBytes);
+  {
+    const uint4 *src2 = (const uint4 *)W.w2f;
+    const uint4 *src3 = (const uint4 *)W.w3f;
+    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
+    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
+    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
+    __syncthreads();
+  }
+  aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
+                      f32x4(&y)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y); });
+}
+
+// ---- per-block steps (one workgroup of 256 threads per active block) ------
+struct AcaStepArgs {
+  const FarBlock *blk;
+  const int32_t *active;
+  const int64_t *ybase;
+  const float *Y;
+  int32_t *it;
+  int32_t *ipiv;
+  int32_t *jpiv;
+  double *mu2;
+  double *nu2;
+  int32_t *stop;
+  int max_iter;
+  double eps2;
+};
+
+template <class P>
+__device__ __forceinline__ void load9(const P *src, P (&d)[9]) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) d[e] = src[e];
+}
+
+template <class P>
+__device__ __forceinline__ P det3(const P (&m)[9]) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// symmetrised entries of the cMat/qMat copy (src/FarDot.cpp:441-456): the
+// off-diagonal pair is added in float, then halved.
+template <class P>
+__device__ __forceinline__ void sym_load(const float *y, P (&d)[9]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      d[3 * r + c] = (r == c) ? (P)y[3 * r + c] : (P)(0.5 * (double)(y[3 * r + c] + y[3 * c + r]));
+}
+
+struct ArgMax {
+  double val;
+  int idx;
+};
+
+__device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b) {
+  // larger value wins; equal values: smaller index (first index wins)
+  if (b.idx < 0) return a;
+  if (a.idx < 0) return b;
+  if (b.val > a.val || (b.val == a.val && b.idx < a.idx)) return b;
+  return a;
+}
+
+__device__ ArgMax block_argmax(ArgMax v, ArgMax *sh) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    ArgMax o;
+    o.val = __shfl_xor(v.val, m, 64);
+    o.idx = __shfl_xor(v.idx, m, 64);
+    v = better(v, o);
+  }
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  ArgMax r = sh[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = better(r, sh[k]);
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_sum(double v, double *sh) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+// Column residual, row pivot and column normalisation (src/FarDot.cpp:474-656).
+template <class P>
+__global__ __launch_bounds__(256) void aca_col_step_kernel(AcaStepArgs A, P *Cp, P *Qp) {
+  __shared__ ArgMax sh_am[8];
+  __shared__ double sh_inv[9];
+  const int slot = blockIdx.x;
+  const int b = A.active[slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
+  const int32_t *ipv = A.ipiv + (size_t)b * A.max_iter;
+  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  ArgMax best{-1.7976931348623157e308, -1};
+  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
+    P c[9];
+    sym_load<P>(A.Y + 9 * (A.ybase[slot] + i), c);
+    P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int l = 0; l < k; l++) {
+      P cl[9], ql[9];
+      load9(Cp + 9 * (fb.coff + (int64_t)l * fb.m + i), cl);
+      load9(Qp + 9 * (fb.qoff + (int64_t)l * fb.n + jk), ql);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+          P &s = sum[3 * r + cc];
+          s += cl[3 * r] * ql[cc];
+          s += cl[3 * r + 1] * ql[3 + cc];
+          s += cl[3 * r + 2] * ql[6 + cc];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) c[e] -= sum[e];
+#pragma unroll
+    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[e];
+    bool used = false;
+    for (int l = 0; l < k; l++) used |= (ipv[l] == i);
+    const double d = (double)det3(c);
+    if (!used && d > best.val) {
+      best.val = d;
+      best.idx = i;
+    }
+  }
+  const ArgMax am = block_argmax(best, sh_am);
+  const int ik = am.idx < 0 ? 0 : am.idx;
+  if (threadIdx.x == 0) {
+    A.ipiv[(size_t)b * A.max_iter + k] = ik;
+    double m[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) m[e] = (double)Ck[9 * (int64_t)ik + e];
+    const double a = m[0], bb = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6],
+                 h = m[7], ii = m[8];
+    const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
+    sh_inv[0] = (e * ii - f * h) / det;
+    sh_inv[1] = -(bb * ii - c * h) / det;
+    sh_inv[2] = (bb * f - c * e) / det;
+    sh_inv[3] = -(d * ii - f * g) / det;
+    sh_inv[4] = (a * ii - c * g) / det;
+    sh_inv[5] = -(a * f - c * d) / det;
+    sh_inv[6] = (d * h - e * g) / det;
+    sh_inv[7] = -(a * h - bb * g) / det;
+    sh_inv[8] = (a * e - bb * d) / det;
+  }
+  __syncthreads();
+  P inv[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++) inv[e] = (P)sh_inv[e];
+  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
+    P c[9], o[9];
+    load9(Ck + 9 * (int64_t)i, c);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++)
+        o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
+#pragma unroll
+    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = o[e];
+  }
+}
+
+// Row residual, column pivot, norms and stop test (src/FarDot.cpp:797-1108).
+template <class P>
+__global__ __launch_bounds__(256) void aca_row_step_kernel(AcaStepArgs A, P *Cp, P *Qp) {
+  __shared__ ArgMax sh_am[8];
+  __shared__ double sh_s[8];
+  const int slot = blockIdx.x;
+  const int b = A.active[slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int ik = A.ipiv[(size_t)b * A.max_iter + k];
+  const int32_t *jpv = A.jpiv + (size_t)b * (A.max_iter + 1);
+  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
+  const P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  ArgMax best{-1.7976931348623157e308, -1};
+  double gq[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = threadIdx.x; j < fb.n; j += blockDim.x) {
+    P q[9];
+    sym_load<P>(A.Y + 9 * (A.ybase[slot] + j), q);
+    P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int l = 0; l < k; l++) {
+      P cl[9], ql[9];
+      load9(Cp + 9 * (fb.coff + (int64_t)l * fb.m + ik), cl);
+      load9(Qp + 9 * (fb.qoff + (int64_t)l * fb.n + j), ql);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+          P &s = sum[3 * r + cc];
+          s += cl[3 * r] * ql[cc];
+          s += cl[3 * r + 1] * ql[3 + cc];
+          s += cl[3 * r + 2] * ql[6 + cc];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) q[e] -= sum[e];
+#pragma unroll
+    for (int e = 0; e < 9; e++) Qk[9 * (int64_t)j + e] = q[e];
+    bool used = false;
+    for (int l = 0; l <= k; l++) used |= (jpv[l] == j);
+    const double d = (double)det3(q);
+    if (!used && d > best.val) {
+      best.val = d;
+      best.idx = j;
+    }
+    double qd[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) qd[e] = (double)q[e];
+    gq[0] += qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+    gq[1] += qd[3] * qd[3] + qd[4] * qd[4] + qd[5] * qd[5];
+    gq[2] += qd[6] * qd[6] + qd[7] * qd[7] + qd[8] * qd[8];
+    gq[3] += qd[0] * qd[3] + qd[1] * qd[4] + qd[2] * qd[5];
+    gq[4] += qd[3] * qd[6] + qd[4] * qd[7] + qd[5] * qd[8];
+    gq[5] += qd[6] * qd[0] + qd[7] * qd[1] + qd[8] * qd[2];
+  }
+  double gc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
+    double c[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) c[e] = (double)Ck[9 * (int64_t)i + e];
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++) gc[kk] += c[kk] * c[kk] + c[kk + 3] * c[kk + 3] + c[kk + 6] * c[kk + 6];
+#pragma unroll
+    for (int kk = 3; kk < 6; kk++) {
+      const int a = kk % 3, bq = (kk + 1) % 3;
+      gc[kk] += c[a] * c[bq] + c[a + 3] * c[bq + 3] + c[a + 6] * c[bq + 6];
+    }
+  }
+  const ArgMax am = block_argmax(best, sh_am);
+  double GC[6], GQ[6];
+#pragma unroll
+  for (int e = 0; e < 6; e++) {
+    GC[e] = block_sum(gc[e], sh_s);
+    GQ[e] = block_sum(gq[e], sh_s);
+  }
+  double mu2 = 0.0, nu2 = 0.0;
+  if (threadIdx.x == 0) {
+    nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
+          2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
+    mu2 = A.mu2[b] + nu2;
+  }
+  // cross terms with the iterations l < k-1 (src/FarDot.cpp:1041-1078)
+  for (int l = 0; l + 1 < k; l++) {
+    double cd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, qd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const P *Cl = Cp + 9 * (fb.coff + (int64_t)l * fb.m);
+    const P *Ql = Qp + 9 * (fb.qoff + (int64_t)l * fb.n);
+    for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
+      double a[9], c[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) a[e] = (double)Ck[9 * (int64_t)i + e], c[e] = (double)Cl[9 * (int64_t)i + e];
+#pragma unroll
+      for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; d2++)
+          cd[3 * d1 + d2] += a[d1] * c[d2] + a[d1 + 3] * c[d2 + 3] + a[d1 + 6] * c[d2 + 6];
+    }
+    for (int j = threadIdx.x; j < fb.n; j += blockDim.x) {
+      double a[9], c[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) a[e] = (double)Qk[9 * (int64_t)j + e], c[e] = (double)Ql[9 * (int64_t)j + e];
+#pragma unroll
+      for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; d2++)
+          qd[3 * d1 + d2] += a[3 * d1] * c[3 * d2] + a[3 * d1 + 1] * c[3 * d2 + 1] +
+                             a[3 * d1 + 2] * c[3 * d2 + 2];
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      const double C_ = block_sum(cd[e], sh_s);
+      const double Q_ = block_sum(qd[e], sh_s);
+      if (threadIdx.x == 0) mu2 += 2.0 * fabs(C_) * fabs(Q_);
+    }
+  }
+  if (threadIdx.x == 0) {
+    A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
+    A.mu2[b] = mu2;
+    A.nu2[b] = nu2;
+    const int itn = k + 1;
+    A.it[b] = itn;
+    const int mn = fb.m < fb.n ? fb.m : fb.n;
+    A.stop[slot] = (nu2 < mu2 * A.eps2 || itn >= A.max_iter || itn >= mn) ? 1 : 0;
+  }
+}
+
+// Apply of stopped blocks (src/FarDot.cpp:1200-1388):
+//   u_I += sum_l C_l (sum_j Q_l[j] f_j);  sym: u_J += sum_l Q_l^T (sum_i C_l[i]^T f_i).
+struct AcaApplyArgs {
+  const FarBlock *blk;
+  const int32_t *list;  // blocks to apply
+  const int32_t *it;
+  const float4 *fp;
+  double *u;
+  int use_symmetry;
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void aca_apply_kernel(AcaApplyArgs A, const P *Cp, const P *Qp) {
+  __shared__ double wsh[128][3];
+  const int b = A.list[blockIdx.x];
+  const FarBlock fb = A.blk[b];
+  const int K = A.it[b];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int pass = 0; pass < (A.use_symmetry ? 2 : 1); pass++) {
+    // pass 0: w_l = sum_j Q_l[j] f_j, then u_i += C_l[i] w_l
+    // pass 1: w_l = sum_i C_l[i]^T f_i, then u_j += Q_l[j]^T w_l
+    const P *Src = pass == 0 ? Qp + 9 * fb.qoff : Cp + 9 * fb.coff;
+    const P *Dst = pass == 0 ? Cp + 9 * fb.coff : Qp + 9 * fb.qoff;
+    const int ns = pass == 0 ? fb.n : fb.m, nd = pass == 0 ? fb.m : fb.n;
+    const int s0 = pass == 0 ? fb.js : fb.is, d0 = pass == 0 ? fb.is : fb.js;
+    for (int l = wv; l < K; l += nw) {
+      double a0 = 0, a1 = 0, a2 = 0;
+      for (int s = lane; s < ns; s += 64) {
+        const P *q = Src + 9 * ((int64_t)l * ns + s);
+        const float4 f = A.fp[s0 + s];
+        if (pass == 0) {
+          a0 += (double)q[0] * f.x + (double)q[1] * f.y + (double)q[2] * f.z;
+          a1 += (double)q[3] * f.x + (double)q[4] * f.y + (double)q[5] * f.z;
+          a2 += (double)q[6] * f.x + (double)q[7] * f.y + (double)q[8] * f.z;
+        } else {
+          a0 += (double)q[0] * f.x + (double)q[3] * f.y + (double)q[6] * f.z;
+          a1 += (double)q[1] * f.x + (double)q[4] * f.y + (double)q[7] * f.z;
+          a2 += (double)q[2] * f.x + (double)q[5] * f.y + (double)q[8] * f.z;
+        }
+      }
+      a0 = wave_sum(a0), a1 = wave_sum(a1), a2 = wave_sum(a2);
+      if (lane == 0 && l < 128) wsh[l][0] = a0, wsh[l][1] = a1, wsh[l][2] = a2;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < nd; d += blockDim.x) {
+      double o0 = 0, o1 = 0, o2 = 0;
+      for (int l = 0; l < K && l < 128; l++) {
+        const P *c = Dst + 9 * ((int64_t)l * nd + d);
+        const double w0 = wsh[l][0], w1 = wsh[l][1], w2 = wsh[l][2];
+        if (pass == 0) {
+          o0 += (double)c[0] * w0 + (double)c[1] * w1 + (double)c[2] * w2;
+          o1 += (double)c[3] * w0 + (double)c[4] * w1 + (double)c[5] * w2;
+          o2 += (double)c[6] * w0 + (double)c[7] * w1 + (double)c[8] * w2;
+        } else {
+          o0 += (double)c[0] * w0 + (double)c[3] * w1 + (double)c[6] * w2;
+          o1 += (double)c[1] * w0 + (double)c[4] * w1 + (double)c[7] * w2;
+          o2 += (double)c[2] * w0 + (double)c[5] * w1 + (double)c[8] * w2;
+        }
+      }
+      double *u = A.u + 3 * (size_t)(d0 + d);
+      unsafeAtomicAdd(u + 0, o0);
+      unsafeAtomicAdd(u + 1, o1);
+      unsafeAtomicAdd(u + 2, o2);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair function on a list of displacements: out9[p] = M(r[p]).
+// ---------------------------------------------------------------------------
+template <class R, class Core>
+__device__ __forceinline__ void pair_body(const float *r, int64_t P, float *out, Core core) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, p = lane & 15;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t base = 64 * wave; base < P; base += 64 * nwaves) {
+    R rx[4], ry[4], rz[4];
+    int64_t idx[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      idx[t] = base + 16 * t + p;
+      const int64_t c = idx[t] < P ? idx[t] : P - 1;
+      rx[t] = (R)r[3 * c], ry[t] = (R)r[3 * c + 1], rz[t] = (R)r[3 * c + 2];
+    }
+    typename Vec4<R>::T y[4];
+    core(rx, ry, rz, y);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (idx[t] >= P) continue;
+      const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int e = 4 * g + rr;
+        if (e < 9) out[9 * idx[t] + e] = (float)block_entry<R>((R)y[t][rr], e, nr);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void pair_f32_kernel(const float *r, int64_t P, float *out,
+                                                       WF<float> W) {
+  pair_body<float>(r, P, out, [&](const float(&x)[4], const float(&y)[4], const float(&z)[4],
+                                  f32x4(&o)[4]) { mlp_tiles_f<float, 4>(W, x, y, z, o); });
+}
+
+__global__ __launch_bounds__(256) void pair_f64_kernel(const float *r, int64_t P, float *out,
+                                                       WF<double> W) {
+  pair_body<double>(r, P, out, [&](const double(&x)[4], const double(&y)[4], const double(&z)[4],
+                                   f64x4(&o)[4]) { mlp_tiles_f<double, 4>(W, x, y, z, o); });
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void pair_h_kernel(const float *r, int64_t P, float *out, WH W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  H *lw2 = (H *)lds;
+  H *lw3 = (H *)(lds + kW2FragBytes);
+  {
+    const uint4 *src2 = (const uint4 *)W.w2f;
+    const uint4 *src3 = (const uint4 *)W.w3f;
+    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
+    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
+    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
+    __syncthreads();
+  }
+  pair_body<float>(r, P, out, [&](const float(&x)[4], const float(&y)[4], const float(&z)[4],
+                                  f32x4(&o)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, x, y, z, o); });
+}
+
+// ---------------------------------------------------------------------------
+// Permutations (src/HignnModel.cpp:252-320).
+// ---------------------------------------------------------------------------
+__global__ void gather_forces_kernel(const float *f, int64_t n, const int32_t *perm, float4 *fp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = perm[i];
+  fp[i] = make_float4(f[3 * s], f[3 * s + 1], f[3 * s + 2], 0.0f);
+}
+
+__global__ void scatter_velocity_kernel(const double *u, int64_t n, const int32_t *perm, float *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = perm[i];
+  out[3 * d] = (float)u[3 * i];
+  out[3 * d + 1] = (float)u[3 * i + 1];
+  out[3 * d + 2] = (float)u[3 * i + 2];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (called from hh_model.cpp).
+// ---------------------------------------------------------------------------
+static void allow_big_lds() {
+  static bool done = false;
+  if (done) return;
+  const int lds = kW2FragBytes + kW3FragBytes;
+  (void)hipFuncSetAttribute((const void *)near_h_kernel<bf16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void *)near_h_kernel<f16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void *)aca_mlp_h_kernel<bf16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void *)aca_mlp_h_kernel<f16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void *)pair_h_kernel<bf16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void *)pair_h_kernel<f16x8>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  done = true;
+}
+
+static int grid_for(int64_t waves_needed, int waves_per_block, int max_blocks) {
+  int64_t blocks = (waves_needed + waves_per_block - 1) / waves_per_block;
+  if (blocks < 1) blocks = 1;
+  if (blocks > max_blocks) blocks = max_blocks;
+  return (int)blocks;
+}
+
+hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
+  if (L.ntasks <= 0) return hipSuccess;
+  if (L.prec == PREC_FP32) {
+    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,
+                (const float *)L.wf.w3f, (const float *)L.wf.b3};
+    const int grid = grid_for(L.ntasks, 4, L.max_blocks);
+    hipLaunchKernelGGL(near_fp32_kernel, dim3(grid), dim3(256), 0, st, S, W);
+  } else if (L.prec == PREC_FP64) {
+    NearSrc<double> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    WF<double> W{(const double *)L.wf.w1b, (const double *)L.wf.w2f, (const double *)L.wf.b2,
+                 (const double *)L.wf.w3f, (const double *)L.wf.b3};
+    const int grid = grid_for(L.ntasks, 4, L.max_blocks);
+    hipLaunchKernelGGL(near_fp64_kernel, dim3(grid), dim3(256), 0, st, S, W);
+  } else {
+    allow_big_lds();
+    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
+    const int grid = grid_for(L.ntasks, 4, L.cus);
+    const size_t lds = kW2FragBytes + kW3FragBytes;
+    if (L.prec == PREC_BF16)
+      hipLaunchKernelGGL(near_h_kernel<bf16x8>, dim3(grid), dim3(256), lds, st, S, W);
+    else
+      hipLaunchKernelGGL(near_h_kernel<f16x8>, dim3(grid), dim3(256), lds, st, S, W);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
+  if (L.ntiles <= 0) return hipSuccess;
+  AcaMlpArgs A{L.xp, L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, (const AcaTile *)L.tiles,
+               L.ntiles, L.max_iter, L.row_pass, L.Y};
+  const int64_t waves = (L.ntiles + 3) / 4;
+  if (L.prec == PREC_FP32) {
+    WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,
+                (const float *)L.wf.w3f, (const float *)L.wf.b3};
+    hipLaunchKernelGGL(aca_mlp_fp32_kernel, dim3(grid_for(waves, 4, L.max_blocks)), dim3(256), 0,
+                       st, A, W);
+  } else {
+    allow_big_lds();
+    WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
+    const size_t lds = kW2FragBytes + kW3FragBytes;
+    const int grid = grid_for(waves, 4, L.cus);
+    if (L.prec == PREC_BF16)
+      hipLaunchKernelGGL(aca_mlp_h_kernel<bf16x8>, dim3(grid), dim3(256), lds, st, A, W);
+    else
+      hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(256), lds, st, A, W);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
+  if (L.nactive <= 0) return hipSuccess;
+  AcaStepArgs A{L.blk, L.active, L.ybase, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2, L.stop,
+                L.max_iter, L.eps2};
+  if (L.pool_f64) {
+    if (which == 0)
+      hipLaunchKernelGGL(aca_col_step_kernel<double>, dim3(L.nactive), dim3(256), 0, st, A,
+                         (double *)L.C, (double *)L.Q);
+    else
+      hipLaunchKernelGGL(aca_row_step_kernel<double>, dim3(L.nactive), dim3(256), 0, st, A,
+                         (double *)L.C, (double *)L.Q);
+  } else {
+    if (which == 0)
+      hipLaunchKernelGGL(aca_col_step_kernel<float>, dim3(L.nactive), dim3(256), 0, st, A,
+                         (float *)L.C, (float *)L.Q);
+    else
+      hipLaunchKernelGGL(aca_row_step_kernel<float>, dim3(L.nactive), dim3(256), 0, st, A,
+                         (float *)L.C, (float *)L.Q);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st) {
+  if (L.nlist <= 0) return hipSuccess;
+  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, L.use_symmetry};
+  if (L.pool_f64)
+    hipLaunchKernelGGL(aca_apply_kernel<double>, dim3(L.nlist), dim3(256), 0, st, A,
+                       (const double *)L.C, (const double *)L.Q);
+  else
+    hipLaunchKernelGGL(aca_apply_kernel<float>, dim3(L.nlist), dim3(256), 0, st, A,
+                       (const float *)L.C, (const float *)L.Q);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair(int prec, const float *r, int64_t P, float *out, DevWeightsF wf,
+                       DevWeightsH wh, int cus, hipStream_t st) {
+  if (P <= 0) return hipSuccess;
+  const int64_t waves = (P + 63) / 64;
+  if (prec == PREC_FP32) {
+    WF<float> W{(const float *)wf.w1b, (const float *)wf.w2f, (const float *)wf.b2,
+                (const float *)wf.w3f, (const float *)wf.b3};
+    hipLaunchKernelGGL(pair_f32_kernel, dim3(grid_for(waves, 4, 8 * cus)), dim3(256), 0, st, r, P,
+                       out, W);
+  } else if (prec == PREC_FP64) {
+    WF<double> W{(const double *)wf.w1b, (const double *)wf.w2f, (const double *)wf.b2,
+                 (const double *)wf.w3f, (const double *)wf.b3};
+    hipLaunchKernelGGL(pair_f64_kernel, dim3(grid_for(waves, 4, 8 * cus)), dim3(256), 0, st, r, P,
+                       out, W);
+  } else {
+    allow_big_lds();
+    WH W{wh.w1b, wh.b2, wh.b3, wh.w2f, wh.w3f};
+    const size_t lds = kW2FragBytes + kW3FragBytes;
+    if (prec == PREC_BF16)
+      hipLaunchKernelGGL(pair_h_kernel<bf16x8>, dim3(grid_for(waves, 4, cus)), dim3(256), lds, st,
+                         r, P, out, W);
+    else
+      hipLaunchKernelGGL(pair_h_kernel<f16x8>, dim3(grid_for(waves, 4, cus)), dim3(256), lds, st,
+                         r, P, out, W);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_forces(const float *f, int64_t n, const int32_t *perm, float4 *fp,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_forces_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f,
+                     n, perm, fp);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_velocity(const double *u, int64_t n, const int32_t *perm, float *out,
+                                   hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_velocity_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     u, n, perm, out);
+  return hipGetLastError();
+}
+
+}  // namespace hh

@@ -1,0 +1,924 @@
+// libhh: model object, orchestration and the C ABI declared in include/hh.h.
+//
+// Mirrors HignnModel (include/HignnModel.hpp, src/HignnModel.cpp): the model
+// owns the coordinates, the tree and block lists built by update_coord, and
+// device copies of everything dot needs.  dot runs
+//   gather f into tree order -> near field (close leaf pairs) -> far field
+//   (batched block ACA + low-rank apply) -> scatter u to the original order,
+// all on one HIP stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hh.h"
+#include "hh_internal.h"
+#include "hh_kernels.h"
+
+using namespace hh;
+
+namespace {
+
+thread_local std::string g_last_error;
+int g_device = -1;
+int g_cus = 256;
+
+struct HHError : std::runtime_error {
+  int code;
+  HHError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      throw HHError(HH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return HH_OK;
+  } catch (const HHError &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return HH_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return HH_ERR_ARG;
+  }
+}
+
+void ensure_device() {
+  if (g_device >= 0) return;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (count <= 0) throw HHError(HH_ERR_DEVICE, "no HIP device");
+  int dev = 0;
+  if (const char *lr = std::getenv("LOCAL_RANK")) dev = std::atoi(lr) % count;
+  HIPCHK(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  g_device = dev;
+}
+
+// Device buffer that only grows.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b) {
+    if (b <= bytes) return;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HHError(HH_ERR_ALLOC, "hipMalloc(" + std::to_string(b) + "): " + hipGetErrorString(e));
+    }
+    bytes = b;
+  }
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Pinned host staging buffer that only grows.
+struct PBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b) {
+    if (b <= bytes) return;
+    if (p) HIPCHK(hipHostFree(p));
+    p = nullptr;
+    bytes = 0;
+    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    bytes = b;
+  }
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+uint16_t to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+uint16_t to_f16(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t u;
+  std::memcpy(&u, &h, 2);
+  return u;
+}
+
+}  // namespace
+
+struct hh_model {
+  int64_t n = 0;
+  int leaf = 0;
+  std::vector<float> x;  // original order
+  // parameters (src/HignnModel.cpp:335-386 defaults)
+  double epsilon = 0.05, eta = 1.0, max_far_dist = 1000.0;
+  int max_iter = 100, pool_factor = 40, far_work = 5000;
+  int64_t max_rel_coord = 500000;
+  bool post_check = false, use_symmetry = true;
+  int prec = PREC_FP32;
+  size_t far_pool_bytes = (size_t)32 << 30;
+  hipStream_t stream = nullptr;
+  // weights
+  bool has_weights = false;
+  std::vector<float> w;
+  DBuf wf32[5], wf64[5], wh_f[3], wh_bf16[2], wh_f16[2];
+  // tree + lists
+  TreeLists tl;
+  bool built = false;
+  DBuf xp, perm, fp, u, fin, uout;
+  DBuf near_blocks, near_tasks;
+  int64_t n_near_tasks = 0;
+  DBuf dense_blocks, dense_tasks;
+  int64_t n_dense_tasks = 0;
+  std::vector<FarBlock> far;  // is, m, js, n (offsets set per batch)
+  std::vector<int32_t> far_rank;
+  // far workspace
+  DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r, tiles_c, tiles_r,
+      applylist, Y, Cpool, Qpool;
+  PBuf h_active, h_ybase_c, h_ybase_r, h_tiles_c, h_tiles_r, h_stop, h_list;
+  // check workspace
+  DBuf chk_blocks, chk_tasks, chk_tlist, chk_u;
+  double t_close = 0, t_far = 0, t_total = 0;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+  ~hh_model() {
+    for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
+                    &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
+                    &active_r, &ybase_c, &ybase_r, &tiles_c, &tiles_r, &applylist, &Y, &Cpool,
+                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u})
+      b->release();
+    for (auto &b : wf32) b.release();
+    for (auto &b : wf64) b.release();
+    for (auto &b : wh_f) b.release();
+    for (auto &b : wh_bf16) b.release();
+    for (auto &b : wh_f16) b.release();
+    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_tiles_c, &h_tiles_r, &h_stop, &h_list})
+      b->release();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void upload(DBuf &b, const void *src, size_t bytes) {
+    b.reserve(bytes);
+    if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+  }
+
+  // ---- weights in fragment order ----------------------------------------
+  template <class R>
+  void prep_f(DBuf (&dst)[5]) {
+    const float *W1 = w.data(), *B1 = W1 + 384, *W2 = B1 + 128, *B2 = W2 + 65536, *W3 = B2 + 512,
+                *B3 = W3 + 4608;
+    std::vector<R> w1b(512), w2f(32 * 32 * 64), b2(512), w3f(32 * 4 * 64), b3(16, (R)0);
+    for (int k = 0; k < 128; k++) {
+      for (int d = 0; d < 3; d++) w1b[4 * k + d] = (R)W1[3 * k + d];
+      w1b[4 * k + 3] = (R)B1[k];
+    }
+    for (int mt = 0; mt < 32; mt++)
+      for (int s = 0; s < 32; s++)
+        for (int l = 0; l < 64; l++) {
+          const int p = l & 15, g = l >> 4;
+          w2f[(mt * 32 + s) * 64 + l] = (R)W2[(16 * mt + p) * 128 + 4 * s + g];
+        }
+    for (int k = 0; k < 512; k++) b2[k] = (R)B2[k];
+    for (int mt = 0; mt < 32; mt++)
+      for (int r = 0; r < 4; r++)
+        for (int l = 0; l < 64; l++) {
+          const int p = l & 15, g = l >> 4;
+          w3f[(mt * 4 + r) * 64 + l] = p < 9 ? (R)W3[p * 512 + 16 * mt + 4 * g + r] : (R)0;
+        }
+    for (int o = 0; o < 9; o++) b3[o] = (R)B3[o];
+    upload(dst[0], w1b.data(), w1b.size() * sizeof(R));
+    upload(dst[1], w2f.data(), w2f.size() * sizeof(R));
+    upload(dst[2], b2.data(), b2.size() * sizeof(R));
+    upload(dst[3], w3f.data(), w3f.size() * sizeof(R));
+    upload(dst[4], b3.data(), b3.size() * sizeof(R));
+  }
+
+  void prep_h(DBuf (&dst)[2], uint16_t (*cvt)(float)) {
+    const float *W2 = w.data() + 512, *W3 = W2 + 65536 + 512;
+    std::vector<uint16_t> w2f(32 * 4 * 64 * 8), w3f(16 * 64 * 8);
+    for (int mt = 0; mt < 32; mt++)
+      for (int s = 0; s < 4; s++)
+        for (int l = 0; l < 64; l++)
+          for (int j = 0; j < 8; j++) {
+            const int p = l & 15, g = l >> 4;
+            w2f[((mt * 4 + s) * 64 + l) * 8 + j] = cvt(W2[(16 * mt + p) * 128 + 32 * s + 8 * g + j]);
+          }
+    for (int q = 0; q < 16; q++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 8; j++) {
+          const int p = l & 15, g = l >> 4;
+          const int feat = j < 4 ? 32 * q + 4 * g + j : 32 * q + 16 + 4 * g + (j - 4);
+          w3f[(q * 64 + l) * 8 + j] = p < 9 ? cvt(W3[p * 512 + feat]) : (uint16_t)0;
+        }
+    upload(dst[0], w2f.data(), w2f.size() * 2);
+    upload(dst[1], w3f.data(), w3f.size() * 2);
+  }
+
+  void load_weights(const float *wts) {
+    w.assign(wts, wts + 71177);
+    prep_f<float>(wf32);
+    prep_f<double>(wf64);
+    // 16-bit path: FP32 layer 1 and biases shared with the FP32 layout
+    const float *W1 = w.data(), *B1 = W1 + 384, *B2 = B1 + 128 + 65536, *B3 = B2 + 512 + 4608;
+    std::vector<float> w1b(512), b2(B2, B2 + 512), b3(16, 0.0f);
+    for (int k = 0; k < 128; k++) {
+      for (int d = 0; d < 3; d++) w1b[4 * k + d] = W1[3 * k + d];
+      w1b[4 * k + 3] = B1[k];
+    }
+    for (int o = 0; o < 9; o++) b3[o] = B3[o];
+    upload(wh_f[0], w1b.data(), 512 * 4);
+    upload(wh_f[1], b2.data(), 512 * 4);
+    upload(wh_f[2], b3.data(), 16 * 4);
+    prep_h(wh_bf16, to_bf16);
+    prep_h(wh_f16, to_f16);
+    has_weights = true;
+  }
+
+  DevWeightsF devw_f(bool f64) const {
+    const DBuf *b = f64 ? wf64 : wf32;
+    return DevWeightsF{b[0].p, b[1].p, b[2].p, b[3].p, b[4].p};
+  }
+  DevWeightsH devw_h(int p) const {
+    const DBuf *b = p == PREC_FP16 ? wh_f16 : wh_bf16;
+    return DevWeightsH{wh_f[0].as<float>(), wh_f[1].as<float>(), wh_f[2].as<float>(), b[0].p,
+                       b[1].p};
+  }
+
+  // ---- update_coord ------------------------------------------------------
+  void update() {
+    build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
+    // positions in tree order
+    std::vector<float> xph(4 * (size_t)n);
+    std::vector<int32_t> ph((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t s = tl.perm[(size_t)i];
+      ph[(size_t)i] = (int32_t)s;
+      for (int d = 0; d < 3; d++) xph[4 * (size_t)i + d] = x[3 * (size_t)s + d];
+      xph[4 * (size_t)i + 3] = 0.0f;
+    }
+    upload(xp, xph.data(), xph.size() * 4);
+    upload(perm, ph.data(), ph.size() * 4);
+    // near-field row blocks and tasks (src/CloseDot.cpp)
+    std::vector<RowBlock> rb;
+    std::vector<NearTask> tk;
+    for (size_t b = 0; b < tl.closeI.size(); b++) {
+      const int64_t I = tl.closeI[b], J = tl.closeJ[b];
+      RowBlock r{(int32_t)tl.start(I), (int32_t)tl.size(I), (int32_t)tl.start(J),
+                 (int32_t)tl.size(J), (use_symmetry && J > I) ? BF_REUSE : 0, 0, 0, 0};
+      rb.push_back(r);
+      for (int32_t o = 0; o < r.m; o += 4) tk.push_back(NearTask{(int32_t)b, o});
+    }
+    upload(near_blocks, rb.data(), rb.size() * sizeof(RowBlock));
+    upload(near_tasks, tk.data(), tk.size() * sizeof(NearTask));
+    n_near_tasks = (int64_t)tk.size();
+    n_dense_tasks = -1;  // built on demand
+    far.clear();
+    for (size_t b = 0; b < tl.farI.size(); b++) {
+      const int64_t I = tl.farI[b], J = tl.farJ[b];
+      far.push_back(FarBlock{(int32_t)tl.start(I), (int32_t)tl.size(I), (int32_t)tl.start(J),
+                             (int32_t)tl.size(J), 0, 0});
+    }
+    far_rank.assign(far.size(), 0);
+    built = true;
+  }
+
+  void ensure_buffers() {
+    fp.reserve(16 * (size_t)n);
+    u.reserve(24 * (size_t)n);
+    fin.reserve(12 * (size_t)n);
+    uout.reserve(12 * (size_t)n);
+  }
+
+  // ---- far field: batched ACA -------------------------------------------
+  void far_field(hipStream_t st) {
+    const int64_t nfar = (int64_t)far.size();
+    if (nfar == 0) return;
+    const bool f64 = prec == PREC_FP32;
+    const size_t esz = (f64 ? 8 : 4) * 9;
+    const int MI = max_iter;
+    // pool budget split between C and Q
+    const int64_t cap = std::max<int64_t>(1, (int64_t)(far_pool_bytes / (2 * esz)));
+    int64_t b0 = 0;
+    while (b0 < nfar) {
+      // batch: blocks [b0, b1) whose reservations fit
+      int64_t b1 = b0, csum = 0, qsum = 0, ysum_c = 0, ysum_r = 0;
+      while (b1 < nfar) {
+        const FarBlock &f = far[(size_t)b1];
+        const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
+        if (b1 > b0 && (csum + kmax * f.m > cap || qsum + kmax * f.n > cap)) break;
+        csum += kmax * f.m;
+        qsum += kmax * f.n;
+        ysum_c += f.m;
+        ysum_r += f.n;
+        b1++;
+      }
+      const int nb = (int)(b1 - b0);
+      std::vector<FarBlock> hb((size_t)nb);
+      int64_t co = 0, qo = 0;
+      for (int b = 0; b < nb; b++) {
+        hb[(size_t)b] = far[(size_t)(b0 + b)];
+        const int64_t kmax = std::min<int64_t>(MI, std::min(hb[(size_t)b].m, hb[(size_t)b].n));
+        hb[(size_t)b].coff = co;
+        hb[(size_t)b].qoff = qo;
+        co += kmax * hb[(size_t)b].m;
+        qo += kmax * hb[(size_t)b].n;
+      }
+      Cpool.reserve((size_t)std::max<int64_t>(co, 1) * esz);
+      Qpool.reserve((size_t)std::max<int64_t>(qo, 1) * esz);
+      Y.reserve((size_t)std::max(ysum_c, ysum_r) * 9 * 4);
+      upload(blk, hb.data(), hb.size() * sizeof(FarBlock));
+      std::vector<int32_t> zeros_it((size_t)nb, 0), jp((size_t)nb * (MI + 1), 0);
+      for (int b = 0; b < nb; b++) jp[(size_t)b * (MI + 1)] = hb[(size_t)b].n / 2;
+      std::vector<double> zd((size_t)nb, 0.0);
+      upload(it, zeros_it.data(), (size_t)nb * 4);
+      ipiv.reserve((size_t)nb * MI * 4);
+      upload(jpiv, jp.data(), jp.size() * 4);
+      upload(mu2, zd.data(), (size_t)nb * 8);
+      upload(nu2, zd.data(), (size_t)nb * 8);
+      stop.reserve((size_t)nb * 4);
+      active_c.reserve((size_t)nb * 4);
+      ybase_c.reserve((size_t)nb * 8);
+      ybase_r.reserve((size_t)nb * 8);
+      applylist.reserve((size_t)nb * 4);
+      h_active.reserve((size_t)nb * 4);
+      h_ybase_c.reserve((size_t)nb * 8);
+      h_ybase_r.reserve((size_t)nb * 8);
+      h_stop.reserve((size_t)nb * 4);
+      h_list.reserve((size_t)nb * 4);
+      std::vector<int32_t> act((size_t)nb);
+      for (int b = 0; b < nb; b++) act[(size_t)b] = b;
+      while (!act.empty()) {
+        const int na = (int)act.size();
+        // tiles of 16 items for the column (i in I) and row (j in J) passes
+        int64_t ntc = 0, ntr = 0;
+        for (int a = 0; a < na; a++) {
+          ntc += (hb[(size_t)act[(size_t)a]].m + 15) / 16;
+          ntr += (hb[(size_t)act[(size_t)a]].n + 15) / 16;
+        }
+        h_tiles_c.reserve((size_t)ntc * 8);
+        h_tiles_r.reserve((size_t)ntr * 8);
+        tiles_c.reserve((size_t)ntc * 8);
+        tiles_r.reserve((size_t)ntr * 8);
+        int32_t *tc = h_tiles_c.as<int32_t>(), *tr = h_tiles_r.as<int32_t>();
+        int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
+        int32_t *ha = h_active.as<int32_t>();
+        int64_t sc = 0, sr = 0, ic = 0, ir = 0;
+        for (int a = 0; a < na; a++) {
+          const FarBlock &f = hb[(size_t)act[(size_t)a]];
+          ha[a] = act[(size_t)a];
+          yc[a] = sc;
+          yr[a] = sr;
+          sc += f.m;
+          sr += f.n;
+          for (int32_t o = 0; o < f.m; o += 16) tc[2 * ic] = a, tc[2 * ic + 1] = o, ic++;
+          for (int32_t o = 0; o < f.n; o += 16) tr[2 * ir] = a, tr[2 * ir + 1] = o, ir++;
+        }
+        HIPCHK(hipMemcpyAsync(active_c.p, ha, (size_t)na * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ybase_c.p, yc, (size_t)na * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ybase_r.p, yr, (size_t)na * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(tiles_c.p, tc, (size_t)ntc * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(tiles_r.p, tr, (size_t)ntr * 8, hipMemcpyHostToDevice, st));
+        AcaMlpLaunch ml{};
+        ml.prec = prec;
+        ml.xp = xp.as<float4>();
+        ml.blk = blk.as<FarBlock>();
+        ml.active = active_c.as<int32_t>();
+        ml.it = it.as<int32_t>();
+        ml.ipiv = ipiv.as<int32_t>();
+        ml.jpiv = jpiv.as<int32_t>();
+        ml.max_iter = MI;
+        ml.Y = Y.as<float>();
+        ml.wf = devw_f(false);
+        ml.wh = devw_h(prec);
+        ml.max_blocks = 8 * g_cus;
+        ml.cus = g_cus;
+        AcaStepLaunch sl{};
+        sl.blk = blk.as<FarBlock>();
+        sl.active = active_c.as<int32_t>();
+        sl.Y = Y.as<float>();
+        sl.it = it.as<int32_t>();
+        sl.ipiv = ipiv.as<int32_t>();
+        sl.jpiv = jpiv.as<int32_t>();
+        sl.mu2 = mu2.as<double>();
+        sl.nu2 = nu2.as<double>();
+        sl.stop = stop.as<int32_t>();
+        sl.max_iter = MI;
+        sl.eps2 = epsilon * epsilon;
+        sl.nactive = na;
+        sl.pool_f64 = f64;
+        sl.C = Cpool.p;
+        sl.Q = Qpool.p;
+        // column pass, residual, row pivot, normalisation
+        ml.row_pass = 0;
+        ml.ybase = ybase_c.as<int64_t>();
+        ml.tiles = tiles_c.p;
+        ml.ntiles = ntc;
+        HIPCHK(launch_aca_mlp(ml, st));
+        sl.ybase = ybase_c.as<int64_t>();
+        HIPCHK(launch_aca_steps(sl, 0, st));
+        // row pass, residual, column pivot, norms, stop test
+        ml.row_pass = 1;
+        ml.ybase = ybase_r.as<int64_t>();
+        ml.tiles = tiles_r.p;
+        ml.ntiles = ntr;
+        HIPCHK(launch_aca_mlp(ml, st));
+        sl.ybase = ybase_r.as<int64_t>();
+        HIPCHK(launch_aca_steps(sl, 1, st));
+        HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const int32_t *hs = h_stop.as<int32_t>();
+        int32_t *hl = h_list.as<int32_t>();
+        int nl = 0;
+        std::vector<int32_t> keep;
+        for (int a = 0; a < na; a++) {
+          if (hs[a])
+            hl[nl++] = act[(size_t)a];
+          else
+            keep.push_back(act[(size_t)a]);
+        }
+        if (nl) {
+          HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+          AcaApplyLaunch al{};
+          al.blk = blk.as<FarBlock>();
+          al.list = applylist.as<int32_t>();
+          al.it = it.as<int32_t>();
+          al.fp = fp.as<float4>();
+          al.u = u.as<double>();
+          al.use_symmetry = use_symmetry;
+          al.nlist = nl;
+          al.pool_f64 = f64;
+          al.C = Cpool.p;
+          al.Q = Qpool.p;
+          HIPCHK(launch_aca_apply(al, st));
+        }
+        act.swap(keep);
+      }
+      std::vector<int32_t> ranks((size_t)nb);
+      HIPCHK(hipMemcpyAsync(ranks.data(), it.p, (size_t)nb * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int b = 0; b < nb; b++) far_rank[(size_t)(b0 + b)] = ranks[(size_t)b];
+      b0 = b1;
+    }
+  }
+
+  void near_field(hipStream_t st, const DBuf &blocks, const DBuf &tasks, int64_t ntasks,
+                  int p, double *uacc, const int32_t *tlist) {
+    NearLaunch L{};
+    L.prec = p;
+    L.xp = xp.as<float4>();
+    L.fp = fp.as<float4>();
+    L.tlist = tlist;
+    L.blocks = blocks.as<RowBlock>();
+    L.tasks = tasks.as<NearTask>();
+    L.ntasks = ntasks;
+    L.u = uacc;
+    L.wf = devw_f(p == PREC_FP64);
+    L.wh = devw_h(p);
+    L.max_blocks = 8 * g_cus;
+    L.cus = g_cus;
+    HIPCHK(launch_near(L, st));
+  }
+
+  void check_ready() {
+    if (!has_weights) throw HHError(HH_ERR_STATE, "two-body model not loaded");
+    if (!built) update();  // the reference would dereference empty lists here
+    ensure_buffers();
+    if (!ev[0])
+      for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  }
+
+  void dot_device(float *u_dev, const float *f_dev, hipStream_t st) {
+    check_ready();
+    HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
+    HIPCHK(hipMemsetAsync(u.p, 0, 24 * (size_t)n, st));
+    HIPCHK(hipEventRecord(ev[0], st));
+    near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+    HIPCHK(hipEventRecord(ev[1], st));
+    far_field(st);
+    HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
+    HIPCHK(hipEventSynchronize(ev[2]));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    t_close = a;
+    t_far = b;
+    t_total = (double)a + (double)b;
+  }
+
+  void dot_host(float *uh, const float *fh, bool dense) {
+    check_ready();
+    HIPCHK(hipMemcpy(fin.p, fh, 12 * (size_t)n, hipMemcpyHostToDevice));
+    if (!dense) {
+      dot_device(uout.as<float>(), fin.as<float>(), stream);
+    } else {
+      build_dense();
+      HIPCHK(launch_gather_forces(fin.as<float>(), n, perm.as<int32_t>(), fp.as<float4>(), stream));
+      HIPCHK(hipMemsetAsync(u.p, 0, 24 * (size_t)n, stream));
+      near_field(stream, dense_blocks, dense_tasks, n_dense_tasks, PREC_FP32, u.as<double>(),
+                 nullptr);
+      HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), uout.as<float>(),
+                                     stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(uh, uout.p, 12 * (size_t)n, hipMemcpyDeviceToHost));
+  }
+
+  // DenseDot blocks: every leaf x leaf pair, symmetrised (src/DenseDot.cpp).
+  void build_dense() {
+    if (n_dense_tasks >= 0) return;
+    std::vector<RowBlock> rb;
+    std::vector<NearTask> tk;
+    for (int64_t I : tl.leaves)
+      for (int64_t J : tl.leaves) {
+        RowBlock r{(int32_t)tl.start(I), (int32_t)tl.size(I), (int32_t)tl.start(J),
+                   (int32_t)tl.size(J), BF_SYMM, 0, 0, 0};
+        const int32_t b = (int32_t)rb.size();
+        rb.push_back(r);
+        for (int32_t o = 0; o < r.m; o += 4) tk.push_back(NearTask{b, o});
+      }
+    upload(dense_blocks, rb.data(), rb.size() * sizeof(RowBlock));
+    upload(dense_tasks, tk.data(), tk.size() * sizeof(NearTask));
+    n_dense_tasks = (int64_t)tk.size();
+  }
+
+  // FP64 evaluation at selected targets (PostCheckDot).
+  void check_fp64(const float *fh, const int64_t *targets, int64_t nt, int conv, double *out) {
+    check_ready();
+    HIPCHK(hipMemcpy(fin.p, fh, 12 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(launch_gather_forces(fin.as<float>(), n, perm.as<int32_t>(), fp.as<float4>(), stream));
+    std::vector<int64_t> inv((size_t)n);
+    for (int64_t i = 0; i < n; i++) inv[(size_t)tl.perm[(size_t)i]] = i;
+    // leaf of every tree position, parent of every node
+    std::vector<int64_t> pos_leaf((size_t)n, -1), parent((size_t)tl.T, -1);
+    for (int64_t lf : tl.leaves)
+      for (int64_t p = tl.start(lf); p < tl.start(lf) + tl.size(lf); p++) pos_leaf[(size_t)p] = lf;
+    for (int64_t nd = 0; nd < tl.T; nd++)
+      if (!tl.leaf(nd)) {
+        parent[(size_t)tl.rows[5 * nd]] = nd;
+        parent[(size_t)tl.rows[5 * nd + 1]] = nd;
+      }
+    // targets grouped by leaf; tlist holds tree positions
+    std::vector<std::pair<int64_t, int64_t>> order;  // (leaf, sample index)
+    for (int64_t q = 0; q < nt; q++) {
+      if (targets[q] < 0 || targets[q] >= n) throw HHError(HH_ERR_ARG, "target out of range");
+      const int64_t p = inv[(size_t)targets[q]];
+      order.push_back({pos_leaf[(size_t)p], q});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int64_t, int64_t> &a, const std::pair<int64_t, int64_t> &b) {
+                       return a.first < b.first;
+                     });
+    std::vector<int32_t> tlist((size_t)nt);
+    for (int64_t q = 0; q < nt; q++) tlist[(size_t)q] = (int32_t)inv[(size_t)targets[order[(size_t)q].second]];
+    // per-node source lists of the block convention
+    struct Src {
+      int32_t j0, n, flags;
+    };
+    std::vector<std::vector<Src>> per((size_t)tl.T);
+    if (conv == 0) {
+      auto add = [&](const std::vector<int64_t> &BI, const std::vector<int64_t> &BJ, bool isfar) {
+        for (size_t b = 0; b < BI.size(); b++) {
+          const int64_t I = BI[b], J = BJ[b];
+          const int32_t fl = isfar ? BF_SYMM : 0;
+          per[(size_t)I].push_back({(int32_t)tl.start(J), (int32_t)tl.size(J), fl});
+          if (use_symmetry && J != I)
+            per[(size_t)J].push_back({(int32_t)tl.start(I), (int32_t)tl.size(I), fl | BF_NEG});
+        }
+      };
+      add(tl.closeI, tl.closeJ, false);
+      add(tl.farI, tl.farJ, true);
+    }
+    std::vector<RowBlock> rb;
+    std::vector<NearTask> tk;
+    int64_t q = 0;
+    while (q < nt) {
+      const int64_t lf = order[(size_t)q].first;
+      int64_t q1 = q;
+      while (q1 < nt && order[(size_t)q1].first == lf) q1++;
+      const int32_t m = (int32_t)(q1 - q);
+      std::vector<Src> srcs;
+      if (conv == 0) {
+        for (int64_t nd = lf; nd >= 0; nd = parent[(size_t)nd])
+          for (const Src &s : per[(size_t)nd]) srcs.push_back(s);
+      } else {
+        srcs.push_back({0, (int32_t)n, BF_SYMM});
+      }
+      for (const Src &s : srcs) {
+        const int32_t b = (int32_t)rb.size();
+        rb.push_back(RowBlock{(int32_t)q, m, s.j0, s.n, s.flags | BF_EXPLICIT_I, 0, 0, 0});
+        for (int32_t o = 0; o < m; o += 4) tk.push_back(NearTask{b, o});
+      }
+      q = q1;
+    }
+    upload(chk_blocks, rb.data(), rb.size() * sizeof(RowBlock));
+    upload(chk_tasks, tk.data(), tk.size() * sizeof(NearTask));
+    upload(chk_tlist, tlist.data(), tlist.size() * 4);
+    chk_u.reserve(24 * (size_t)std::max<int64_t>(nt, 1));
+    HIPCHK(hipMemsetAsync(chk_u.p, 0, 24 * (size_t)nt, stream));
+    near_field(stream, chk_blocks, chk_tasks, (int64_t)tk.size(), PREC_FP64, chk_u.as<double>(),
+               chk_tlist.as<int32_t>());
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<double> tmp(3 * (size_t)nt);
+    HIPCHK(hipMemcpy(tmp.data(), chk_u.p, 24 * (size_t)nt, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nt; k++) {
+      const int64_t qq = order[(size_t)k].second;
+      for (int d = 0; d < 3; d++) out[3 * qq + d] = tmp[3 * (size_t)k + d];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static void export_lists(const TreeLists &t, uint64_t *tree5, uint64_t *perm, float *aux6,
+                         int64_t *close_ij, int64_t *far_ij, int64_t *leaves);
+
+extern "C" {
+
+const char *hh_last_error(void) { return g_last_error.c_str(); }
+
+int hh_init(void) {
+  return guarded([&]() { ensure_device(); });
+}
+
+int hh_finalize(void) {
+  return guarded([&]() {
+    if (g_device >= 0) HIPCHK(hipDeviceSynchronize());
+  });
+}
+
+int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **out) {
+  return guarded([&]() {
+    if (!out) throw HHError(HH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n <= 0 || !coord) throw HHError(HH_ERR_ARG, "need n > 0 coordinates");
+    if (n > (int64_t)INT32_MAX / 4) throw HHError(HH_ERR_ARG, "too many particles");
+    if (block_size <= 0) throw HHError(HH_ERR_ARG, "block size must be positive");
+    ensure_device();
+    hh_model *m = new hh_model();
+    m->n = n;
+    m->leaf = block_size;
+    m->x.assign(coord, coord + 3 * n);
+    if (hipStreamCreate(&m->stream) != hipSuccess) {
+      delete m;
+      throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
+      m->far_pool_bytes = (size_t)(std::atof(gb) * (double)(1ull << 30));
+    *out = m;
+  });
+}
+
+void hh_model_destroy(hh_model *m) { delete m; }
+
+int hh_model_load_two_body(hh_model *m, const float *weights, int64_t count) {
+  return guarded([&]() {
+    if (!m || !weights) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (count != 71177)
+      throw HHError(HH_ERR_ARG, "expected 71177 weights (3-128-512-9 MLP), got " + std::to_string(count));
+    m->load_weights(weights);
+  });
+}
+
+#define HH_SETTER(name, type, field, check)                                     \
+  int hh_model_set_##name(hh_model *m, type v) {                               \
+    return guarded([&]() {                                                      \
+      if (!m) throw HHError(HH_ERR_ARG, "NULL model");                          \
+      check;                                                                    \
+      m->field = v;                                                             \
+    });                                                                         \
+  }
+
+HH_SETTER(epsilon, double, epsilon, if (!(v >= 0)) throw HHError(HH_ERR_ARG, "epsilon < 0"))
+HH_SETTER(eta, double, eta, (void)0)
+HH_SETTER(max_iter, int, max_iter,
+          if (v < 1 || v > 128) throw HHError(HH_ERR_ARG, "max_iter must be in [1, 128]"))
+HH_SETTER(mat_pool_size_factor, int, pool_factor, (void)0)
+HH_SETTER(max_far_dot_work_node_size, int, far_work, (void)0)
+HH_SETTER(max_far_field_distance, double, max_far_dist, (void)0)
+
+int hh_model_set_max_relative_coord(hh_model *m, int64_t v) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (v < 1) throw HHError(HH_ERR_ARG, "max_relative_coord must be positive");
+    m->max_rel_coord = v;
+  });
+}
+
+int hh_model_set_post_check_flag(hh_model *m, int v) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    m->post_check = v != 0;
+  });
+}
+
+int hh_model_set_use_symmetry_flag(hh_model *m, int v) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (m->use_symmetry != (v != 0)) m->built = false;  // lists depend on it
+    m->use_symmetry = v != 0;
+  });
+}
+
+int hh_model_set_precision(hh_model *m, int p) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (p != HH_PREC_FP32 && p != HH_PREC_BF16 && p != HH_PREC_FP16)
+      throw HHError(HH_ERR_ARG, "unknown precision");
+    m->prec = p;
+  });
+}
+
+int hh_model_update_coord(hh_model *m, const float *coord, int64_t n) {
+  return guarded([&]() {
+    if (!m || !coord) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "particle count differs from the model's");
+    m->x.assign(coord, coord + 3 * n);
+    m->update();
+  });
+}
+
+int hh_model_dot(hh_model *m, float *u, const float *f, int64_t n) {
+  return guarded([&]() {
+    if (!m || !u || !f) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    m->dot_host(u, f, false);
+  });
+}
+
+int hh_model_dense_dot(hh_model *m, float *u, const float *f, int64_t n) {
+  return guarded([&]() {
+    if (!m || !u || !f) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    m->dot_host(u, f, true);
+  });
+}
+
+int hh_model_dot_device(hh_model *m, float *u_dev, const float *f_dev, int64_t n, void *stream) {
+  return guarded([&]() {
+    if (!m || !u_dev || !f_dev) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    m->dot_device(u_dev, f_dev, stream ? (hipStream_t)stream : m->stream);
+  });
+}
+
+int hh_model_tree_counts(hh_model *m, int64_t counts[4]) {
+  return guarded([&]() {
+    if (!m || !counts) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (!m->built) throw HHError(HH_ERR_STATE, "update_coord has not run");
+    counts[0] = m->tl.T;
+    counts[1] = (int64_t)m->tl.closeI.size();
+    counts[2] = (int64_t)m->tl.farI.size();
+    counts[3] = (int64_t)m->tl.leaves.size();
+  });
+}
+
+int hh_model_tree_export(hh_model *m, uint64_t *tree5, uint64_t *perm, float *aux6,
+                         int64_t *close_ij, int64_t *far_ij, int64_t *leaves) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (!m->built) throw HHError(HH_ERR_STATE, "update_coord has not run");
+    export_lists(m->tl, tree5, perm, aux6, close_ij, far_ij, leaves);
+  });
+}
+
+int hh_model_far_ranks(hh_model *m, int32_t *ranks) {
+  return guarded([&]() {
+    if (!m || !ranks) throw HHError(HH_ERR_ARG, "NULL argument");
+    std::copy(m->far_rank.begin(), m->far_rank.end(), ranks);
+  });
+}
+
+int hh_model_check_fp64(hh_model *m, const float *f, const int64_t *targets, int64_t nt,
+                        int convention, double *u_out) {
+  return guarded([&]() {
+    if (!m || !f || !targets || !u_out) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (nt <= 0) throw HHError(HH_ERR_ARG, "need at least one target");
+    if (convention != 0 && convention != 1) throw HHError(HH_ERR_ARG, "convention must be 0 or 1");
+    m->check_fp64(f, targets, nt, convention, u_out);
+  });
+}
+
+int hh_model_eval_pair(hh_model *m, const float *r, int64_t P, float *out9, int prec) {
+  return guarded([&]() {
+    if (!m || !r || !out9) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (P <= 0) throw HHError(HH_ERR_ARG, "need P > 0");
+    if (prec < 0 || prec > 3) throw HHError(HH_ERR_ARG, "unknown precision");
+    if (!m->has_weights) throw HHError(HH_ERR_STATE, "two-body model not loaded");
+    DBuf dr, dout;
+    dr.reserve(12 * (size_t)P);
+    dout.reserve(36 * (size_t)P);
+    try {
+      HIPCHK(hipMemcpy(dr.p, r, 12 * (size_t)P, hipMemcpyHostToDevice));
+      HIPCHK(launch_pair(prec, dr.as<float>(), P, dout.as<float>(), m->devw_f(prec == PREC_FP64),
+                         m->devw_h(prec), g_cus, m->stream));
+      HIPCHK(hipStreamSynchronize(m->stream));
+      HIPCHK(hipMemcpy(out9, dout.p, 36 * (size_t)P, hipMemcpyDeviceToHost));
+    } catch (...) {
+      dr.release();
+      dout.release();
+      throw;
+    }
+    dr.release();
+    dout.release();
+  });
+}
+
+struct hh_tree {
+  TreeLists tl;
+};
+
+int hh_tree_build(const float *x, int64_t n, int block_size, int use_symmetry,
+                  int64_t max_relative_coord, hh_tree **out) {
+  return guarded([&]() {
+    if (!x || !out) throw HHError(HH_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (n <= 0 || block_size <= 0 || max_relative_coord <= 0)
+      throw HHError(HH_ERR_ARG, "need n, block_size, max_relative_coord > 0");
+    hh_tree *t = new hh_tree();
+    build_tree_lists(x, n, block_size, use_symmetry != 0, max_relative_coord, 0, t->tl);
+    *out = t;
+  });
+}
+
+int hh_tree_counts(hh_tree *t, int64_t counts[4]) {
+  return guarded([&]() {
+    if (!t || !counts) throw HHError(HH_ERR_ARG, "NULL argument");
+    counts[0] = t->tl.T;
+    counts[1] = (int64_t)t->tl.closeI.size();
+    counts[2] = (int64_t)t->tl.farI.size();
+    counts[3] = (int64_t)t->tl.leaves.size();
+  });
+}
+
+static void export_lists(const TreeLists &t, uint64_t *tree5, uint64_t *perm, float *aux6,
+                         int64_t *close_ij, int64_t *far_ij, int64_t *leaves) {
+  if (tree5) std::memcpy(tree5, t.rows.data(), t.rows.size() * 8);
+  if (perm)
+    for (int64_t i = 0; i < t.n; i++) perm[i] = (uint64_t)t.perm[(size_t)i];
+  if (aux6) std::memcpy(aux6, t.aux.data(), t.aux.size() * 4);
+  if (close_ij)
+    for (size_t b = 0; b < t.closeI.size(); b++) close_ij[2 * b] = t.closeI[b], close_ij[2 * b + 1] = t.closeJ[b];
+  if (far_ij)
+    for (size_t b = 0; b < t.farI.size(); b++) far_ij[2 * b] = t.farI[b], far_ij[2 * b + 1] = t.farJ[b];
+  if (leaves) std::memcpy(leaves, t.leaves.data(), t.leaves.size() * 8);
+}
+
+int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int64_t *close_ij,
+                   int64_t *far_ij, int64_t *leaves) {
+  return guarded([&]() {
+    if (!t) throw HHError(HH_ERR_ARG, "NULL tree");
+    export_lists(t->tl, tree5, perm, aux6, close_ij, far_ij, leaves);
+  });
+}
+
+void hh_tree_free(hh_tree *t) { delete t; }
+
+int hh_model_last_timings(hh_model *m, double t[3]) {
+  return guarded([&]() {
+    if (!m || !t) throw HHError(HH_ERR_ARG, "NULL argument");
+    t[0] = m->t_close;
+    t[1] = m->t_far;
+    t[2] = m->t_total;
+  });
+}
+
+}  // extern "C"

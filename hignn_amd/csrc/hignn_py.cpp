@@ -1,0 +1,453 @@
+// pybind11 module `hignn`: the reference's Python surface
+// (interface/HignnInterface.cpp:9-92) on top of the C ABI of libhh.
+//
+// HignnModel keeps the reference's method names, argument order and types.
+// dot/dense_dot write into the caller's float32 (N,3) array in place
+// (src/HignnModel.cpp:570, 618-624); a wrongly typed or shaped output array
+// raises instead of being silently copied.  ExplicitEuler and ExplicitRk4
+// restate include/TimeIntegrator.hpp:431-860 (see the notes on each).
+
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hh.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(int rc) {
+  if (rc != HH_OK) throw std::runtime_error(std::string("hignn: ") + hh_last_error());
+}
+
+// (N,3) float32 copy of any array-like (honours strides, e.g. x[:, 0:3]).
+std::vector<float> rows3(const py::array_t<float, py::array::forcecast> &a, const char *what) {
+  if (a.ndim() != 2 || a.shape(1) < 3)
+    throw std::invalid_argument(std::string(what) + " must be a 2-D array with 3 columns");
+  auto v = a.unchecked<2>();
+  std::vector<float> out(3 * (size_t)a.shape(0));
+  for (py::ssize_t i = 0; i < a.shape(0); i++)
+    for (int d = 0; d < 3; d++) out[3 * (size_t)i + d] = v(i, d);
+  return out;
+}
+
+void write_rows3(py::array &u, const std::vector<float> &src, py::ssize_t n) {
+  if (!py::isinstance<py::array_t<float>>(u) || !u.dtype().is(py::dtype::of<float>()))
+    throw py::type_error("u must be a float32 numpy array (it is written in place)");
+  if (u.ndim() != 2 || u.shape(0) != n || u.shape(1) < 3)
+    throw std::invalid_argument("u must have shape (N, 3)");
+  if (!u.writeable()) throw std::invalid_argument("u is read-only");
+  auto a = py::array_t<float>(u);
+  auto v = a.mutable_unchecked<2>();
+  for (py::ssize_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) v(i, d) = src[3 * (size_t)i + d];
+}
+
+class Model {
+ public:
+  Model(py::array_t<float, py::array::forcecast> coord, int block_size) {
+    auto x = rows3(coord, "coord");
+    n_ = coord.shape(0);
+    check(hh_model_create(x.data(), n_, block_size, &m_));
+  }
+  ~Model() { hh_model_destroy(m_); }
+
+  // load_two_body_model(path): path(.pkl|.npy), or "xavier:<seed>" for the
+  // seeded initialisation of the same layer shapes.
+  void load_two_body_model(const std::string &path) {
+    py::object W = py::module_::import("hignn_amd.weights");
+    py::array_t<float, py::array::forcecast | py::array::c_style> w;
+    if (path.rfind("xavier:", 0) == 0)
+      w = W.attr("xavier")(std::stoi(path.substr(7)));
+    else
+      w = W.attr("load")(path);
+    load_two_body_weights(w);
+  }
+  void load_two_body_weights(py::array_t<float, py::array::forcecast | py::array::c_style> w) {
+    check(hh_model_load_two_body(m_, w.data(), (int64_t)w.size()));
+  }
+  void load_three_body_model(const std::string &) {}  // the reference's is a no-op too
+
+  void set_epsilon(double v) { check(hh_model_set_epsilon(m_, v)); }
+  void set_eta(double v) { check(hh_model_set_eta(m_, v)); }
+  void set_max_iter(int v) { check(hh_model_set_max_iter(m_, v)); }
+  void set_mat_pool_size_factor(int v) { check(hh_model_set_mat_pool_size_factor(m_, v)); }
+  void set_max_far_dot_work_node_size(int v) { check(hh_model_set_max_far_dot_work_node_size(m_, v)); }
+  void set_max_relative_coord(size_t v) { check(hh_model_set_max_relative_coord(m_, (int64_t)v)); }
+  void set_post_check_flag(bool v) { check(hh_model_set_post_check_flag(m_, v)); }
+  void set_use_symmetry_flag(bool v) { check(hh_model_set_use_symmetry_flag(m_, v)); }
+  void set_max_far_field_distance(double v) { check(hh_model_set_max_far_field_distance(m_, v)); }
+
+  void set_precision(const std::string &p) {
+    int v;
+    if (p == "fp32") v = HH_PREC_FP32;
+    else if (p == "bf16") v = HH_PREC_BF16;
+    else if (p == "fp16") v = HH_PREC_FP16;
+    else throw std::invalid_argument("precision must be fp32, bf16 or fp16");
+    check(hh_model_set_precision(m_, v));
+  }
+
+  void update_coord(py::array_t<float, py::array::forcecast> coord) {
+    auto x = rows3(coord, "coord");
+    if (coord.shape(0) != n_) throw std::invalid_argument("coordinate count differs from the model's");
+    check(hh_model_update_coord(m_, x.data(), n_));
+  }
+
+  void dot(py::array &u, py::array_t<float, py::array::forcecast> f) { apply(u, f, false); }
+  void dense_dot(py::array &u, py::array_t<float, py::array::forcecast> f) { apply(u, f, true); }
+
+  // u, f are device pointers (e.g. torch tensor data_ptr()); stream 0 = own.
+  void dot_device(uintptr_t u, uintptr_t f, uintptr_t stream) {
+    py::gil_scoped_release nogil;
+    check(hh_model_dot_device(m_, (float *)u, (const float *)f, n_, (void *)stream));
+  }
+
+  py::dict tree_export() {
+    int64_t c[4];
+    check(hh_model_tree_counts(m_, c));
+    py::array_t<uint64_t> tree({(py::ssize_t)c[0], (py::ssize_t)5});
+    py::array_t<uint64_t> perm(n_);
+    py::array_t<float> aux({(py::ssize_t)c[0], (py::ssize_t)6});
+    py::array_t<int64_t> close({(py::ssize_t)c[1], (py::ssize_t)2});
+    py::array_t<int64_t> far({(py::ssize_t)c[2], (py::ssize_t)2});
+    py::array_t<int64_t> leaves(c[3]);
+    check(hh_model_tree_export(m_, tree.mutable_data(), perm.mutable_data(), aux.mutable_data(),
+                               close.mutable_data(), far.mutable_data(), leaves.mutable_data()));
+    py::dict d;
+    d["tree"] = tree;
+    d["perm"] = perm;
+    d["aux"] = aux;
+    d["close"] = close;
+    d["far"] = far;
+    d["leaves"] = leaves;
+    return d;
+  }
+
+  py::array_t<int32_t> far_ranks() {
+    int64_t c[4];
+    check(hh_model_tree_counts(m_, c));
+    py::array_t<int32_t> r(c[2]);
+    check(hh_model_far_ranks(m_, r.mutable_data()));
+    return r;
+  }
+
+  py::array_t<double> check_fp64(py::array_t<float, py::array::forcecast> f,
+                                 py::array_t<int64_t, py::array::forcecast> targets, int convention) {
+    auto fv = rows3(f, "f");
+    py::array_t<double> out({(py::ssize_t)targets.size(), (py::ssize_t)3});
+    check(hh_model_check_fp64(m_, fv.data(), targets.data(), (int64_t)targets.size(), convention,
+                              out.mutable_data()));
+    return out;
+  }
+
+  py::tuple last_timings() {
+    double t[3];
+    check(hh_model_last_timings(m_, t));
+    return py::make_tuple(t[0], t[1], t[2]);
+  }
+
+ private:
+  void apply(py::array &u, py::array_t<float, py::array::forcecast> f, bool dense) {
+    auto fv = rows3(f, "f");
+    if (f.shape(0) != n_) throw std::invalid_argument("f has the wrong number of rows");
+    std::vector<float> uv(3 * (size_t)n_);
+    {
+      py::gil_scoped_release nogil;
+      check(dense ? hh_model_dense_dot(m_, uv.data(), fv.data(), n_)
+                  : hh_model_dot(m_, uv.data(), fv.data(), n_));
+    }
+    write_rows3(u, uv, n_);
+  }
+
+  hh_model *m_ = nullptr;
+  py::ssize_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Time integrators (include/TimeIntegrator.hpp).  Positions are float
+// triples updated with float arithmetic in the reference's order.  The
+// reference never sets angular velocities (TimeIntegrator.hpp:137 is
+// commented out), so orientations do not evolve: the Euler angles passed to
+// the velocity callback stay those given to initialize() (the reference's
+// zero-rate quaternion update yields NaN there, Quaternion.hpp:62-70).
+// No VTK writer: without set_output_func no output is written.
+// ---------------------------------------------------------------------------
+struct V3 {
+  float v[3] = {0, 0, 0};
+};
+
+class Integrator {
+ public:
+  virtual ~Integrator() = default;
+  void set_time_step(float dt) { dt_ = dt; }
+  void set_final_time(float t) { tf_ = t; }
+  virtual void set_num_rigid_body(size_t n) {
+    n_ = n;
+    pos_.assign(n, V3());
+    pos0_.assign(n, V3());
+    vel_.assign(n, V3());
+    off_.assign(n, V3());
+    ang_.assign(3 * n, 0.0f);
+  }
+  void set_velocity_func(std::function<py::array_t<float>(float, py::array_t<float>)> f) { vf_ = f; }
+  void set_output_func(std::function<void(py::array_t<float>, py::array_t<float>)> f) {
+    of_ = f;
+    custom_out_ = true;
+  }
+  void set_x_lim(py::list l) { set_lim(0, l); }
+  void set_y_lim(py::list l) { set_lim(1, l); }
+  void set_z_lim(py::list l) { set_lim(2, l); }
+  void set_output_step(int s) { out_step_ = s; }
+  void initialize(py::array_t<float, py::array::forcecast> init) {
+    if (init.ndim() != 2) throw std::runtime_error("Number of dimensions of input positions must be two");
+    if ((size_t)init.shape(0) != n_) throw std::runtime_error("Inconsistent number of rigid bodies");
+    auto d = init.unchecked<2>();
+    for (size_t i = 0; i < n_; i++) {
+      for (int j = 0; j < 3; j++) pos0_[i].v[j] = pos_[i].v[j] = d(i, j);
+      for (int j = 0; j < 3; j++) ang_[3 * i + j] = init.shape(1) >= 6 ? d(i, 3 + j) : 0.0f;
+    }
+  }
+  int func_count() const { return count_; }
+
+ protected:
+  void set_lim(int d, py::list l) {
+    lim_[0][d] = py::cast<float>(l[0]);
+    lim_[1][d] = py::cast<float>(l[1]);
+    periodic_ = true;
+  }
+  // TimeIntegrator::VelocityUpdate (TimeIntegrator.hpp:92-141)
+  void velocity(float t, std::vector<V3> &out) {
+    py::array_t<float> in({(py::ssize_t)n_, (py::ssize_t)6});
+    auto a = in.mutable_unchecked<2>();
+    for (size_t i = 0; i < n_; i++) {
+      for (int j = 0; j < 3; j++) {
+        if (periodic_) {
+          const float x = pos_[i].v[j];
+          off_[i].v[j] = x > lim_[1][j] ? lim_[0][j] - lim_[1][j]
+                                        : (x < lim_[0][j] ? lim_[1][j] - lim_[0][j] : 0.0f);
+          a(i, j) = x + off_[i].v[j];
+        } else {
+          a(i, j) = pos_[i].v[j];
+        }
+        a(i, 3 + j) = ang_[3 * i + j];
+      }
+    }
+    py::array_t<float, py::array::forcecast> r = vf_(t, in);
+    if (r.ndim() != 2 || (size_t)r.shape(0) != n_ || r.shape(1) < 3)
+      throw std::runtime_error("velocity callback must return an (N,3) array");
+    auto rv = r.unchecked<2>();
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) out[i].v[j] = rv(i, j);
+  }
+  void output() {
+    if (!custom_out_) return;
+    py::array_t<float> p({(py::ssize_t)n_, (py::ssize_t)3}), v({(py::ssize_t)n_, (py::ssize_t)3});
+    auto pa = p.mutable_unchecked<2>();
+    auto va = v.mutable_unchecked<2>();
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) pa(i, j) = pos_[i].v[j], va(i, j) = vel_[i].v[j];
+    of_(p, v);
+  }
+  void wrap_offsets() {
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) {
+        const float x = pos_[i].v[j];
+        off_[i].v[j] = x > lim_[1][j] ? lim_[0][j] - lim_[1][j]
+                                      : (x < lim_[0][j] ? lim_[1][j] - lim_[0][j] : 0.0f);
+      }
+  }
+
+  size_t n_ = 0;
+  double dt_ = 1.0, tf_ = 10.0;
+  int out_step_ = 1, count_ = 0;
+  bool periodic_ = false, custom_out_ = false;
+  float lim_[2][3] = {{-1, -1, -1}, {1, 1, 1}};
+  std::vector<V3> pos_, pos0_, vel_, off_;
+  std::vector<float> ang_;
+  std::function<py::array_t<float>(float, py::array_t<float>)> vf_;
+  std::function<void(py::array_t<float>, py::array_t<float>)> of_;
+};
+
+// ExplicitEuler::Run (TimeIntegrator.hpp:431-488).
+class Euler : public Integrator {
+ public:
+  void run() {
+    count_ = 0;
+    double t = 0.0;
+    const double h0 = dt_;
+    while (t < tf_ - 1e-5) {
+      const double dt = std::min(h0, tf_ - t);
+      velocity((float)(t + dt), vel_);
+      const float fdt = (float)dt;
+      for (size_t i = 0; i < n_; i++)
+        for (int j = 0; j < 3; j++) pos_[i].v[j] = pos_[i].v[j] + fdt * vel_[i].v[j];
+      if (periodic_) wrap_offsets();
+      t += dt;
+      if (count_ % out_step_ == 0) output();
+      if (periodic_)
+        for (size_t i = 0; i < n_; i++)
+          for (int j = 0; j < 3; j++) pos_[i].v[j] = pos_[i].v[j] + off_[i].v[j];
+      count_++;
+    }
+  }
+};
+
+// ExplicitRk4::Run (TimeIntegrator.hpp:602-860): Dormand-Prince 5(4) with
+// FSAL.  K1 starts at zero (the first call's velocity is not stored in K1),
+// a rejected step shrinks dt by max(0.8 (err/thr)^-0.2, 0.1) (>= 1e-6), and
+// the step size is reset to the nominal one at the start of every step.
+class Rk4 : public Integrator {
+ public:
+  void set_num_rigid_body(size_t n) override {
+    Integrator::set_num_rigid_body(n);
+    for (auto &k : K_) k.assign(n, V3());
+  }
+  void set_threshold(double t) { thr_ = t; }
+  void run() {
+    const float a21 = 1.0f / 5.0f;
+    const float a31 = 3.0f / 40.0f, a32 = 9.0f / 40.0f;
+    const float a41 = 44.0f / 45.0f, a42 = -56.0f / 15.0f, a43 = 32.0f / 9.0f;
+    const float a51 = 19372.0f / 6561.0f, a52 = -25360.0f / 2187.0f, a53 = 64448.0f / 6561.0f,
+                a54 = -212.0f / 729.0f;
+    const float a61 = 9017.0f / 3168.0f, a62 = -355.0f / 33.0f, a63 = 46732.0f / 5247.0f,
+                a64 = 49.0f / 176.0f, a65 = -5103.0f / 18656.0f;
+    const float b1 = 35.0f / 384.0f, b3 = 500.0f / 1113.0f, b4 = 125.0f / 192.0f,
+                b5 = -2187.0f / 6784.0f, b6 = 11.0f / 84.0f;
+    const float dc1 = b1 - 5179.0f / 57600.0f, dc3 = b3 - 7571.0f / 16695.0f,
+                dc4 = b4 - 393.0f / 640.0f, dc5 = b5 - (-92097.0f / 339200.0f),
+                dc6 = b6 - 187.0f / 2100.0f, dc7 = -1.0f / 40.0f;
+    const float c[7] = {0.0f, 1.0f / 5.0f, 3.0f / 10.0f, 4.0f / 5.0f, 8.0f / 9.0f, 1.0f, 1.0f};
+    const float A[6][5] = {{a21, 0, 0, 0, 0},       {a31, a32, 0, 0, 0},
+                           {a41, a42, a43, 0, 0},   {a51, a52, a53, a54, 0},
+                           {a61, a62, a63, a64, a65}, {b1, 0, b3, b4, b5}};
+    count_ = 0;
+    double t = 0.0;
+    const double h0 = dt_;
+    velocity((float)t, vel_);
+    while (t < tf_ - 1e-5) {
+      double dt = std::min(h0, tf_ - t);
+      for (size_t i = 0; i < n_; i++)
+        for (int j = 0; j < 3; j++)
+          pos0_[i].v[j] = periodic_ ? pos_[i].v[j] + off_[i].v[j] : pos_[i].v[j];
+      double err = 100.0;
+      while (err > thr_) {
+        for (int s = 1; s < 7; s++) {
+          // stage positions: x0 + sum_k K_k (a_sk dt), left to right
+          for (size_t i = 0; i < n_; i++)
+            for (int j = 0; j < 3; j++) {
+              float x = pos0_[i].v[j];
+              for (int k = 0; k < s && k < 5; k++) {
+                if (s == 6 && k == 1) continue;  // b2 = 0
+                x = x + (float)(A[s - 1][k] * dt) * K_[k][i].v[j];
+              }
+              if (s == 6) x = x + (float)(b6 * dt) * K_[5][i].v[j];
+              pos_[i].v[j] = x;
+            }
+          velocity((float)(t + c[s] * dt), vel_);
+          K_[s] = vel_;
+        }
+        err = 0.0;
+        const float fdt = (float)dt;
+        for (size_t i = 0; i < n_; i++) {
+          float e2 = 0.0f;
+          for (int j = 0; j < 3; j++) {
+            float v = K_[0][i].v[j] * dc1 + K_[2][i].v[j] * dc3;
+            v = v + K_[3][i].v[j] * dc4;
+            v = v + K_[4][i].v[j] * dc5;
+            v = v + K_[5][i].v[j] * dc6;
+            v = v + K_[6][i].v[j] * dc7;
+            v = fdt * v;
+            e2 += v * v;
+          }
+          err += e2;
+        }
+        err = std::sqrt(err);
+        if (err > thr_) {
+          dt = dt * std::max(0.8 * std::pow(err / thr_, -0.2), 0.1);
+          dt = std::max(dt, 1e-6);
+        }
+      }
+      t += dt;
+      if (count_ % out_step_ == 0) output();
+      K_[0] = K_[6];
+      count_++;
+    }
+  }
+
+ private:
+  double thr_ = 1e-3;
+  std::vector<V3> K_[7];
+};
+
+}  // namespace
+
+PYBIND11_MODULE(hignn, m) {
+  m.doc() = "H-HIGNN mobility matvec on MI355X (gfx950)";
+  m.def("Init", []() { check(hh_init()); }, "Select the HIP device");
+  m.def("Finalize", []() { check(hh_finalize()); }, "Synchronise the device");
+
+  py::class_<Model>(m, "HignnModel")
+      .def(py::init<py::array_t<float, py::array::forcecast>, int>())
+      .def("load_two_body_model", &Model::load_two_body_model)
+      .def("load_two_body_weights", &Model::load_two_body_weights)
+      .def("load_three_body_model", &Model::load_three_body_model)
+      .def("set_epsilon", &Model::set_epsilon)
+      .def("set_eta", &Model::set_eta)
+      .def("set_max_iter", &Model::set_max_iter)
+      .def("set_mat_pool_size_factor", &Model::set_mat_pool_size_factor)
+      .def("set_max_far_dot_work_node_size", &Model::set_max_far_dot_work_node_size)
+      .def("set_max_relative_coord", &Model::set_max_relative_coord)
+      .def("set_post_check_flag", &Model::set_post_check_flag)
+      .def("set_use_symmetry_flag", &Model::set_use_symmetry_flag)
+      .def("set_max_far_field_distance", &Model::set_max_far_field_distance)
+      .def("set_precision", &Model::set_precision)
+      .def("update_coord", &Model::update_coord)
+      .def("dot", &Model::dot)
+      .def("dense_dot", &Model::dense_dot)
+      .def("dot_device", &Model::dot_device, py::arg("u_ptr"), py::arg("f_ptr"),
+           py::arg("stream") = 0)
+      .def("tree_export", &Model::tree_export)
+      .def("far_ranks", &Model::far_ranks)
+      .def("check_fp64", &Model::check_fp64, py::arg("f"), py::arg("targets"),
+           py::arg("convention") = 0)
+      .def("last_timings", &Model::last_timings);
+
+  py::class_<Euler>(m, "ExplicitEuler")
+      .def(py::init())
+      .def("set_time_step", &Euler::set_time_step)
+      .def("set_final_time", &Euler::set_final_time)
+      .def("set_num_rigid_body", &Euler::set_num_rigid_body)
+      .def("set_velocity_func", &Euler::set_velocity_func)
+      .def("set_output_func", &Euler::set_output_func)
+      .def("set_x_lim", &Euler::set_x_lim)
+      .def("set_y_lim", &Euler::set_y_lim)
+      .def("set_z_lim", &Euler::set_z_lim)
+      .def("set_output_step", &Euler::set_output_step)
+      .def("initialize", &Euler::initialize)
+      .def("run", &Euler::run)
+      .def("func_count", &Euler::func_count);
+
+  py::class_<Rk4>(m, "ExplicitRk4")
+      .def(py::init())
+      .def("set_time_step", &Rk4::set_time_step)
+      .def("set_threshold", &Rk4::set_threshold)
+      .def("set_final_time", &Rk4::set_final_time)
+      .def("set_num_rigid_body", &Rk4::set_num_rigid_body)
+      .def("set_velocity_func", &Rk4::set_velocity_func)
+      .def("set_output_func", &Rk4::set_output_func)
+      .def("set_x_lim", &Rk4::set_x_lim)
+      .def("set_y_lim", &Rk4::set_y_lim)
+      .def("set_z_lim", &Rk4::set_z_lim)
+      .def("set_output_step", &Rk4::set_output_step)
+      .def("initialize", &Rk4::initialize)
+      .def("run", &Rk4::run)
+      .def("func_count", &Rk4::func_count);
+}

@@ -1,0 +1,125 @@
+/*
+ * hh.h -- C ABI of libhh, the MI355X (gfx950) implementation of the H-HIGNN
+ * mobility matvec u = M(x) f.
+ *
+ * Every entry point returns 0 on success or a negative status; the message
+ * of the last failure on the calling thread is returned by hh_last_error().
+ * Arrays are plain C pointers: coordinates, forces and velocities are
+ * row-major (n,3) float32 in the caller's original particle order.  A model
+ * handle is not thread-safe; one device stream per handle.
+ *
+ * Each declaration names the reference interface it replaces
+ * (jonaspleyer/hignn; pybind11 module `hignn`, interface/HignnInterface.cpp).
+ */
+#ifndef HH_H
+#define HH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HH_OK 0
+#define HH_ERR_ARG (-1)     /* invalid argument or shape */
+#define HH_ERR_STATE (-2)   /* call not valid in the current state */
+#define HH_ERR_DEVICE (-3)  /* HIP runtime error */
+#define HH_ERR_ALLOC (-4)   /* device or host allocation failed */
+
+/* precision modes (hh_model_set_precision) */
+#define HH_PREC_FP32 0 /* plain: FP32 MLP, FP64 ACA */
+#define HH_PREC_BF16 1 /* BF16 MLP operands, FP32 accumulate, FP32 ACA factors */
+#define HH_PREC_FP16 2 /* FP16 MLP operands, FP32 accumulate, FP32 ACA factors */
+
+typedef struct hh_model hh_model;
+
+const char *hh_last_error(void);
+
+/* Init() -- src/HignnModel.cpp:16-42 (MPI + Kokkos init; here: selects the
+ * HIP device and queries its properties). */
+int hh_init(void);
+/* Finalize() -- src/HignnModel.cpp:48-52. */
+int hh_finalize(void);
+
+/* HignnModel(coord, blockSize) -- src/HignnModel.cpp:335-386. */
+int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **out);
+void hh_model_destroy(hh_model *m);
+
+/* load_two_body_model(path) -- src/HignnModel.cpp:402-433.  The host layer
+ * resolves the file; this call takes the 71,177 raw float32 weights
+ * W1(128,3) b1(128) W2(512,128) b2(512) W3(9,512) b3(9). */
+int hh_model_load_two_body(hh_model *m, const float *weights, int64_t count);
+
+/* Setters -- src/HignnModel.cpp:773-874 (interface/HignnInterface.cpp:21-30). */
+int hh_model_set_epsilon(hh_model *m, double epsilon);
+int hh_model_set_eta(hh_model *m, double eta); /* stored, not used (as the reference) */
+int hh_model_set_max_iter(hh_model *m, int max_iter);
+int hh_model_set_mat_pool_size_factor(hh_model *m, int factor);
+int hh_model_set_max_far_dot_work_node_size(hh_model *m, int size);
+int hh_model_set_max_relative_coord(hh_model *m, int64_t size);
+int hh_model_set_post_check_flag(hh_model *m, int flag);
+int hh_model_set_use_symmetry_flag(hh_model *m, int flag);
+int hh_model_set_max_far_field_distance(hh_model *m, double distance); /* stored, unused */
+
+/* update_coord(coord) -- src/HignnModel.cpp:744-762: new coordinates, then
+ * tree + close/far lists (Build + CloseFarCheck). */
+int hh_model_update_coord(hh_model *m, const float *coord, int64_t n);
+
+/* dot(u, f) -- src/HignnModel.cpp:545-634: u = M(x) f, host arrays. */
+int hh_model_dot(hh_model *m, float *u, const float *f, int64_t n);
+
+/* dense_dot(u, f) -- src/HignnModel.cpp:660-728 with src/DenseDot.cpp. */
+int hh_model_dense_dot(hh_model *m, float *u, const float *f, int64_t n);
+
+/* Device-resident dot: f_dev and u_dev are device pointers (n,3) float32 in
+ * the original order; stream is a hipStream_t (NULL: the model's stream).
+ * Same arithmetic as hh_model_dot, without host transfers. */
+int hh_model_dot_device(hh_model *m, float *u_dev, const float *f_dev, int64_t n, void *stream);
+
+/* Precision of dot (no counterpart in the reference, which is FP32 only). */
+int hh_model_set_precision(hh_model *m, int prec);
+
+/* Tree and block lists of the last update_coord (the reference keeps them in
+ * mClusterTreePtr, mReorderedMap, mCloseMatI/J, mFarMatI/J, mLeafNodeList,
+ * include/HignnModel.hpp:54-75).  counts: [tree rows, close pairs, far pairs,
+ * leaves].  Any output pointer may be NULL. */
+int hh_model_tree_counts(hh_model *m, int64_t counts[4]);
+int hh_model_tree_export(hh_model *m, uint64_t *tree5, uint64_t *perm, float *aux6,
+                         int64_t *close_ij, int64_t *far_ij, int64_t *leaves);
+
+/* Final ACA rank of every far block in the last dot (far-list order). */
+int hh_model_far_ranks(hh_model *m, int32_t *ranks);
+
+/* PostCheckDot -- src/PostCheckDot.cpp:3-377: FP64 evaluation of u at the
+ * given target particles (original indices).  convention 0: the H-matrix's
+ * own block convention with exact far blocks; 1: ordered-pair dense with
+ * symmetrised blocks (src/DenseDot.cpp:230-246). */
+int hh_model_check_fp64(hh_model *m, const float *f, const int64_t *targets, int64_t nt,
+                        int convention, double *u_out);
+
+/* Pair function on the device: out9[p] = M(r[p]) (row-major 3x3) for P
+ * displacements r (host arrays), i.e. the TorchScript Net.forward of
+ * python/convert.py:11-20 as called at src/CloseDot.cpp:233.  prec:
+ * HH_PREC_FP32/BF16/FP16; 3 = FP64 (the result is rounded to float). */
+int hh_model_eval_pair(hh_model *m, const float *r, int64_t P, float *out9, int prec);
+
+/* Host-only cluster tree and block lists (Build + CloseFarCheck,
+ * src/Build.cpp:17-265, src/CloseFarCheck.cpp:11-465) for coordinates
+ * x (n,3); no device is touched.  Export as hh_model_tree_export. */
+typedef struct hh_tree hh_tree;
+int hh_tree_build(const float *x, int64_t n, int block_size, int use_symmetry,
+                  int64_t max_relative_coord, hh_tree **out);
+int hh_tree_counts(hh_tree *t, int64_t counts[4]);
+int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int64_t *close_ij,
+                   int64_t *far_ij, int64_t *leaves);
+void hh_tree_free(hh_tree *t);
+
+/* Per-phase device times (ms) of the last dot: [close, far, total]. */
+int hh_model_last_timings(hh_model *m, double t[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HH_H */

@@ -1,0 +1,37 @@
+"""libhh loads and exports every symbol that include/hh.h declares; the
+pybind11 module exposes the reference's surface (no device calls here)."""
+import ctypes
+
+from hignn_amd import abi
+
+
+def test_library_exports_header_symbols():
+    lib = ctypes.CDLL(abi.LIB_PATH)
+    missing = [s for s in abi.header_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+    assert len(abi.header_symbols()) >= 25
+
+
+def test_pybind_surface_matches_reference():
+    from hignn_amd import hignn
+    # interface/HignnInterface.cpp:14-64
+    assert callable(hignn.Init) and callable(hignn.Finalize)
+    for meth in ["load_two_body_model", "load_three_body_model", "set_epsilon", "set_eta",
+                 "set_max_iter", "set_mat_pool_size_factor", "set_max_far_dot_work_node_size",
+                 "set_max_relative_coord", "set_post_check_flag", "set_use_symmetry_flag",
+                 "set_max_far_field_distance", "update_coord", "dot", "dense_dot"]:
+        assert hasattr(hignn.HignnModel, meth), meth
+    for cls in (hignn.ExplicitEuler, hignn.ExplicitRk4):
+        for meth in ["set_time_step", "set_final_time", "set_num_rigid_body",
+                     "set_velocity_func", "set_output_func", "set_x_lim", "set_y_lim",
+                     "set_z_lim", "set_output_step", "initialize", "run"]:
+            assert hasattr(cls, meth), (cls, meth)
+    assert hasattr(hignn.ExplicitRk4, "set_threshold")
+
+
+def test_errors_are_reported():
+    lib = abi.lib()
+    h = ctypes.c_void_p()
+    rc = lib.hh_tree_build(None, 10, 64, 1, 500000, ctypes.byref(h))
+    assert rc == -1
+    assert b"NULL" in lib.hh_last_error()

@@ -1,0 +1,128 @@
+"""GPU parity of libhh against the CPU oracle, called through the C ABI.
+
+Bars: FP32 path bit-exact lists and <= 1e-5 relative L2 velocity error
+against the oracle; FP64 check path <= 1e-10 against the oracle's FP64
+evaluation; 16-bit forms <= 1e-3 against the FP64 block-convention matvec.
+"""
+import numpy as np
+import pytest
+
+from hignn_amd import abi, weights, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _disp(n, seed):
+    g = np.random.default_rng(seed)
+    d = g.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.exp(g.uniform(np.log(1e-6), np.log(1e3), size=n))
+    out = (d * r[:, None]).astype(np.float32)
+    out[0] = 0.0
+    out[1] = [3e-6, 0, 0]  # below the 1e-5 identity cut-off
+    return out
+
+
+@pytest.fixture(scope="module")
+def model_small(xavier_w):
+    x = workloads.uniform_cube(1200, 0.05, 0)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50)
+    m.update_coord()
+    om = O.OracleModel(x, 64, epsilon=1e-5, max_iter=50)
+    om.update_coord()
+    f = workloads.gravity(x.shape[0])
+    return dict(x=x, m=m, om=om, f=f)
+
+
+def test_pair_fp32_matches_oracle(xavier_w):
+    x = workloads.uniform_cube(100, 0.05, 0)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    r = _disp(4099, 1)
+    got = m.eval_pair(r, "fp32").astype(np.float64)
+    ref = O.mlp_f64(xavier_w, r.astype(np.float64))
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    err = np.abs(got - ref) / scale
+    print("fp32 pair max rel err", err.max())
+    assert err.max() < 2e-5
+    np.testing.assert_array_equal(got[0], np.eye(3).reshape(-1))
+    np.testing.assert_array_equal(got[1], np.eye(3).reshape(-1))
+
+
+def test_pair_fp64_matches_oracle(xavier_w):
+    x = workloads.uniform_cube(100, 0.05, 0)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    r = _disp(1000, 2)
+    got = m.eval_pair(r, "fp64").astype(np.float64)
+    ref = O.mlp_f64(xavier_w, r.astype(np.float64))
+    # output is rounded to float32
+    err = np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)
+    print("fp64 pair max rel err", err.max())
+    assert err.max() < 2e-7
+
+
+@pytest.mark.parametrize("prec,bound", [("bf16", 3e-2), ("fp16", 4e-3)])
+def test_pair_16bit_error(xavier_w, prec, bound):
+    x = workloads.uniform_cube(100, 0.05, 0)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    r = _disp(4096, 3)
+    got = m.eval_pair(r, prec).astype(np.float64)
+    ref = O.mlp_f64(xavier_w, r.astype(np.float64))
+    err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(prec, "pair rel err: median", np.median(err), "max", err.max())
+    assert err.max() < bound
+
+
+def test_lists_bit_exact(model_small):
+    a = model_small["m"].export()
+    b = model_small["om"].export()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_dot_fp32_parity(model_small, xavier_w):
+    m, om, f = model_small["m"], model_small["om"], model_small["f"]
+    u = m.dot(f)
+    ref = om.dot(xavier_w, f)
+    err = O.rel_l2(u, ref)
+    print("fp32 dot rel err vs oracle", err)
+    assert err <= 1e-5
+    np.testing.assert_array_equal(m.far_ranks(), om.far_ranks())
+
+
+def test_dense_dot_parity(model_small, xavier_w):
+    m, om, f = model_small["m"], model_small["om"], model_small["f"]
+    err = O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f))
+    print("dense dot rel err", err)
+    assert err <= 1e-5
+
+
+def test_check_fp64_matches_oracle(model_small, xavier_w):
+    m, om, f, x = model_small["m"], model_small["om"], model_small["f"], model_small["x"]
+    tg = np.arange(0, x.shape[0], 37)
+    e0 = O.rel_l2(m.check_fp64(f, tg, 0), om.dense64_blocks(xavier_w, f, tg))
+    e1 = O.rel_l2(m.check_fp64(f, tg, 1), O.dense64_ordered(xavier_w, x, f, tg))
+    print("fp64 check vs oracle", e0, e1)
+    assert e0 < 1e-10 and e1 < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_chosen_form_accuracy(model_small, prec):
+    m, f, x = model_small["m"], model_small["f"], model_small["x"]
+    tg = np.arange(0, x.shape[0], 5)
+    ref = m.check_fp64(f, tg, 0)
+    m.set(precision=prec)
+    try:
+        u = m.dot(f)
+    finally:
+        m.set(precision="fp32")
+    u32 = m.dot(f)
+    e = O.rel_l2(u[tg], ref)
+    e32 = O.rel_l2(u32[tg], ref)
+    print(prec, "error vs fp64 blocks", e, "fp32 path", e32)
+    assert e <= 1e-3
