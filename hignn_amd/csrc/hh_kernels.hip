@@ -70,6 +70,18 @@ struct Vec4<double> {
   typedef f64x4 T;
 };
 
+// Row of the 16x16 MFMA output held by lane group g in register r:
+// 4g + r for the f32 (and 16-bit) instructions, 4r + g for
+// v_mfma_f64_16x16x4f64 (its lane 16(i%4)+j, register i/4 holds D[i][j]).
+template <class R>
+__device__ __forceinline__ int out_row(int g, int r) {
+  return 4 * g + r;
+}
+template <>
+__device__ __forceinline__ int out_row<double>(int g, int r) {
+  return 4 * r + g;
+}
+
 // ---------------------------------------------------------------------------
 // FP32 / FP64 MLP on NT tiles of 16 displacements.
 // ---------------------------------------------------------------------------
@@ -95,24 +107,23 @@ __device__ __forceinline__ void mlp_tiles_f(const WF<R> &W, const R (&rx)[NT], c
     const R *w = W.w1b + 4 * (4 * s + g);
     const R w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      R a = b;
-      a += w0 * rx[t];
-      a += w1 * ry[t];
-      a += w2 * rz[t];
-      h1[t][s] = tanh_f(a);
-    }
+    for (int t = 0; t < NT; t++) h1[t][s] = tanh_f(fma(w2, rz[t], fma(w1, ry[t], fma(w0, rx[t], b))));
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) y[t] = V{0, 0, 0, 0};
   for (int mt = 0; mt < 32; mt++) {
-    const R *bb = W.b2 + 16 * mt + 4 * g;
+    const R *bb = W.b2 + 16 * mt;
+    const V b2v = V{bb[out_row<R>(g, 0)], bb[out_row<R>(g, 1)], bb[out_row<R>(g, 2)],
+                    bb[out_row<R>(g, 3)]};
     V acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = V{bb[0], bb[1], bb[2], bb[3]};
     const R *a2 = W.w2f + (size_t)mt * 32 * 64 + lane;
+    {
+      const R a = a2[0];
 #pragma unroll
-    for (int s = 0; s < 32; s++) {
+      for (int t = 0; t < NT; t++) acc[t] = mma(a, h1[t][0], b2v);  // bias enters as C
+    }
+#pragma unroll
+    for (int s = 1; s < 32; s++) {
       const R a = a2[s * 64];
 #pragma unroll
       for (int t = 0; t < NT; t++) acc[t] = mma(a, h1[t][s], acc[t]);
@@ -125,11 +136,10 @@ __device__ __forceinline__ void mlp_tiles_f(const WF<R> &W, const R (&rx)[NT], c
       for (int t = 0; t < NT; t++) y[t] = mma(a, tanh_f(acc[t][r]), y[t]);
     }
   }
-  const R *b3 = W.b3 + 4 * g;
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) y[t][r] += b3[r];
+    for (int r = 0; r < 4; r++) y[t][r] += W.b3[out_row<R>(g, r)];
 }
 
 // ---------------------------------------------------------------------------
@@ -172,13 +182,8 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
       const float *w = W.w1b + 4 * (32 * s + 8 * g + j);
       const float w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        float a = b;
-        a += w0 * rx[t];
-        a += w1 * ry[t];
-        a += w2 * rz[t];
-        h1[t][s][j] = (E)tanh_f(a);
-      }
+      for (int t = 0; t < NT; t++)
+        h1[t][s][j] = (E)tanh_f(__builtin_fmaf(w2, rz[t], __builtin_fmaf(w1, ry[t], __builtin_fmaf(w0, rx[t], b))));
     }
   }
 #pragma unroll
@@ -186,16 +191,22 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
   for (int q = 0; q < 16; q++) {
     const float *bb0 = W.b2 + 32 * q + 4 * g;
     const float *bb1 = bb0 + 16;
+    const f32x4 b0v = f32x4{bb0[0], bb0[1], bb0[2], bb0[3]};
+    const f32x4 b1v = f32x4{bb1[0], bb1[1], bb1[2], bb1[3]};
     f32x4 acc0[NT], acc1[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      acc0[t] = f32x4{bb0[0], bb0[1], bb0[2], bb0[3]};
-      acc1[t] = f32x4{bb1[0], bb1[1], bb1[2], bb1[3]};
-    }
     const H *a0p = lw2 + (size_t)(2 * q) * 4 * 64 + lane;
     const H *a1p = lw2 + (size_t)(2 * q + 1) * 4 * 64 + lane;
+    {
+      const H a0 = a0p[0];
+      const H a1 = a1p[0];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+      for (int t = 0; t < NT; t++) {
+        acc0[t] = mma(a0, h1[t][0], b0v);  // bias enters as C
+        acc1[t] = mma(a1, h1[t][0], b1v);
+      }
+    }
+#pragma unroll
+    for (int s = 1; s < 4; s++) {
       const H a0 = a0p[s * 64];
       const H a1 = a1p[s * 64];
 #pragma unroll
@@ -265,20 +276,20 @@ struct NearSrc {
   double *u;           // (.,3) accumulators
 };
 
-template <class R>
-__device__ __forceinline__ void near_epilogue(const typename Vec4<R>::T (&y)[4], const R (&rx)[4],
-                                              const R (&ry)[4], const R (&rz)[4],
-                                              const bool (&ivalid)[4], bool jvalid,
-                                              const float4 &fj, const float4 (&fi)[4], int flags,
-                                              double (&v)[4][3], double (&w)[3]) {
+template <class R, int NT>
+__device__ __forceinline__ void near_epilogue(const typename Vec4<R>::T (&y)[NT], const R (&rx)[NT],
+                                              const R (&ry)[NT], const R (&rz)[NT],
+                                              const bool (&ivalid)[NT], bool jvalid,
+                                              const float4 &fj, const float4 (&fi)[NT], int flags,
+                                              double (&v)[NT][3], double (&w)[3]) {
   const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-  for (int t = 0; t < 4; t++) {
+  for (int t = 0; t < NT; t++) {
     if (!(ivalid[t] && jvalid)) continue;
     const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const int e = 4 * g + rr;
+      const int e = out_row<R>(g, rr);
       if (e >= 9) continue;
       const int row = e / 3, col = e - 3 * (e / 3);
       const double val = (double)block_entry<R>((R)y[t][rr], e, nr);
@@ -298,8 +309,11 @@ __device__ __forceinline__ void near_epilogue(const typename Vec4<R>::T (&y)[4],
   }
 }
 
-template <class R>
+// NT tiles (targets) per MLP call; a task's four targets are covered in
+// 4/NT passes over the sources (NT = 2 keeps the FP64 variant in registers).
+template <class R, int NT>
 __device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
+  static_assert(4 % NT == 0, "NT must divide 4");
   const int lane = threadIdx.x & 63;
   const int p = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -307,63 +321,70 @@ __device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
   for (int64_t task = wave; task < S.ntasks; task += nwaves) {
     const NearTask tk = S.tasks[task];
     const RowBlock B = S.blocks[tk.block];
-    int ti[4];
-    bool iv[4];
-    float4 xi[4], fi[4];
+    for (int tg = 0; tg < 4; tg += NT) {
+      int ti[NT];
+      bool iv[NT];
+      float4 xi[NT], fi[NT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      iv[t] = tk.ioff + t < B.m;
-      const int li = B.i0 + tk.ioff + (iv[t] ? t : 0);
-      ti[t] = li;
-      const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
-      xi[t] = S.xp[xi_idx];
-      fi[t] = S.fp[xi_idx];
-    }
-    double v[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int j0 = 0; j0 < B.n; j0 += 16) {
-      const bool jv = j0 + p < B.n;
-      const int j = B.j0 + j0 + (jv ? p : 0);
-      const float4 xj = S.xp[j];
-      const float4 fj = S.fp[j];
-      R rx[4], ry[4], rz[4];
+      for (int t = 0; t < NT; t++) {
+        iv[t] = tk.ioff + tg + t < B.m;
+        const int li = B.i0 + tk.ioff + (iv[t] ? tg + t : 0);
+        ti[t] = li;
+        const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
+        xi[t] = S.xp[xi_idx];
+        fi[t] = S.fp[xi_idx];
+      }
+      if (!iv[0]) break;
+      double v[NT][3];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        if (B.flags & BF_NEG) {
-          rx[t] = (R)(xi[t].x - xj.x), ry[t] = (R)(xi[t].y - xj.y), rz[t] = (R)(xi[t].z - xj.z);
-        } else {
-          rx[t] = (R)(xj.x - xi[t].x), ry[t] = (R)(xj.y - xi[t].y), rz[t] = (R)(xj.z - xi[t].z);
+      for (int t = 0; t < NT; t++) v[t][0] = v[t][1] = v[t][2] = 0.0;
+      for (int j0 = 0; j0 < B.n; j0 += 16) {
+        const bool jv = j0 + p < B.n;
+        const int j = B.j0 + j0 + (jv ? p : 0);
+        const float4 xj = S.xp[j];
+        const float4 fj = S.fp[j];
+        R rx[NT], ry[NT], rz[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          // (R)x - (R)y: the float difference for R = float (as the
+          // reference, src/CloseDot.cpp:206), the exact one for R = double
+          if (B.flags & BF_NEG) {
+            rx[t] = (R)xi[t].x - (R)xj.x, ry[t] = (R)xi[t].y - (R)xj.y, rz[t] = (R)xi[t].z - (R)xj.z;
+          } else {
+            rx[t] = (R)xj.x - (R)xi[t].x, ry[t] = (R)xj.y - (R)xi[t].y, rz[t] = (R)xj.z - (R)xi[t].z;
+          }
+        }
+        typename Vec4<R>::T y[NT];
+        mlp_tiles_f<R, NT>(W, rx, ry, rz, y);
+        double w[3] = {0, 0, 0};
+        near_epilogue<R, NT>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
+        if (B.flags & BF_REUSE) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
+          if (lane < 16 && jv)
+#pragma unroll
+            for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
         }
       }
-      typename Vec4<R>::T y[4];
-      mlp_tiles_f<R, 4>(W, rx, ry, rz, y);
-      double w[3] = {0, 0, 0};
-      near_epilogue<R>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
-      if (B.flags & BF_REUSE) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
-        if (lane < 16 && jv)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
-          for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
-      }
+        for (int c = 0; c < 3; c++) {
+          const double s = wave_sum(v[t][c]);
+          if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
+        }
     }
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double s = wave_sum(v[t][c]);
-        if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
-      }
   }
 }
 
 __global__ __launch_bounds__(256) void near_fp32_kernel(NearSrc<float> S, WF<float> W) {
-  near_body_f<float>(S, W);
+  near_body_f<float, 4>(S, W);
 }
 
 // FP64 variant, two tiles at a time is not needed for correctness; the same
 // four-target task shape is used with double registers.
 __global__ __launch_bounds__(256) void near_fp64_kernel(NearSrc<double> S, WF<double> W) {
-  near_body_f<double>(S, W);
+  near_body_f<double, 2>(S, W);
 }
 
 // 16-bit near kernel: persistent workgroups stage W2/W3 fragments in LDS.
@@ -417,7 +438,7 @@ __global__ __launch_bounds__(256) void near_h_kernel(NearSrc<float> S, WH W) {
       f32x4 y[4];
       mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y);
       double w[3] = {0, 0, 0};
-      near_epilogue<float>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
+      near_epilogue<float, 4>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
       if (B.flags & BF_REUSE) {
 #pragma unroll
         for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
@@ -693,6 +714,7 @@ template <class P>
 __global__ __launch_bounds__(256) void aca_row_step_kernel(AcaStepArgs A, P *Cp, P *Qp) {
   __shared__ ArgMax sh_am[8];
   __shared__ double sh_s[8];
+  __shared__ double sh_x[4][72];
   const int slot = blockIdx.x;
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
@@ -768,38 +790,78 @@ __global__ __launch_bounds__(256) void aca_row_step_kernel(AcaStepArgs A, P *Cp,
           2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
     mu2 = A.mu2[b] + nu2;
   }
-  // cross terms with the iterations l < k-1 (src/FarDot.cpp:1041-1078)
-  for (int l = 0; l + 1 < k; l++) {
-    double cd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, qd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const P *Cl = Cp + 9 * (fb.coff + (int64_t)l * fb.m);
-    const P *Ql = Qp + 9 * (fb.qoff + (int64_t)l * fb.n);
+  // cross terms with the iterations l < k-1 (src/FarDot.cpp:1041-1078), four
+  // iterations per pass over the block, all 72 sums reduced together; mu2
+  // is accumulated in the reference's order (l, d1, d2 ascending).
+  constexpr int LC = 4;
+  for (int l0 = 0; l0 + 1 < k; l0 += LC) {
+    const int nl = (k - 1 - l0) < LC ? (k - 1 - l0) : LC;
+    double cd[LC][9], qd[LC][9];
+#pragma unroll
+    for (int c = 0; c < LC; c++)
+#pragma unroll
+      for (int e = 0; e < 9; e++) cd[c][e] = qd[c][e] = 0.0;
     for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
-      double a[9], c[9];
+      double a[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) a[e] = (double)Ck[9 * (int64_t)i + e], c[e] = (double)Cl[9 * (int64_t)i + e];
+      for (int e = 0; e < 9; e++) a[e] = (double)Ck[9 * (int64_t)i + e];
 #pragma unroll
-      for (int d1 = 0; d1 < 3; d1++)
+      for (int c = 0; c < LC; c++) {
+        if (c >= nl) break;
+        const P *Cl = Cp + 9 * (fb.coff + (int64_t)(l0 + c) * fb.m + i);
+        double v[9];
 #pragma unroll
-        for (int d2 = 0; d2 < 3; d2++)
-          cd[3 * d1 + d2] += a[d1] * c[d2] + a[d1 + 3] * c[d2 + 3] + a[d1 + 6] * c[d2 + 6];
+        for (int e = 0; e < 9; e++) v[e] = (double)Cl[e];
+#pragma unroll
+        for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; d2++)
+            cd[c][3 * d1 + d2] += a[d1] * v[d2] + a[d1 + 3] * v[d2 + 3] + a[d1 + 6] * v[d2 + 6];
+      }
     }
     for (int j = threadIdx.x; j < fb.n; j += blockDim.x) {
-      double a[9], c[9];
+      double a[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) a[e] = (double)Qk[9 * (int64_t)j + e], c[e] = (double)Ql[9 * (int64_t)j + e];
+      for (int e = 0; e < 9; e++) a[e] = (double)Qk[9 * (int64_t)j + e];
 #pragma unroll
-      for (int d1 = 0; d1 < 3; d1++)
+      for (int c = 0; c < LC; c++) {
+        if (c >= nl) break;
+        const P *Ql = Qp + 9 * (fb.qoff + (int64_t)(l0 + c) * fb.n + j);
+        double v[9];
 #pragma unroll
-        for (int d2 = 0; d2 < 3; d2++)
-          qd[3 * d1 + d2] += a[3 * d1] * c[3 * d2] + a[3 * d1 + 1] * c[3 * d2 + 1] +
-                             a[3 * d1 + 2] * c[3 * d2 + 2];
+        for (int e = 0; e < 9; e++) v[e] = (double)Ql[e];
+#pragma unroll
+        for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; d2++)
+            qd[c][3 * d1 + d2] += a[3 * d1] * v[3 * d2] + a[3 * d1 + 1] * v[3 * d2 + 1] +
+                                  a[3 * d1 + 2] * v[3 * d2 + 2];
+      }
     }
+    // one reduction for all 2 x LC x 9 sums
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int e = 0; e < 9; e++) {
-      const double C_ = block_sum(cd[e], sh_s);
-      const double Q_ = block_sum(qd[e], sh_s);
-      if (threadIdx.x == 0) mu2 += 2.0 * fabs(C_) * fabs(Q_);
+    for (int c = 0; c < LC; c++)
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        const double s1 = wave_sum(cd[c][e]);
+        const double s2 = wave_sum(qd[c][e]);
+        if (lane == 0) {
+          sh_x[wv][c * 9 + e] = s1;
+          sh_x[wv][LC * 9 + c * 9 + e] = s2;
+        }
+      }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = blockDim.x >> 6;
+      for (int c = 0; c < nl; c++)
+        for (int e = 0; e < 9; e++) {
+          double C_ = 0.0, Q_ = 0.0;
+          for (int q = 0; q < nw; q++) C_ += sh_x[q][c * 9 + e], Q_ += sh_x[q][LC * 9 + c * 9 + e];
+          mu2 += 2.0 * fabs(C_) * fabs(Q_);
+        }
     }
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
     A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
@@ -906,7 +968,7 @@ __device__ __forceinline__ void pair_body(const float *r, int64_t P, float *out,
       const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int e = 4 * g + rr;
+        const int e = out_row<R>(g, rr);
         if (e < 9) out[9 * idx[t] + e] = (float)block_entry<R>((R)y[t][rr], e, nr);
       }
     }

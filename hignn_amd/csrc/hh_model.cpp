@@ -216,11 +216,15 @@ struct hh_model {
           w2f[(mt * 32 + s) * 64 + l] = (R)W2[(16 * mt + p) * 128 + 4 * s + g];
         }
     for (int k = 0; k < 512; k++) b2[k] = (R)B2[k];
+    // layer-2 output register r of lane group g is feature 16mt + row(g, r):
+    // 4g + r for the f32 MFMA, 4r + g for the f64 MFMA
+    const bool f64 = sizeof(R) == 8;
     for (int mt = 0; mt < 32; mt++)
       for (int r = 0; r < 4; r++)
         for (int l = 0; l < 64; l++) {
           const int p = l & 15, g = l >> 4;
-          w3f[(mt * 4 + r) * 64 + l] = p < 9 ? (R)W3[p * 512 + 16 * mt + 4 * g + r] : (R)0;
+          const int row = f64 ? 4 * r + g : 4 * g + r;
+          w3f[(mt * 4 + r) * 64 + l] = p < 9 ? (R)W3[p * 512 + 16 * mt + row] : (R)0;
         }
     for (int o = 0; o < 9; o++) b3[o] = (R)B3[o];
     upload(dst[0], w1b.data(), w1b.size() * sizeof(R));
