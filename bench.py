@@ -44,13 +44,13 @@ def work_counts(exp, ranks):
     return e_near, e_far
 
 
-def cpu_baseline(x, w, f, leaf, eps, max_iter, exp, budget_s):
+def cpu_baseline(x, w, f, leaf, eps, max_iter, mrc, exp, budget_s):
     """Oracle (C++ restatement, all host threads) timed on a bounded sample of
     the same workload: CloseDot on every k-th close block and the full ACA of
     every k-th far block, extrapolated to the whole matvec by pair-function
     evaluation counts."""
     from oracle import oracle as O
-    om = O.OracleModel(x, leaf, epsilon=eps, max_iter=max_iter)
+    om = O.OracleModel(x, leaf, epsilon=eps, max_iter=max_iter, max_relative_coord=mrc)
     om.update_coord()
     nclose, nfar = len(exp["close"]), len(exp["far"])
     # start small, grow the sample until it costs a few seconds
@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--check-targets", type=int, default=2000)
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--max-iter", type=int, default=48)
+    ap.add_argument("--max-relative-coord", type=int, default=512)
+    ap.add_argument("--epsilon", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,13 +94,14 @@ def main():
     log(f"[bench] positions N={n} in {time.time() - t0:.1f}s")
     f = workloads.gravity(n)
     w = weights.xavier(0)
-    leaf, eps, max_iter = 256, 1e-4, 32
+    leaf, eps, max_iter = 256, args.epsilon, args.max_iter
 
     hignn.Init()
     model = hignn.HignnModel(x, leaf)
     model.load_two_body_weights(w)
     model.set_epsilon(eps)
     model.set_max_iter(max_iter)
+    model.set_max_relative_coord(args.max_relative_coord)
     t0 = time.time()
     model.update_coord(x)
     t_update = time.time() - t0
@@ -177,7 +181,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            t, wn, wf, cs, fs = cpu_baseline(x, w, f, leaf, eps, max_iter, exp, args.cpu_budget)
+            t, wn, wf, cs, fs = cpu_baseline(x, w, f, leaf, eps, max_iter, args.max_relative_coord,
+                                             exp, args.cpu_budget)
             t_full = t * (e_near + e_far) / max(1, wn + wf)
             cpu = {"value": round(n / t_full, 2), "unit": "particle-updates/s",
                    "cores": os.cpu_count(), "kind": "port",
@@ -203,7 +208,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"gaussian blobs N={n} (200 blobs, sigma 15, 10% mean volume fraction, "
-                        f"seed 2), gravity; leaf {leaf}, eps {eps}, max_iter {max_iter}",
+                        f"seed 2), gravity; leaf {leaf}, eps {eps}, max_iter {max_iter}, "
+                        f"max_relative_coord {args.max_relative_coord}",
             "far_entries": f"{args.prec} MLP (surrogate not used)",
             "fp32_value": round(fp32_rate, 2) if fp32_rate else None,
             "speedup_vs_fp32": round(value / fp32_rate, 3) if fp32_rate else None,

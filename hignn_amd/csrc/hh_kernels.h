@@ -85,6 +85,28 @@ struct AcaApplyLaunch {
   const void *Q;
 };
 
+struct PAcaLaunch {
+  int prec;
+  const float4 *xp;
+  const float4 *fp;
+  double *u;
+  const FarBlock *blk;
+  const int32_t *order;
+  int32_t *rank_out;
+  int64_t nblocks;
+  int max_iter;
+  double eps2;
+  int use_symmetry;
+  void *poolC, *poolQ;
+  int64_t slotC, slotQ;
+  float *Y;
+  int64_t slotY;
+  int nslots;
+  DevWeightsF wf;
+  DevWeightsH wh;
+};
+
+hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st);
 hipError_t launch_near(const NearLaunch &L, hipStream_t st);
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st);

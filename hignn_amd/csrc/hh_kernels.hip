@@ -943,6 +943,395 @@ __global__ __launch_bounds__(256) void aca_apply_kernel(AcaApplyArgs A, const P 
 }
 
 // ---------------------------------------------------------------------------
+// Persistent far field: one workgroup (4 waves) takes far blocks from the
+// cost-sorted list and runs the whole block ACA of src/FarDot.cpp:186-1388 for
+// each of them -- column pass, residual, row pivot and normalisation, row
+// pass, residual, column pivot, norms and stop test, then the low-rank apply
+// -- with the block's factors in a per-workgroup slot of the C/Q pools (so
+// they stay cache-resident between iterations) and no host round trip per
+// iteration.  The per-block arithmetic is that of the batched reference.
+// ---------------------------------------------------------------------------
+struct PAcaArgs {
+  const float4 *xp;
+  const float4 *fp;
+  double *u;
+  const FarBlock *blk;   // blocks in processing order
+  const int32_t *order;  // position -> far-list index (for the rank output)
+  int32_t *rank_out;     // per far-list index
+  int64_t nblocks;
+  int max_iter;
+  double eps2;
+  int use_symmetry;
+  void *poolC;           // nslots x slotC entries (3x3)
+  void *poolQ;
+  int64_t slotC, slotQ;  // entries per slot
+  float *Y;              // nslots x slotY x 9
+  int64_t slotY;
+};
+
+// MLP over nitems displacements of one block: column pass (row_pass = 0):
+// x[js + piv] - x[is + i]; row pass: x[js + j] - x[is + piv].  Writes the
+// 3x3 entries (after +I, /|r|) to Y[item][9].
+template <class Core>
+__device__ __forceinline__ void pass_entries(const float4 *xp, int is, int js, int piv, int nitems,
+                                             int row_pass, float *Y, Core core) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int g = lane >> 4, p = lane & 15;
+  const float4 xpiv = row_pass ? xp[is + piv] : xp[js + piv];
+  for (int base = 64 * wv; base < nitems; base += 64 * nw) {
+    float rx[4], ry[4], rz[4];
+    int item[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      item[t] = base + 16 * t + p;
+      const int it = item[t] < nitems ? item[t] : 0;
+      if (row_pass) {
+        const float4 xb = xp[js + it];
+        rx[t] = xb.x - xpiv.x, ry[t] = xb.y - xpiv.y, rz[t] = xb.z - xpiv.z;
+      } else {
+        const float4 xa = xp[is + it];
+        rx[t] = xpiv.x - xa.x, ry[t] = xpiv.y - xa.y, rz[t] = xpiv.z - xa.z;
+      }
+    }
+    f32x4 y[4];
+    core(rx, ry, rz, y);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (item[t] >= nitems) continue;
+      const float nr = sqrtf(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int e = 4 * g + rr;
+        if (e < 9) Y[9 * (int64_t)item[t] + e] = block_entry<float>(y[t][rr], e, nr);
+      }
+    }
+  }
+}
+
+template <class P, class Core>
+__device__ void persistent_aca(const PAcaArgs &A, Core core) {
+  __shared__ int sh_stop;
+  __shared__ int ipiv[128];
+  __shared__ int jpiv[129];
+  __shared__ double sh_inv[9];
+  __shared__ double sh_mu2;
+  __shared__ ArgMax sh_am[8];
+  __shared__ double sh_s[8];
+  __shared__ double sh_x[4][72];
+  __shared__ double wsh[128][3];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  P *Cp = (P *)A.poolC + (int64_t)blockIdx.x * A.slotC * 9;
+  P *Qp = (P *)A.poolQ + (int64_t)blockIdx.x * A.slotQ * 9;
+  float *Y = A.Y + (int64_t)blockIdx.x * A.slotY * 9;
+  // Blocks are dealt out statically (b = blockIdx.x + i * gridDim.x over the
+  // cost-sorted list) so that the loop bounds are wave-uniform scalars.
+  // Every lane-0 region below is followed by a barrier: two such regions on
+  // either side of a loop's back edge could otherwise be merged into one
+  // divergent detour, letting the other lanes pass a barrier ahead of lane 0
+  // (seen in the generated code of an atomic-queue version of this loop).
+  for (int64_t bb = blockIdx.x; bb < A.nblocks; bb += gridDim.x) {
+    const int b = (int)bb;
+    const FarBlock fb = A.blk[b];
+    const int m = fb.m, n = fb.n;
+    if (tid == 0) {
+      jpiv[0] = n / 2;
+      sh_mu2 = 0.0;
+    }
+    __syncthreads();
+    int k = 0;
+    for (;;) {
+      // ---- column pass and residual, row pivot (src/FarDot.cpp:346-611)
+      const int jk = jpiv[k];
+      pass_entries(A.xp, fb.is, fb.js, jk, m, 0, Y, core);
+      __syncthreads();
+      P *Ck = Cp + 9 * (int64_t)k * m;
+      ArgMax best{-1.7976931348623157e308, -1};
+      for (int i = tid; i < m; i += blockDim.x) {
+        P c[9];
+        sym_load<P>(Y + 9 * (int64_t)i, c);
+        P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int l = 0; l < k; l++) {
+          P cl[9], ql[9];
+          load9(Cp + 9 * ((int64_t)l * m + i), cl);
+          load9(Qp + 9 * ((int64_t)l * n + jk), ql);
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+              P &s = sum[3 * r + cc];
+              s += cl[3 * r] * ql[cc];
+              s += cl[3 * r + 1] * ql[3 + cc];
+              s += cl[3 * r + 2] * ql[6 + cc];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 9; e++) c[e] -= sum[e];
+#pragma unroll
+        for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[e];
+        bool used = false;
+        for (int l = 0; l < k; l++) used |= (ipiv[l] == i);
+        const double d = (double)det3(c);
+        if (!used && d > best.val) best.val = d, best.idx = i;
+      }
+      const ArgMax am = block_argmax(best, sh_am);
+      const int ik = am.idx < 0 ? 0 : am.idx;
+      if (tid == 0) {
+        ipiv[k] = ik;
+        double mm[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) mm[e] = (double)Ck[9 * (int64_t)ik + e];
+        const double a = mm[0], bb = mm[1], c = mm[2], d = mm[3], e = mm[4], f = mm[5], g = mm[6],
+                     h = mm[7], ii = mm[8];
+        const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
+        sh_inv[0] = (e * ii - f * h) / det;
+        sh_inv[1] = -(bb * ii - c * h) / det;
+        sh_inv[2] = (bb * f - c * e) / det;
+        sh_inv[3] = -(d * ii - f * g) / det;
+        sh_inv[4] = (a * ii - c * g) / det;
+        sh_inv[5] = -(a * f - c * d) / det;
+        sh_inv[6] = (d * h - e * g) / det;
+        sh_inv[7] = -(a * h - bb * g) / det;
+        sh_inv[8] = (a * e - bb * d) / det;
+      }
+      __syncthreads();
+      {
+        P inv[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) inv[e] = (P)sh_inv[e];
+        for (int i = tid; i < m; i += blockDim.x) {
+          P c[9], o[9];
+          load9(Ck + 9 * (int64_t)i, c);
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++)
+              o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
+#pragma unroll
+          for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = o[e];
+        }
+      }
+      // ---- row pass and residual, column pivot, norms (src/FarDot.cpp:667-1108)
+      pass_entries(A.xp, fb.is, fb.js, ik, n, 1, Y, core);
+      __syncthreads();
+      P *Qk = Qp + 9 * (int64_t)k * n;
+      ArgMax bq{-1.7976931348623157e308, -1};
+      double gq[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = tid; j < n; j += blockDim.x) {
+        P q[9];
+        sym_load<P>(Y + 9 * (int64_t)j, q);
+        P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int l = 0; l < k; l++) {
+          P cl[9], ql[9];
+          load9(Cp + 9 * ((int64_t)l * m + ik), cl);
+          load9(Qp + 9 * ((int64_t)l * n + j), ql);
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+              P &s = sum[3 * r + cc];
+              s += cl[3 * r] * ql[cc];
+              s += cl[3 * r + 1] * ql[3 + cc];
+              s += cl[3 * r + 2] * ql[6 + cc];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 9; e++) q[e] -= sum[e];
+#pragma unroll
+        for (int e = 0; e < 9; e++) Qk[9 * (int64_t)j + e] = q[e];
+        bool used = false;
+        for (int l = 0; l <= k; l++) used |= (jpiv[l] == j);
+        const double d = (double)det3(q);
+        if (!used && d > bq.val) bq.val = d, bq.idx = j;
+        double qd[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) qd[e] = (double)q[e];
+        gq[0] += qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+        gq[1] += qd[3] * qd[3] + qd[4] * qd[4] + qd[5] * qd[5];
+        gq[2] += qd[6] * qd[6] + qd[7] * qd[7] + qd[8] * qd[8];
+        gq[3] += qd[0] * qd[3] + qd[1] * qd[4] + qd[2] * qd[5];
+        gq[4] += qd[3] * qd[6] + qd[4] * qd[7] + qd[5] * qd[8];
+        gq[5] += qd[6] * qd[0] + qd[7] * qd[1] + qd[8] * qd[2];
+      }
+      double gc[6] = {0, 0, 0, 0, 0, 0};
+      for (int i = tid; i < m; i += blockDim.x) {
+        double c[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) c[e] = (double)Ck[9 * (int64_t)i + e];
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) gc[kk] += c[kk] * c[kk] + c[kk + 3] * c[kk + 3] + c[kk + 6] * c[kk + 6];
+#pragma unroll
+        for (int kk = 3; kk < 6; kk++) {
+          const int a = kk % 3, bb = (kk + 1) % 3;
+          gc[kk] += c[a] * c[bb] + c[a + 3] * c[bb + 3] + c[a + 6] * c[bb + 6];
+        }
+      }
+      const ArgMax aq = block_argmax(bq, sh_am);
+      double GC[6], GQ[6];
+#pragma unroll
+      for (int e = 0; e < 6; e++) {
+        GC[e] = block_sum(gc[e], sh_s);
+        GQ[e] = block_sum(gq[e], sh_s);
+      }
+      double nu2 = 0.0;
+      if (tid == 0) {
+        nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
+              2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
+        sh_mu2 += nu2;
+      }
+      constexpr int LC = 4;
+      for (int l0 = 0; l0 + 1 < k; l0 += LC) {
+        const int nl = (k - 1 - l0) < LC ? (k - 1 - l0) : LC;
+        double cd[LC][9], qd[LC][9];
+#pragma unroll
+        for (int c = 0; c < LC; c++)
+#pragma unroll
+          for (int e = 0; e < 9; e++) cd[c][e] = qd[c][e] = 0.0;
+        for (int i = tid; i < m; i += blockDim.x) {
+          double a[9];
+#pragma unroll
+          for (int e = 0; e < 9; e++) a[e] = (double)Ck[9 * (int64_t)i + e];
+#pragma unroll
+          for (int c = 0; c < LC; c++) {
+            if (c >= nl) break;
+            const P *Cl = Cp + 9 * ((int64_t)(l0 + c) * m + i);
+            double v[9];
+#pragma unroll
+            for (int e = 0; e < 9; e++) v[e] = (double)Cl[e];
+#pragma unroll
+            for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+              for (int d2 = 0; d2 < 3; d2++)
+                cd[c][3 * d1 + d2] += a[d1] * v[d2] + a[d1 + 3] * v[d2 + 3] + a[d1 + 6] * v[d2 + 6];
+          }
+        }
+        for (int j = tid; j < n; j += blockDim.x) {
+          double a[9];
+#pragma unroll
+          for (int e = 0; e < 9; e++) a[e] = (double)Qk[9 * (int64_t)j + e];
+#pragma unroll
+          for (int c = 0; c < LC; c++) {
+            if (c >= nl) break;
+            const P *Ql = Qp + 9 * ((int64_t)(l0 + c) * n + j);
+            double v[9];
+#pragma unroll
+            for (int e = 0; e < 9; e++) v[e] = (double)Ql[e];
+#pragma unroll
+            for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+              for (int d2 = 0; d2 < 3; d2++)
+                qd[c][3 * d1 + d2] += a[3 * d1] * v[3 * d2] + a[3 * d1 + 1] * v[3 * d2 + 1] +
+                                      a[3 * d1 + 2] * v[3 * d2 + 2];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < LC; c++)
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            const double s1 = wave_sum(cd[c][e]);
+            const double s2 = wave_sum(qd[c][e]);
+            if (lane == 0) {
+              sh_x[wv][c * 9 + e] = s1;
+              sh_x[wv][LC * 9 + c * 9 + e] = s2;
+            }
+          }
+        __syncthreads();
+        if (tid == 0) {
+          for (int c = 0; c < nl; c++)
+            for (int e = 0; e < 9; e++) {
+              double C_ = 0.0, Q_ = 0.0;
+              for (int q = 0; q < nw; q++) C_ += sh_x[q][c * 9 + e], Q_ += sh_x[q][LC * 9 + c * 9 + e];
+              sh_mu2 += 2.0 * fabs(C_) * fabs(Q_);
+            }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
+        jpiv[k + 1] = aq.idx < 0 ? 0 : aq.idx;
+        const int itn = k + 1;
+        const int mn = m < n ? m : n;
+        sh_stop = (nu2 < sh_mu2 * A.eps2 || itn >= A.max_iter || itn >= mn) ? 1 : 0;
+      }
+      __syncthreads();
+      k++;
+      if (__builtin_amdgcn_readfirstlane(sh_stop)) break;  // uniform exit, see above
+    }
+    // ---- apply (src/FarDot.cpp:1200-1388)
+    const int K = k;
+    for (int pass = 0; pass < (A.use_symmetry ? 2 : 1); pass++) {
+      const P *Src = pass == 0 ? Qp : Cp;
+      const P *Dst = pass == 0 ? Cp : Qp;
+      const int ns = pass == 0 ? n : m, nd = pass == 0 ? m : n;
+      const int s0 = pass == 0 ? fb.js : fb.is, d0 = pass == 0 ? fb.is : fb.js;
+      for (int l = wv; l < K; l += nw) {
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int s = lane; s < ns; s += 64) {
+          const P *q = Src + 9 * ((int64_t)l * ns + s);
+          const float4 f = A.fp[s0 + s];
+          if (pass == 0) {
+            a0 += (double)q[0] * f.x + (double)q[1] * f.y + (double)q[2] * f.z;
+            a1 += (double)q[3] * f.x + (double)q[4] * f.y + (double)q[5] * f.z;
+            a2 += (double)q[6] * f.x + (double)q[7] * f.y + (double)q[8] * f.z;
+          } else {
+            a0 += (double)q[0] * f.x + (double)q[3] * f.y + (double)q[6] * f.z;
+            a1 += (double)q[1] * f.x + (double)q[4] * f.y + (double)q[7] * f.z;
+            a2 += (double)q[2] * f.x + (double)q[5] * f.y + (double)q[8] * f.z;
+          }
+        }
+        a0 = wave_sum(a0), a1 = wave_sum(a1), a2 = wave_sum(a2);
+        if (lane == 0) wsh[l][0] = a0, wsh[l][1] = a1, wsh[l][2] = a2;
+      }
+      __syncthreads();
+      for (int d = tid; d < nd; d += blockDim.x) {
+        double o0 = 0, o1 = 0, o2 = 0;
+        for (int l = 0; l < K; l++) {
+          const P *c = Dst + 9 * ((int64_t)l * nd + d);
+          const double w0 = wsh[l][0], w1 = wsh[l][1], w2 = wsh[l][2];
+          if (pass == 0) {
+            o0 += (double)c[0] * w0 + (double)c[1] * w1 + (double)c[2] * w2;
+            o1 += (double)c[3] * w0 + (double)c[4] * w1 + (double)c[5] * w2;
+            o2 += (double)c[6] * w0 + (double)c[7] * w1 + (double)c[8] * w2;
+          } else {
+            o0 += (double)c[0] * w0 + (double)c[3] * w1 + (double)c[6] * w2;
+            o1 += (double)c[1] * w0 + (double)c[4] * w1 + (double)c[7] * w2;
+            o2 += (double)c[2] * w0 + (double)c[5] * w1 + (double)c[8] * w2;
+          }
+        }
+        double *uu = A.u + 3 * (size_t)(d0 + d);
+        unsafeAtomicAdd(uu + 0, o0);
+        unsafeAtomicAdd(uu + 1, o1);
+        unsafeAtomicAdd(uu + 2, o2);
+      }
+      __syncthreads();
+    }
+    if (tid == 0) A.rank_out[A.order[b]] = K;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void persistent_aca_fp32_kernel(PAcaArgs A, WF<float> W) {
+  persistent_aca<double>(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
+                                f32x4(&y)[4]) { mlp_tiles_f<float, 4>(W, rx, ry, rz, y); });
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void persistent_aca_h_kernel(PAcaArgs A, WH W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  H *lw2 = (H *)lds;
+  H *lw3 = (H *)(lds + kW2FragBytes);
+  {
+    const uint4 *src2 = (const uint4 *)W.w2f;
+    const uint4 *src3 = (const uint4 *)W.w3f;
+    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
+    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
+    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
+    __syncthreads();
+  }
+  persistent_aca<float>(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
+                               f32x4(&y)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y); });
+}
+
+// ---------------------------------------------------------------------------
 // Pair function on a list of displacements: out9[p] = M(r[p]).
 // ---------------------------------------------------------------------------
 template <class R, class Core>
@@ -1161,6 +1550,33 @@ hipError_t launch_pair(int prec, const float *r, int64_t P, float *out, DevWeigh
     else
       hipLaunchKernelGGL(pair_h_kernel<f16x8>, dim3(grid_for(waves, 4, cus)), dim3(256), lds, st,
                          r, P, out, W);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st) {
+  if (L.nblocks <= 0) return hipSuccess;
+  PAcaArgs A{L.xp, L.fp, L.u, L.blk, L.order, L.rank_out, L.nblocks, L.max_iter, L.eps2,
+             L.use_symmetry, L.poolC, L.poolQ, L.slotC, L.slotQ, L.Y, L.slotY};
+  if (L.prec == PREC_FP32) {
+    WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,
+                (const float *)L.wf.w3f, (const float *)L.wf.b3};
+    hipLaunchKernelGGL(persistent_aca_fp32_kernel, dim3(L.nslots), dim3(256), 0, st, A, W);
+  } else {
+    static bool attr = false;
+    const int lds = kW2FragBytes + kW3FragBytes;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void *)persistent_aca_h_kernel<bf16x8>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      (void)hipFuncSetAttribute((const void *)persistent_aca_h_kernel<f16x8>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      attr = true;
+    }
+    WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
+    if (L.prec == PREC_BF16)
+      hipLaunchKernelGGL(persistent_aca_h_kernel<bf16x8>, dim3(L.nslots), dim3(256), lds, st, A, W);
+    else
+      hipLaunchKernelGGL(persistent_aca_h_kernel<f16x8>, dim3(L.nslots), dim3(256), lds, st, A, W);
   }
   return hipGetLastError();
 }

@@ -171,12 +171,19 @@ struct hh_model {
   DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r, tiles_c, tiles_r,
       applylist, Y, Cpool, Qpool;
   PBuf h_active, h_ybase_c, h_ybase_r, h_tiles_c, h_tiles_r, h_stop, h_list;
+  // persistent far-field workspace
+  int far_mode = 1;  // 1: batched iterations (default), 0: persistent kernel
+  DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
+  bool p_ready = false;
+  int64_t p_slotC = 0, p_slotQ = 0, p_slotY = 0;
+  int p_nslots = 0, p_maxiter = 0, p_f64 = -1;
   // check workspace
   DBuf chk_blocks, chk_tasks, chk_tlist, chk_u;
   double t_close = 0, t_far = 0, t_total = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
   ~hh_model() {
+    for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y}) b->release();
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
                     &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
                     &active_r, &ybase_c, &ybase_r, &tiles_c, &tiles_r, &applylist, &Y, &Cpool,
@@ -320,6 +327,7 @@ struct hh_model {
                              (int32_t)tl.size(J), 0, 0});
     }
     far_rank.assign(far.size(), 0);
+    p_ready = false;
     built = true;
   }
 
@@ -330,8 +338,84 @@ struct hh_model {
     uout.reserve(12 * (size_t)n);
   }
 
+  // ---- far field: persistent kernel -------------------------------------
+  // Blocks go largest first (cost k_max (m + n)); each workgroup owns one
+  // slot of the C/Q pools sized for the largest block.
+  void far_field_persistent(hipStream_t st) {
+    const int64_t nfar = (int64_t)far.size();
+    if (nfar == 0) return;
+    const bool f64 = prec == PREC_FP32;
+    const size_t esz = (f64 ? 8 : 4) * 9;
+    const int MI = max_iter;
+    if (!p_ready || p_maxiter != MI || p_f64 != (int)f64) {
+      std::vector<int32_t> order((size_t)nfar);
+      for (int64_t b = 0; b < nfar; b++) order[(size_t)b] = (int32_t)b;
+      auto cost = [&](int32_t b) {
+        const FarBlock &f = far[(size_t)b];
+        return (int64_t)std::min<int64_t>(MI, std::min(f.m, f.n)) * (f.m + f.n);
+      };
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
+      std::vector<FarBlock> hb((size_t)nfar);
+      int64_t sc = 1, sq = 1, sy = 1;
+      for (int64_t p = 0; p < nfar; p++) {
+        hb[(size_t)p] = far[(size_t)order[(size_t)p]];
+        const FarBlock &f = hb[(size_t)p];
+        const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
+        sc = std::max(sc, kmax * f.m);
+        sq = std::max(sq, kmax * f.n);
+        sy = std::max<int64_t>(sy, std::max(f.m, f.n));
+      }
+      upload(p_blk, hb.data(), hb.size() * sizeof(FarBlock));
+      upload(p_order, order.data(), order.size() * 4);
+      p_rank.reserve((size_t)nfar * 4);
+      const size_t per_slot = (size_t)(sc + sq) * esz + (size_t)sy * 36;
+      int ns = g_cus;
+      while (ns > 1 && (size_t)ns * per_slot > far_pool_bytes) ns /= 2;
+      p_poolC.reserve((size_t)ns * sc * esz);
+      p_poolQ.reserve((size_t)ns * sq * esz);
+      p_Y.reserve((size_t)ns * sy * 36);
+      p_slotC = sc, p_slotQ = sq, p_slotY = sy, p_nslots = ns, p_maxiter = MI, p_f64 = (int)f64;
+      p_ready = true;
+    }
+    PAcaLaunch L{};
+    L.prec = prec;
+    L.xp = xp.as<float4>();
+    L.fp = fp.as<float4>();
+    L.u = u.as<double>();
+    L.blk = p_blk.as<FarBlock>();
+    L.order = p_order.as<int32_t>();
+    L.rank_out = p_rank.as<int32_t>();
+    L.nblocks = nfar;
+    L.max_iter = MI;
+    L.eps2 = epsilon * epsilon;
+    L.use_symmetry = use_symmetry;
+    L.poolC = p_poolC.p;
+    L.poolQ = p_poolQ.p;
+    L.slotC = p_slotC;
+    L.slotQ = p_slotQ;
+    L.Y = p_Y.as<float>();
+    L.slotY = p_slotY;
+    L.nslots = p_nslots;
+    L.wf = devw_f(false);
+    L.wh = devw_h(prec);
+    HIPCHK(launch_persistent_aca(L, st));
+    rank_pending = true;
+  }
+
+  bool rank_pending = false;
+  void fetch_ranks() {
+    if (!rank_pending) return;
+    HIPCHK(hipMemcpy(far_rank.data(), p_rank.p, far_rank.size() * 4, hipMemcpyDeviceToHost));
+    rank_pending = false;
+  }
+
   // ---- far field: batched ACA -------------------------------------------
   void far_field(hipStream_t st) {
+    if (far_mode == 0) {
+      far_field_persistent(st);
+      return;
+    }
     const int64_t nfar = (int64_t)far.size();
     if (nfar == 0) return;
     const bool f64 = prec == PREC_FP32;
@@ -705,6 +789,7 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       delete m;
       throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
     }
+    if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
     if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
       m->far_pool_bytes = (size_t)(std::atof(gb) * (double)(1ull << 30));
     *out = m;
@@ -827,6 +912,7 @@ int hh_model_tree_export(hh_model *m, uint64_t *tree5, uint64_t *perm, float *au
 int hh_model_far_ranks(hh_model *m, int32_t *ranks) {
   return guarded([&]() {
     if (!m || !ranks) throw HHError(HH_ERR_ARG, "NULL argument");
+    m->fetch_ranks();
     std::copy(m->far_rank.begin(), m->far_rank.end(), ranks);
   });
 }

@@ -292,12 +292,14 @@ void traverse(TreeLists &tl, bool sym, int64_t R, int nthreads) {
       }
     }
   };
+  // (a leaf is taken even when it is not smaller than R: the reference would
+  // push its child ids 0,0 -- the root -- and never terminate)
   auto bounded_descendants = [&](int64_t m, std::vector<int64_t> &outv) {
     std::vector<int64_t> st{m};
     while (!st.empty()) {
       const int64_t c = st.back();
       st.pop_back();
-      if (tl.size(c) < R) {
+      if (tl.size(c) < R || tl.leaf(c)) {
         outv.push_back(c);
       } else {
         st.push_back(kids(c).first);

@@ -21,7 +21,10 @@
 //   * a split that leaves one side empty is not taken (the node stays a leaf);
 //   * N <= leaf leaves the root a leaf (src/Build.cpp:239-240 would write
 //     child ids 3,4 beyond a 3-row tree);
-//   * ACA pivots: first index wins among equal determinants.
+//   * ACA pivots: first index wins among equal determinants;
+//   * the far-node descent below maxRelativeCoord takes a leaf even when it
+//     is not smaller than the bound (src/CloseFarCheck.cpp:242-257 would
+//     push the leaf's child ids 0,0 and loop).
 // Build flags: -O3 -ffp-contract=off (no fused multiply-add anywhere).
 //
 // Parity status per function (how each is pinned) is listed in the header of
@@ -437,7 +440,9 @@ void close_far(const Tree &tr, const std::vector<float> &aux, bool sym, int64_t 
             while (!st.empty()) {
               const int64_t c = st.top();
               st.pop();
-              if (tr.size(c) < R) {
+              // a leaf is taken even when not smaller than R (the reference
+              // would push its child ids 0,0, i.e. the root, forever)
+              if (tr.size(c) < R || tr.leaf(c)) {
                 farM[nd].push_back(c);
               } else {
                 st.push((int64_t)tr.at(c, 0));

@@ -95,6 +95,21 @@ def test_dot_fp32_parity(model_small, xavier_w):
     np.testing.assert_array_equal(m.far_ranks(), om.far_ranks())
 
 
+def test_dot_fp32_persistent_far_field_parity(model_small, xavier_w, monkeypatch):
+    """The persistent far-field kernel (one workgroup runs a block's whole
+    ACA) gives the result of the batched iterations."""
+    monkeypatch.setenv("HH_FAR_MODE", "0")
+    x, f, om = model_small["x"], model_small["f"], model_small["om"]
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50)
+    m.update_coord()
+    err = O.rel_l2(m.dot(f), om.dot(xavier_w, f))
+    print("persistent fp32 dot rel err vs oracle", err)
+    assert err <= 1e-5
+    np.testing.assert_array_equal(m.far_ranks(), om.far_ranks())
+
+
 def test_dense_dot_parity(model_small, xavier_w):
     m, om, f = model_small["m"], model_small["om"], model_small["f"]
     err = O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f))

@@ -50,6 +50,17 @@ def test_max_relative_coord_bounds_far_nodes():
     np.testing.assert_array_equal(a["far"], om.export()["far"])
 
 
+def test_max_relative_coord_at_leaf_size_terminates():
+    """maxRelativeCoord <= leaf size: leaves are taken as far nodes."""
+    x = workloads.uniform_cube(3000, 0.1, 6)
+    a = abi.tree_build(x, 64, max_relative_coord=64)
+    om = O.OracleModel(x, 64, max_relative_coord=64)
+    om.update_coord()
+    b = om.export()
+    for k in ("close", "far", "leaves"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_invalid_arguments_raise():
     with pytest.raises(RuntimeError):
         abi.tree_build(np.zeros((0, 3), np.float32), 64)
