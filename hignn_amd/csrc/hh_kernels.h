@@ -53,10 +53,15 @@ struct AcaMlpLaunch {
   int cus;
 };
 
+// Far-field ACA steps and apply work on chunks of at most kAcaChunk items of
+// a block (rows of I or columns of J); a chunk is an AcaTile (slot, first item).
+constexpr int kAcaChunk = 1024;
+constexpr size_t kAcaPartBytes = 64;  // per-chunk partials (arg-max + 6 Gram sums)
+
 struct AcaStepLaunch {
   const FarBlock *blk;
   const int32_t *active;
-  const int64_t *ybase;
+  const int64_t *ybase_c, *ybase_r;
   const float *Y;
   int32_t *it;
   int32_t *ipiv;
@@ -66,7 +71,14 @@ struct AcaStepLaunch {
   int32_t *stop;
   int max_iter;
   double eps2;
+  const void *chunks_c, *chunks_r;  // column-side (I) and row-side (J) chunks
+  const int32_t *cfirst_c, *cfirst_r;  // slot -> first chunk (nactive + 1)
+  void *part_c, *part_r;            // kAcaPartBytes per chunk
+  double *x_c, *x_r;                // max_iter x 9 per chunk
+  double *inv;                      // 9 per slot
+  int chunk;                        // items per chunk, 1..kAcaChunk
   int nactive;
+  int64_t nchunks_c, nchunks_r;
   int pool_f64;
   void *C;
   void *Q;
@@ -83,6 +95,12 @@ struct AcaApplyLaunch {
   int pool_f64;
   const void *C;
   const void *Q;
+  const void *chunks_i, *chunks_j;  // (apply slot, first item) over I and J
+  const int32_t *first_i, *first_j;  // apply slot -> first chunk (nlist + 1)
+  int64_t nchunks_i, nchunks_j;
+  double *w_i, *w_j;                // max_iter x 3 per chunk
+  int max_iter;
+  int chunk;                        // items per chunk, 1..kAcaChunk
 };
 
 struct PAcaLaunch {
@@ -109,6 +127,8 @@ struct PAcaLaunch {
 hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st);
 hipError_t launch_near(const NearLaunch &L, hipStream_t st);
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);
+// which = 0: column residual, row pivot, column normalisation;
+// which = 1: row residual, column pivot, norms and stop test.
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st);
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st);
 hipError_t launch_pair(int prec, const float *r, int64_t P, float *out, DevWeightsF wf,

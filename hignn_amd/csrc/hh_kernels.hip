@@ -50,14 +50,18 @@ __device__ __forceinline__ f32x4 mma(f16x8 a, f16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// tanh x = 1 - 2/(e^{2x} + 1), with e^{2x} = 2^{2x log2 e}: one v_exp_f32 and
-// one v_rcp_f32.  Saturates correctly at +-1 (e = inf -> 1, e = 0 -> -1);
-// absolute error a few ulp of 1.
-__device__ __forceinline__ float tanh_f(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.885390081777927f);
-  return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+// Hidden activation.  The FP32 and 16-bit paths receive W1, b1, W2, b2
+// scaled by 2 log2 e on the host, so their pre-activation is x' = 2x log2 e
+// and tanh x = 1 - 2/(2^x' + 1): one v_exp_f32 and one v_rcp_f32, no scaling
+// multiply.  Saturates correctly at +-1 (2^x' = inf -> 1, 0 -> -1); absolute
+// error a few ulp of 1.  The FP64 path keeps the weights unscaled.
+__device__ __forceinline__ float act(float xs) {
+  return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(xs) + 1.0f), 1.0f);
 }
-__device__ __forceinline__ double tanh_f(double x) { return tanh(x); }
+__device__ __forceinline__ double act(double x) { return tanh(x); }
+
+__device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
 
 template <class R>
 struct Vec4;
@@ -73,6 +77,9 @@ struct Vec4<double> {
 // Row of the 16x16 MFMA output held by lane group g in register r:
 // 4g + r for the f32 (and 16-bit) instructions, 4r + g for
 // v_mfma_f64_16x16x4f64 (its lane 16(i%4)+j, register i/4 holds D[i][j]).
+// The host orders the rows of W3 (and b3) so that the layer-3 output row
+// out_row(g, r) is block entry M[g][r] for g, r < 3 and zero otherwise: lane
+// group g < 3 holds row g of the 3x3 block of its pair in registers 0..2.
 template <class R>
 __device__ __forceinline__ int out_row(int g, int r) {
   return 4 * g + r;
@@ -107,7 +114,7 @@ __device__ __forceinline__ void mlp_tiles_f(const WF<R> &W, const R (&rx)[NT], c
     const R *w = W.w1b + 4 * (4 * s + g);
     const R w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
 #pragma unroll
-    for (int t = 0; t < NT; t++) h1[t][s] = tanh_f(fma(w2, rz[t], fma(w1, ry[t], fma(w0, rx[t], b))));
+    for (int t = 0; t < NT; t++) h1[t][s] = act(fma(w2, rz[t], fma(w1, ry[t], fma(w0, rx[t], b))));
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) y[t] = V{0, 0, 0, 0};
@@ -133,7 +140,7 @@ __device__ __forceinline__ void mlp_tiles_f(const WF<R> &W, const R (&rx)[NT], c
     for (int r = 0; r < 4; r++) {
       const R a = a3[r * 64];
 #pragma unroll
-      for (int t = 0; t < NT; t++) y[t] = mma(a, tanh_f(acc[t][r]), y[t]);
+      for (int t = 0; t < NT; t++) y[t] = mma(a, act(acc[t][r]), y[t]);
     }
   }
 #pragma unroll
@@ -183,7 +190,7 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
       const float w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
 #pragma unroll
       for (int t = 0; t < NT; t++)
-        h1[t][s][j] = (E)tanh_f(__builtin_fmaf(w2, rz[t], __builtin_fmaf(w1, ry[t], __builtin_fmaf(w0, rx[t], b))));
+        h1[t][s][j] = (E)act(__builtin_fmaf(w2, rz[t], __builtin_fmaf(w1, ry[t], __builtin_fmaf(w0, rx[t], b))));
     }
   }
 #pragma unroll
@@ -221,8 +228,8 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
       H b;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        b[r] = (E)tanh_f(acc0[t][r]);
-        b[4 + r] = (E)tanh_f(acc1[t][r]);
+        b[r] = (E)act(acc0[t][r]);
+        b[4 + r] = (E)act(acc1[t][r]);
       }
       y[t] = mma(a3, b, y[t]);
     }
@@ -237,25 +244,24 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
 // ---------------------------------------------------------------------------
 // Shared epilogue helpers.
 // ---------------------------------------------------------------------------
-// Value of block entry e (0..8) of pair with displacement r given the raw
-// network output yv: (y + delta)/|r|, or delta when |r| < 1e-5
-// (python/convert.py:14-19).  Computed in the network's precision.
+// Row g of the 3x3 block of a pair with displacement r from the raw network
+// output y (registers 0..2 of lane group g): (y + I)/|r|, or I when
+// |r| < 1e-5 (python/convert.py:14-19).  One reciprocal square root per pair
+// instead of a division per entry; lanes of group 3 get zeros.
 template <class R>
-__device__ __forceinline__ R block_entry(R yv, int e, R nr) {
-  const R delta = (e == 0 || e == 4 || e == 8) ? (R)1 : (R)0;
-  return nr < (R)1e-5 ? delta : (yv + delta) / nr;
+__device__ __forceinline__ void block_row(const typename Vec4<R>::T &y, R rx, R ry, R rz, int g,
+                                          R (&m)[3]) {
+  const R nr2 = rx * rx + ry * ry + rz * rz;
+  const bool small = nr2 < (R)1e-10;
+  const R inv = small ? (R)0 : rsqrt_r(nr2);
+  const R d = small ? (R)1 : inv;
+#pragma unroll
+  for (int c = 0; c < 3; c++) m[c] = fma((R)y[c], inv, c == g ? d : (R)0);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-
-// sum over the four lane groups g (lanes p, p+16, p+32, p+48)
-__device__ __forceinline__ double group_sum(double v) {
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
   return v;
 }
 
@@ -276,52 +282,52 @@ struct NearSrc {
   double *u;           // (.,3) accumulators
 };
 
+// Epilogue of one MLP call: lane (g < 3, p) holds row g of the block of pair
+// (target t, source p) and accumulates
+//   v[t]     += sum_c M[g][c] f_p[c]           (target t, component g)
+//   v2[t][c] += M[g][c] f_p[g]                 (target t, component c; with
+//               BF_SYMM both halves are weighted 1/2: u_i += (M + M^T)/2 f_p)
+//   w        += sum_t sum_c M[g][c] f_t[c]     (BF_REUSE: source p,
+//               component g, the same block un-transposed)
+// fj is zero for an invalid source and fi[t] for an invalid target, so padded
+// pairs add nothing.  Products and three-term sums in R, accumulation in FP64.
 template <class R, int NT>
 __device__ __forceinline__ void near_epilogue(const typename Vec4<R>::T (&y)[NT], const R (&rx)[NT],
-                                              const R (&ry)[NT], const R (&rz)[NT],
-                                              const bool (&ivalid)[NT], bool jvalid,
-                                              const float4 &fj, const float4 (&fi)[NT], int flags,
-                                              double (&v)[NT][3], double (&w)[3]) {
-  const int g = (threadIdx.x & 63) >> 4;
+                                              const R (&ry)[NT], const R (&rz)[NT], int g,
+                                              const float4 &fj, float fjg,
+                                              const float4 (&fi)[NT], int flags, double (&v)[NT],
+                                              double (&v2)[NT][3], double &w) {
 #pragma unroll
   for (int t = 0; t < NT; t++) {
-    if (!(ivalid[t] && jvalid)) continue;
-    const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+    R m[3];
+    block_row<R>(y[t], rx[t], ry[t], rz[t], g, m);
+    const R s = m[0] * (R)fj.x + m[1] * (R)fj.y + m[2] * (R)fj.z;
+    if (flags & BF_SYMM) {
+      v[t] += 0.5 * (double)s;
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++) {
-      const int e = out_row<R>(g, rr);
-      if (e >= 9) continue;
-      const int row = e / 3, col = e - 3 * (e / 3);
-      const double val = (double)block_entry<R>((R)y[t][rr], e, nr);
-      const float fjc = col == 0 ? fj.x : (col == 1 ? fj.y : fj.z);
-      if (flags & BF_SYMM) {
-        const float fjr = row == 0 ? fj.x : (row == 1 ? fj.y : fj.z);
-        v[t][row] += 0.5 * val * (double)fjc;
-        v[t][col] += 0.5 * val * (double)fjr;
-      } else {
-        v[t][row] += val * (double)fjc;
-      }
-      if (flags & BF_REUSE) {
-        const float fic = col == 0 ? fi[t].x : (col == 1 ? fi[t].y : fi[t].z);
-        w[row] += val * (double)fic;
-      }
+      for (int c = 0; c < 3; c++) v2[t][c] += 0.5 * (double)(m[c] * (R)fjg);
+    } else {
+      v[t] += (double)s;
     }
+    if (flags & BF_REUSE) w += (double)(m[0] * (R)fi[t].x + m[1] * (R)fi[t].y + m[2] * (R)fi[t].z);
   }
 }
 
-// NT tiles (targets) per MLP call; a task's four targets are covered in
-// 4/NT passes over the sources (NT = 2 keeps the FP64 variant in registers).
-template <class R, int NT>
-__device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
+// One wave handles one task (4 consecutive targets of a row block) and walks
+// the block's sources 16 at a time; tile t of an MLP call is target tg + t.
+// NT tiles per MLP call, 4/NT passes over the sources.
+template <class R, int NT, class Core>
+__device__ __forceinline__ void near_body(const NearSrc<R> &S, Core core) {
   static_assert(4 % NT == 0, "NT must divide 4");
   const int lane = threadIdx.x & 63;
-  const int p = lane & 15;
+  const int g = lane >> 4, p = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t task = wave; task < S.ntasks; task += nwaves) {
     const NearTask tk = S.tasks[task];
     const RowBlock B = S.blocks[tk.block];
     for (int tg = 0; tg < 4; tg += NT) {
+      if (tk.ioff + tg >= B.m) break;
       int ti[NT];
       bool iv[NT];
       float4 xi[NT], fi[NT];
@@ -332,17 +338,17 @@ __device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
         ti[t] = li;
         const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
         xi[t] = S.xp[xi_idx];
-        fi[t] = S.fp[xi_idx];
+        fi[t] = iv[t] ? S.fp[xi_idx] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      if (!iv[0]) break;
-      double v[NT][3];
+      double v[NT], v2[NT][3];
 #pragma unroll
-      for (int t = 0; t < NT; t++) v[t][0] = v[t][1] = v[t][2] = 0.0;
+      for (int t = 0; t < NT; t++) v[t] = v2[t][0] = v2[t][1] = v2[t][2] = 0.0;
       for (int j0 = 0; j0 < B.n; j0 += 16) {
         const bool jv = j0 + p < B.n;
         const int j = B.j0 + j0 + (jv ? p : 0);
         const float4 xj = S.xp[j];
-        const float4 fj = S.fp[j];
+        const float4 fj = jv ? S.fp[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float fjg = g == 0 ? fj.x : (g == 1 ? fj.y : (g == 2 ? fj.z : 0.f));
         R rx[NT], ry[NT], rz[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) {
@@ -355,22 +361,16 @@ __device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
           }
         }
         typename Vec4<R>::T y[NT];
-        mlp_tiles_f<R, NT>(W, rx, ry, rz, y);
-        double w[3] = {0, 0, 0};
-        near_epilogue<R, NT>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
-        if (B.flags & BF_REUSE) {
-#pragma unroll
-          for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
-          if (lane < 16 && jv)
-#pragma unroll
-            for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
-        }
+        core(rx, ry, rz, y);
+        double w = 0.0;
+        near_epilogue<R, NT>(y, rx, ry, rz, g, fj, fjg, fi, B.flags, v, v2, w);
+        if ((B.flags & BF_REUSE) && g < 3 && jv) unsafeAtomicAdd(&S.u[3 * (size_t)j + g], w);
       }
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const double s = wave_sum(v[t][c]);
+          const double s = wave_sum((g == c ? v[t] : 0.0) + v2[t][c]);
           if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
         }
     }
@@ -378,83 +378,41 @@ __device__ void near_body_f(const NearSrc<R> &S, const WF<R> &W) {
 }
 
 __global__ __launch_bounds__(256) void near_fp32_kernel(NearSrc<float> S, WF<float> W) {
-  near_body_f<float, 4>(S, W);
+  near_body<float, 4>(S, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
+                             f32x4(&y)[4]) { mlp_tiles_f<float, 4>(W, rx, ry, rz, y); });
 }
 
-// FP64 variant, two tiles at a time is not needed for correctness; the same
-// four-target task shape is used with double registers.
+// FP64 variant: two tiles per MLP call keep the double registers in budget.
 __global__ __launch_bounds__(256) void near_fp64_kernel(NearSrc<double> S, WF<double> W) {
-  near_body_f<double, 2>(S, W);
+  near_body<double, 2>(S, [&](const double(&rx)[2], const double(&ry)[2], const double(&rz)[2],
+                              f64x4(&y)[2]) { mlp_tiles_f<double, 2>(W, rx, ry, rz, y); });
 }
 
-// 16-bit near kernel: persistent workgroups stage W2/W3 fragments in LDS.
+// Stage the 16-bit W2/W3 fragments of a persistent workgroup in LDS.
 template <class H>
-__global__ __launch_bounds__(256) void near_h_kernel(NearSrc<float> S, WH W) {
+__device__ __forceinline__ void stage_fragments(const WH &W, H *lw2, H *lw3) {
+  const uint4 *src2 = (const uint4 *)W.w2f;
+  const uint4 *src3 = (const uint4 *)W.w3f;
+  uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
+  for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
+  for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
+  __syncthreads();
+}
+
+// 16-bit near kernel: persistent workgroups of kNearHThreads threads (two
+// waves per SIMD) share the LDS-staged W2/W3 fragments.
+constexpr int kNearHThreads = 512;
+constexpr int kNearHTiles = 4;
+
+template <class H>
+__global__ __launch_bounds__(kNearHThreads) void near_h_kernel(NearSrc<float> S, WH W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
-  {
-    const uint4 *src2 = (const uint4 *)W.w2f;
-    const uint4 *src3 = (const uint4 *)W.w3f;
-    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
-    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
-    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
-    __syncthreads();
-  }
-  const int lane = threadIdx.x & 63;
-  const int p = lane & 15;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t task = wave; task < S.ntasks; task += nwaves) {
-    const NearTask tk = S.tasks[task];
-    const RowBlock B = S.blocks[tk.block];
-    int ti[4];
-    bool iv[4];
-    float4 xi[4], fi[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      iv[t] = tk.ioff + t < B.m;
-      const int li = B.i0 + tk.ioff + (iv[t] ? t : 0);
-      ti[t] = li;
-      const int xi_idx = (B.flags & BF_EXPLICIT_I) ? S.tlist[li] : li;
-      xi[t] = S.xp[xi_idx];
-      fi[t] = S.fp[xi_idx];
-    }
-    double v[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int j0 = 0; j0 < B.n; j0 += 16) {
-      const bool jv = j0 + p < B.n;
-      const int j = B.j0 + j0 + (jv ? p : 0);
-      const float4 xj = S.xp[j];
-      const float4 fj = S.fp[j];
-      float rx[4], ry[4], rz[4];
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        if (B.flags & BF_NEG) {
-          rx[t] = xi[t].x - xj.x, ry[t] = xi[t].y - xj.y, rz[t] = xi[t].z - xj.z;
-        } else {
-          rx[t] = xj.x - xi[t].x, ry[t] = xj.y - xi[t].y, rz[t] = xj.z - xi[t].z;
-        }
-      }
-      f32x4 y[4];
-      mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y);
-      double w[3] = {0, 0, 0};
-      near_epilogue<float, 4>(y, rx, ry, rz, iv, jv, fj, fi, B.flags, v, w);
-      if (B.flags & BF_REUSE) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) w[c] = group_sum(w[c]);
-        if (lane < 16 && jv)
-#pragma unroll
-          for (int c = 0; c < 3; c++) unsafeAtomicAdd(&S.u[3 * (size_t)j + c], w[c]);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double s = wave_sum(v[t][c]);
-        if (lane == 0 && iv[t]) unsafeAtomicAdd(&S.u[3 * (size_t)ti[t] + c], s);
-      }
-  }
+  stage_fragments<H>(W, lw2, lw3);
+  constexpr int NT = kNearHTiles;
+  near_body<float, NT>(S, [&](const float(&rx)[NT], const float(&ry)[NT], const float(&rz)[NT],
+                              f32x4(&y)[NT]) { mlp_tiles_h<H, NT>(W, lw2, lw3, rx, ry, rz, y); });
 }
 
 // ---------------------------------------------------------------------------
@@ -516,13 +474,11 @@ __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
     core(rx, ry, rz, y);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      if (!valid[t]) continue;
-      const float nr = sqrtf(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+      if (!valid[t] || g >= 3) continue;
+      float m[3];
+      block_row<float>(y[t], rx[t], ry[t], rz[t], g, m);
 #pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        const int e = 4 * g + rr;
-        if (e < 9) A.Y[9 * yrow[t] + e] = block_entry<float>(y[t][rr], e, nr);
-      }
+      for (int c = 0; c < 3; c++) A.Y[9 * yrow[t] + 3 * g + c] = m[c];
     }
   }
 }
@@ -537,32 +493,53 @@ __global__ __launch_bounds__(256) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
-  {
-    const uint4 *src2 = (const uint4 *)W.w2f;
-    const uint4 *src3 = (const uint4 *)W.w3f;
-    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
-    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
-    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
-    __syncthreads();
-  }
+  stage_fragments<H>(W, lw2, lw3);
   aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
                       f32x4(&y)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y); });
 }
 
-// ---- per-block steps (one workgroup of 256 threads per active block) ------
+// ---- chunked per-block steps ------------------------------------------------
+// The items of the active blocks (rows i of I for the column side, columns j
+// of J for the row side) are cut into chunks of at most kAcaChunk items and one
+// workgroup of 256 threads handles one chunk, so a block of 10^5 items spreads
+// over the whole chip instead of one CU.  Per-chunk partial results (pivot
+// arg-max, Gram sums, cross-term sums) are combined per block, in chunk order,
+// by one-workgroup kernels.
+constexpr int kAcaIpt = kAcaChunk / 256;  // items per thread
+constexpr double kNegHuge = -1.7976931348623157e308;
+
+struct AcaPart {  // per chunk
+  double val;     // arg-max of det over the chunk's eligible items
+  int32_t idx;    // -1: none
+  int32_t pad;
+  double g[6];    // Gram sums: GC (column side) or GQ (row side)
+};
+static_assert(sizeof(AcaPart) == kAcaPartBytes, "AcaPart layout");
+
 struct AcaStepArgs {
   const FarBlock *blk;
-  const int32_t *active;
-  const int64_t *ybase;
+  const int32_t *active;    // slot -> block
+  const int64_t *ybase_c;   // slot -> first row of Y in the column pass
+  const int64_t *ybase_r;   // slot -> first row of Y in the row pass
   const float *Y;
   int32_t *it;
-  int32_t *ipiv;
-  int32_t *jpiv;
+  int32_t *ipiv;            // per block [max_iter]
+  int32_t *jpiv;            // per block [max_iter + 1]
   double *mu2;
   double *nu2;
-  int32_t *stop;
+  int32_t *stop;            // per slot
   int max_iter;
   double eps2;
+  const AcaTile *chunks_c;  // (slot, first item) of the column-side chunks
+  const AcaTile *chunks_r;
+  const int32_t *cfirst_c;  // slot -> first column-side chunk (nactive + 1 entries)
+  const int32_t *cfirst_r;
+  AcaPart *part_c;          // per column-side chunk
+  AcaPart *part_r;
+  double *x_c;              // per chunk: max_iter x 9 cross-term sums
+  double *x_r;
+  double *inv;              // per slot: inverse of the pivot block (9)
+  int chunk;                // items per chunk, 1..kAcaChunk
 };
 
 template <class P>
@@ -601,8 +578,7 @@ __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b) {
   return a;
 }
 
-__device__ ArgMax block_argmax(ArgMax v, ArgMax *sh) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ ArgMax wave_argmax(ArgMax v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     ArgMax o;
@@ -610,6 +586,12 @@ __device__ ArgMax block_argmax(ArgMax v, ArgMax *sh) {
     o.idx = __shfl_xor(v.idx, m, 64);
     v = better(v, o);
   }
+  return v;
+}
+
+__device__ ArgMax block_argmax(ArgMax v, ArgMax *sh) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_argmax(v);
   if (lane == 0) sh[wv] = v;
   __syncthreads();
   ArgMax r = sh[0];
@@ -629,241 +611,335 @@ __device__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// Column residual, row pivot and column normalisation (src/FarDot.cpp:474-656).
-template <class P>
-__global__ __launch_bounds__(256) void aca_col_step_kernel(AcaStepArgs A, P *Cp, P *Qp) {
-  __shared__ ArgMax sh_am[8];
-  __shared__ double sh_inv[9];
-  const int slot = blockIdx.x;
-  const int b = A.active[slot];
-  const FarBlock fb = A.blk[b];
-  const int k = A.it[b];
-  const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
-  const int32_t *ipv = A.ipiv + (size_t)b * A.max_iter;
-  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
-  ArgMax best{-1.7976931348623157e308, -1};
-  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
-    P c[9];
-    sym_load<P>(A.Y + 9 * (A.ybase[slot] + i), c);
-    P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int l = 0; l < k; l++) {
-      P cl[9], ql[9];
-      load9(Cp + 9 * (fb.coff + (int64_t)l * fb.m + i), cl);
-      load9(Qp + 9 * (fb.qoff + (int64_t)l * fb.n + jk), ql);
+// Sum N per-thread values over a 256-thread workgroup and store the first
+// nvalid sums to dst (sh: 4 x N doubles of LDS).
+template <int N>
+__device__ __forceinline__ void block_sums_store(const double (&v)[N], double *sh, double *dst,
+                                                 int nvalid) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-      for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-          P &s = sum[3 * r + cc];
-          s += cl[3 * r] * ql[cc];
-          s += cl[3 * r + 1] * ql[3 + cc];
-          s += cl[3 * r + 2] * ql[6 + cc];
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < 9; e++) c[e] -= sum[e];
-#pragma unroll
-    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[e];
-    bool used = false;
-    for (int l = 0; l < k; l++) used |= (ipv[l] == i);
-    const double d = (double)det3(c);
-    if (!used && d > best.val) {
-      best.val = d;
-      best.idx = i;
-    }
-  }
-  const ArgMax am = block_argmax(best, sh_am);
-  const int ik = am.idx < 0 ? 0 : am.idx;
-  if (threadIdx.x == 0) {
-    A.ipiv[(size_t)b * A.max_iter + k] = ik;
-    double m[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) m[e] = (double)Ck[9 * (int64_t)ik + e];
-    const double a = m[0], bb = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6],
-                 h = m[7], ii = m[8];
-    const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
-    sh_inv[0] = (e * ii - f * h) / det;
-    sh_inv[1] = -(bb * ii - c * h) / det;
-    sh_inv[2] = (bb * f - c * e) / det;
-    sh_inv[3] = -(d * ii - f * g) / det;
-    sh_inv[4] = (a * ii - c * g) / det;
-    sh_inv[5] = -(a * f - c * d) / det;
-    sh_inv[6] = (d * h - e * g) / det;
-    sh_inv[7] = -(a * h - bb * g) / det;
-    sh_inv[8] = (a * e - bb * d) / det;
+  for (int e = 0; e < N; e++) {
+    const double s = wave_sum(v[e]);
+    if (lane == 0) sh[wv * N + e] = s;
   }
   __syncthreads();
-  P inv[9];
+  for (int e = threadIdx.x; e < nvalid; e += blockDim.x)
+    dst[e] = ((sh[e] + sh[N + e]) + sh[2 * N + e]) + sh[3 * N + e];
+  __syncthreads();
+}
+
+// 3x3 Gram sums of one item's block a (row-major): column side GC = a^T a,
+// row side GQ = a a^T; entries (00, 11, 22, 01, 12, 20).
+__device__ __forceinline__ void gram_col(const double (&c)[9], double (&g)[6]) {
 #pragma unroll
-  for (int e = 0; e < 9; e++) inv[e] = (P)sh_inv[e];
-  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
-    P c[9], o[9];
-    load9(Ck + 9 * (int64_t)i, c);
+  for (int kk = 0; kk < 3; kk++) g[kk] += c[kk] * c[kk] + c[kk + 3] * c[kk + 3] + c[kk + 6] * c[kk + 6];
+#pragma unroll
+  for (int kk = 3; kk < 6; kk++) {
+    const int a = kk % 3, bq = (kk + 1) % 3;
+    g[kk] += c[a] * c[bq] + c[a + 3] * c[bq + 3] + c[a + 6] * c[bq + 6];
+  }
+}
+__device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
+  g[0] += q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  g[1] += q[3] * q[3] + q[4] * q[4] + q[5] * q[5];
+  g[2] += q[6] * q[6] + q[7] * q[7] + q[8] * q[8];
+  g[3] += q[0] * q[3] + q[1] * q[4] + q[2] * q[5];
+  g[4] += q[3] * q[6] + q[4] * q[7] + q[5] * q[8];
+  g[5] += q[6] * q[0] + q[7] * q[1] + q[8] * q[2];
+}
+
+// Residual of one item: d -= sum_{l<k} L_l R_l with the 3x3 products of the
+// reference's dot loops (src/FarDot.cpp:487-511, 812-836); side 0 (column):
+// L_l = C_l(i) from the pool, R_l = Q_l(jk) from LDS; side 1 (row):
+// L_l = C_l(ik) from LDS, R_l = Q_l(j) from the pool.
+template <class P, int SIDE>
+__device__ __forceinline__ void residual(P (&d)[9], const P *pool, int64_t stride, int k,
+                                         const P (*piv)[9]) {
+  P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int l = 0; l < k; l++) {
+    P v[9];
+    load9(pool + 9 * (int64_t)l * stride, v);
+    const P *cl = SIDE == 0 ? v : piv[l];
+    const P *ql = SIDE == 0 ? piv[l] : v;
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
-      for (int cc = 0; cc < 3; cc++)
-        o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
+      for (int cc = 0; cc < 3; cc++) {
+        P &s = sum[3 * r + cc];
+        s += cl[3 * r] * ql[cc];
+        s += cl[3 * r + 1] * ql[3 + cc];
+        s += cl[3 * r + 2] * ql[6 + cc];
+      }
+  }
 #pragma unroll
-    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = o[e];
+  for (int e = 0; e < 9; e++) d[e] -= sum[e];
+}
+
+// Column pass residual and the row pivot candidates of one chunk
+// (src/FarDot.cpp:474-580).
+template <class P>
+__global__ __launch_bounds__(256) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
+  __shared__ P piv[128][9];
+  __shared__ int32_t used[128];
+  __shared__ ArgMax sh_am[4];
+  const AcaTile ch = A.chunks_c[blockIdx.x];
+  const int b = A.active[ch.slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
+  for (int t = threadIdx.x; t < 9 * k; t += blockDim.x)
+    piv[t / 9][t % 9] = Qp[9 * (fb.qoff + (int64_t)(t / 9) * fb.n + jk) + t % 9];
+  for (int l = threadIdx.x; l < k; l += blockDim.x) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
+  __syncthreads();
+  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  const int iend = min(ch.off + A.chunk, fb.m);
+  ArgMax best{kNegHuge, -1};
+  for (int i = ch.off + threadIdx.x; i < iend; i += blockDim.x) {
+    P c[9];
+    sym_load<P>(A.Y + 9 * (A.ybase_c[ch.slot] + i), c);
+    residual<P, 0>(c, Cp + 9 * (fb.coff + i), fb.m, k, piv);
+#pragma unroll
+    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[e];
+    bool u = false;
+    for (int l = 0; l < k; l++) u |= (used[l] == i);
+    const double d = (double)det3(c);
+    if (!u && d > best.val) best = ArgMax{d, i};
+  }
+  const ArgMax am = block_argmax(best, sh_am);
+  if (threadIdx.x == 0) {
+    A.part_c[blockIdx.x].val = am.val;
+    A.part_c[blockIdx.x].idx = am.idx;
   }
 }
 
-// Row residual, column pivot, norms and stop test (src/FarDot.cpp:797-1108).
+// Row pivot of each active block and the inverse of its pivot block
+// (src/FarDot.cpp:537-656).  One wave per block.
 template <class P>
-__global__ __launch_bounds__(256) void aca_row_step_kernel(AcaStepArgs A, P *Cp, P *Qp) {
-  __shared__ ArgMax sh_am[8];
-  __shared__ double sh_s[8];
-  __shared__ double sh_x[4][72];
+__global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const P *Cp) {
   const int slot = blockIdx.x;
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
-  const int ik = A.ipiv[(size_t)b * A.max_iter + k];
-  const int32_t *jpv = A.jpiv + (size_t)b * (A.max_iter + 1);
-  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
-  const P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
-  ArgMax best{-1.7976931348623157e308, -1};
-  double gq[6] = {0, 0, 0, 0, 0, 0};
-  for (int j = threadIdx.x; j < fb.n; j += blockDim.x) {
-    P q[9];
-    sym_load<P>(A.Y + 9 * (A.ybase[slot] + j), q);
-    P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int l = 0; l < k; l++) {
-      P cl[9], ql[9];
-      load9(Cp + 9 * (fb.coff + (int64_t)l * fb.m + ik), cl);
-      load9(Qp + 9 * (fb.qoff + (int64_t)l * fb.n + j), ql);
+  ArgMax best{kNegHuge, -1};
+  for (int c = A.cfirst_c[slot] + (int)threadIdx.x; c < A.cfirst_c[slot + 1]; c += 64)
+    best = better(best, ArgMax{A.part_c[c].val, A.part_c[c].idx});
+  best = wave_argmax(best);
+  if (threadIdx.x == 0) {
+    const int ik = best.idx < 0 ? 0 : best.idx;
+    A.ipiv[(size_t)b * A.max_iter + k] = ik;
+    const P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
+    double m[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) m[e] = (double)Ck[e];
+    const double a = m[0], bb = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6],
+                 h = m[7], ii = m[8];
+    const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
+    double *inv = A.inv + 9 * (size_t)slot;
+    inv[0] = (e * ii - f * h) / det;
+    inv[1] = -(bb * ii - c * h) / det;
+    inv[2] = (bb * f - c * e) / det;
+    inv[3] = -(d * ii - f * g) / det;
+    inv[4] = (a * ii - c * g) / det;
+    inv[5] = -(a * f - c * d) / det;
+    inv[6] = (d * h - e * g) / det;
+    inv[7] = -(a * h - bb * g) / det;
+    inv[8] = (a * e - bb * d) / det;
+  }
+}
+
+constexpr int kCrossLC = 4;  // cross-term iterations per pass over a chunk
+
+// Column normalisation C_k = R_k inv (src/FarDot.cpp:630-656), the Gram sums
+// of C_k and the column halves of the cross terms C_k^T C_l, l < k-1
+// (src/FarDot.cpp:1002-1078), for one chunk.
+template <class P>
+__global__ __launch_bounds__(256) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) {
+  __shared__ double sh[4 * 9 * kCrossLC];
+  const AcaTile ch = A.chunks_c[blockIdx.x];
+  const int b = A.active[ch.slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  P inv[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++) inv[e] = (P)A.inv[9 * (size_t)ch.slot + e];
+  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  const int iend = min(ch.off + A.chunk, fb.m);
+  double a[kAcaIpt][9];
+  double g[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int i = ch.off + threadIdx.x + 256 * s;
+#pragma unroll
+    for (int e = 0; e < 9; e++) a[s][e] = 0.0;
+    if (i < iend) {
+      P c[9], o[9];
+      load9(Ck + 9 * (int64_t)i, c);
 #pragma unroll
       for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-          P &s = sum[3 * r + cc];
-          s += cl[3 * r] * ql[cc];
-          s += cl[3 * r + 1] * ql[3 + cc];
-          s += cl[3 * r + 2] * ql[6 + cc];
-        }
+        for (int cc = 0; cc < 3; cc++)
+          o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        Ck[9 * (int64_t)i + e] = o[e];
+        a[s][e] = (double)o[e];
+      }
+      gram_col(a[s], g);
     }
-#pragma unroll
-    for (int e = 0; e < 9; e++) q[e] -= sum[e];
-#pragma unroll
-    for (int e = 0; e < 9; e++) Qk[9 * (int64_t)j + e] = q[e];
-    bool used = false;
-    for (int l = 0; l <= k; l++) used |= (jpv[l] == j);
-    const double d = (double)det3(q);
-    if (!used && d > best.val) {
-      best.val = d;
-      best.idx = j;
-    }
-    double qd[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) qd[e] = (double)q[e];
-    gq[0] += qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
-    gq[1] += qd[3] * qd[3] + qd[4] * qd[4] + qd[5] * qd[5];
-    gq[2] += qd[6] * qd[6] + qd[7] * qd[7] + qd[8] * qd[8];
-    gq[3] += qd[0] * qd[3] + qd[1] * qd[4] + qd[2] * qd[5];
-    gq[4] += qd[3] * qd[6] + qd[4] * qd[7] + qd[5] * qd[8];
-    gq[5] += qd[6] * qd[0] + qd[7] * qd[1] + qd[8] * qd[2];
   }
-  double gc[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
-    double c[9];
+  {
+    AcaPart &pt = A.part_c[blockIdx.x];
 #pragma unroll
-    for (int e = 0; e < 9; e++) c[e] = (double)Ck[9 * (int64_t)i + e];
+    for (int e = 0; e < 6; e++) {
+      const double s = block_sum(g[e], sh);
+      if (threadIdx.x == 0) pt.g[e] = s;
+    }
+  }
+  double *X = A.x_c + (size_t)blockIdx.x * A.max_iter * 9;
+  for (int l0 = 0; l0 + 1 < k; l0 += kCrossLC) {
+    const int nl = min(kCrossLC, k - 1 - l0);
+    double cd[kCrossLC * 9];
 #pragma unroll
-    for (int kk = 0; kk < 3; kk++) gc[kk] += c[kk] * c[kk] + c[kk + 3] * c[kk + 3] + c[kk + 6] * c[kk + 6];
+    for (int e = 0; e < kCrossLC * 9; e++) cd[e] = 0.0;
 #pragma unroll
-    for (int kk = 3; kk < 6; kk++) {
-      const int a = kk % 3, bq = (kk + 1) % 3;
-      gc[kk] += c[a] * c[bq] + c[a + 3] * c[bq + 3] + c[a + 6] * c[bq + 6];
+    for (int s = 0; s < kAcaIpt; s++) {
+      const int i = ch.off + threadIdx.x + 256 * s;
+      if (i >= iend) break;
+#pragma unroll
+      for (int c = 0; c < kCrossLC; c++) {
+        if (c >= nl) break;
+        P v[9];
+        load9(Cp + 9 * (fb.coff + (int64_t)(l0 + c) * fb.m + i), v);
+#pragma unroll
+        for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; d2++)
+            cd[9 * c + 3 * d1 + d2] += a[s][d1] * (double)v[d2] + a[s][d1 + 3] * (double)v[d2 + 3] +
+                                       a[s][d1 + 6] * (double)v[d2 + 6];
+      }
+    }
+    block_sums_store<kCrossLC * 9>(cd, sh, X + 9 * l0, 9 * nl);
+  }
+}
+
+// Row pass residual Q_k, column pivot candidates, Gram sums of Q_k and the
+// row halves of the cross terms Q_k Q_l^T, l < k-1 (src/FarDot.cpp:797-1078),
+// for one chunk.
+template <class P>
+__global__ __launch_bounds__(256) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
+  __shared__ P piv[128][9];
+  __shared__ int32_t used[129];
+  __shared__ ArgMax sh_am[4];
+  __shared__ double sh[4 * 9 * kCrossLC];
+  const AcaTile ch = A.chunks_r[blockIdx.x];
+  const int b = A.active[ch.slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int ik = A.ipiv[(size_t)b * A.max_iter + k];
+  for (int t = threadIdx.x; t < 9 * k; t += blockDim.x)
+    piv[t / 9][t % 9] = Cp[9 * (fb.coff + (int64_t)(t / 9) * fb.m + ik) + t % 9];
+  for (int l = threadIdx.x; l <= k; l += blockDim.x)
+    used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
+  __syncthreads();
+  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
+  const int jend = min(ch.off + A.chunk, fb.n);
+  ArgMax best{kNegHuge, -1};
+  double a[kAcaIpt][9];
+  double g[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int j = ch.off + threadIdx.x + 256 * s;
+#pragma unroll
+    for (int e = 0; e < 9; e++) a[s][e] = 0.0;
+    if (j < jend) {
+      P q[9];
+      sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + j), q);
+      residual<P, 1>(q, Qp + 9 * (fb.qoff + j), fb.n, k, piv);
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        Qk[9 * (int64_t)j + e] = q[e];
+        a[s][e] = (double)q[e];
+      }
+      bool u = false;
+      for (int l = 0; l <= k; l++) u |= (used[l] == j);
+      const double d = (double)det3(q);
+      if (!u && d > best.val) best = ArgMax{d, j};
+      gram_row(a[s], g);
     }
   }
   const ArgMax am = block_argmax(best, sh_am);
-  double GC[6], GQ[6];
+  {
+    AcaPart &pt = A.part_r[blockIdx.x];
+    if (threadIdx.x == 0) pt.val = am.val, pt.idx = am.idx;
 #pragma unroll
-  for (int e = 0; e < 6; e++) {
-    GC[e] = block_sum(gc[e], sh_s);
-    GQ[e] = block_sum(gq[e], sh_s);
+    for (int e = 0; e < 6; e++) {
+      const double s = block_sum(g[e], sh);
+      if (threadIdx.x == 0) pt.g[e] = s;
+    }
   }
-  double mu2 = 0.0, nu2 = 0.0;
-  if (threadIdx.x == 0) {
-    nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
-          2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
-    mu2 = A.mu2[b] + nu2;
-  }
-  // cross terms with the iterations l < k-1 (src/FarDot.cpp:1041-1078), four
-  // iterations per pass over the block, all 72 sums reduced together; mu2
-  // is accumulated in the reference's order (l, d1, d2 ascending).
-  constexpr int LC = 4;
-  for (int l0 = 0; l0 + 1 < k; l0 += LC) {
-    const int nl = (k - 1 - l0) < LC ? (k - 1 - l0) : LC;
-    double cd[LC][9], qd[LC][9];
+  double *X = A.x_r + (size_t)blockIdx.x * A.max_iter * 9;
+  for (int l0 = 0; l0 + 1 < k; l0 += kCrossLC) {
+    const int nl = min(kCrossLC, k - 1 - l0);
+    double qd[kCrossLC * 9];
 #pragma unroll
-    for (int c = 0; c < LC; c++)
+    for (int e = 0; e < kCrossLC * 9; e++) qd[e] = 0.0;
 #pragma unroll
-      for (int e = 0; e < 9; e++) cd[c][e] = qd[c][e] = 0.0;
-    for (int i = threadIdx.x; i < fb.m; i += blockDim.x) {
-      double a[9];
+    for (int s = 0; s < kAcaIpt; s++) {
+      const int j = ch.off + threadIdx.x + 256 * s;
+      if (j >= jend) break;
 #pragma unroll
-      for (int e = 0; e < 9; e++) a[e] = (double)Ck[9 * (int64_t)i + e];
-#pragma unroll
-      for (int c = 0; c < LC; c++) {
+      for (int c = 0; c < kCrossLC; c++) {
         if (c >= nl) break;
-        const P *Cl = Cp + 9 * (fb.coff + (int64_t)(l0 + c) * fb.m + i);
-        double v[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) v[e] = (double)Cl[e];
+        P v[9];
+        load9(Qp + 9 * (fb.qoff + (int64_t)(l0 + c) * fb.n + j), v);
 #pragma unroll
         for (int d1 = 0; d1 < 3; d1++)
 #pragma unroll
           for (int d2 = 0; d2 < 3; d2++)
-            cd[c][3 * d1 + d2] += a[d1] * v[d2] + a[d1 + 3] * v[d2 + 3] + a[d1 + 6] * v[d2 + 6];
+            qd[9 * c + 3 * d1 + d2] += a[s][3 * d1] * (double)v[3 * d2] +
+                                       a[s][3 * d1 + 1] * (double)v[3 * d2 + 1] +
+                                       a[s][3 * d1 + 2] * (double)v[3 * d2 + 2];
       }
     }
-    for (int j = threadIdx.x; j < fb.n; j += blockDim.x) {
-      double a[9];
-#pragma unroll
-      for (int e = 0; e < 9; e++) a[e] = (double)Qk[9 * (int64_t)j + e];
-#pragma unroll
-      for (int c = 0; c < LC; c++) {
-        if (c >= nl) break;
-        const P *Ql = Qp + 9 * (fb.qoff + (int64_t)(l0 + c) * fb.n + j);
-        double v[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) v[e] = (double)Ql[e];
-#pragma unroll
-        for (int d1 = 0; d1 < 3; d1++)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; d2++)
-            qd[c][3 * d1 + d2] += a[3 * d1] * v[3 * d2] + a[3 * d1 + 1] * v[3 * d2 + 1] +
-                                  a[3 * d1 + 2] * v[3 * d2 + 2];
-      }
-    }
-    // one reduction for all 2 x LC x 9 sums
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < LC; c++)
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        const double s1 = wave_sum(cd[c][e]);
-        const double s2 = wave_sum(qd[c][e]);
-        if (lane == 0) {
-          sh_x[wv][c * 9 + e] = s1;
-          sh_x[wv][LC * 9 + c * 9 + e] = s2;
-        }
-      }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int nw = blockDim.x >> 6;
-      for (int c = 0; c < nl; c++)
-        for (int e = 0; e < 9; e++) {
-          double C_ = 0.0, Q_ = 0.0;
-          for (int q = 0; q < nw; q++) C_ += sh_x[q][c * 9 + e], Q_ += sh_x[q][LC * 9 + c * 9 + e];
-          mu2 += 2.0 * fabs(C_) * fabs(Q_);
-        }
-    }
-    __syncthreads();
+    block_sums_store<kCrossLC * 9>(qd, sh, X + 9 * l0, 9 * nl);
   }
+}
+
+// Per block: next column pivot, nu2 = |C_k Q_k|_F^2, mu2 update with the
+// cross terms in the reference's order (l, d1, d2 ascending) and the stop
+// test (src/FarDot.cpp:1002-1108).
+__global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
+  __shared__ double cs[127 * 9], qs[127 * 9];
+  __shared__ ArgMax sh_am[4];
+  const int slot = blockIdx.x;
+  const int b = A.active[slot];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int c0 = A.cfirst_c[slot], c1 = A.cfirst_c[slot + 1];
+  const int r0 = A.cfirst_r[slot], r1 = A.cfirst_r[slot + 1];
+  ArgMax best{kNegHuge, -1};
+  for (int c = r0 + (int)threadIdx.x; c < r1; c += blockDim.x)
+    best = better(best, ArgMax{A.part_r[c].val, A.part_r[c].idx});
+  const ArgMax am = block_argmax(best, sh_am);
+  const int ncross = 9 * (k > 1 ? k - 1 : 0);
+  for (int t = threadIdx.x; t < ncross; t += blockDim.x) {
+    double sc = 0.0, sq = 0.0;
+    for (int c = c0; c < c1; c++) sc += A.x_c[(size_t)c * A.max_iter * 9 + t];
+    for (int c = r0; c < r1; c++) sq += A.x_r[(size_t)c * A.max_iter * 9 + t];
+    cs[t] = sc;
+    qs[t] = sq;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double GC[6] = {0, 0, 0, 0, 0, 0}, GQ[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = c0; c < c1; c++)
+      for (int e = 0; e < 6; e++) GC[e] += A.part_c[c].g[e];
+    for (int c = r0; c < r1; c++)
+      for (int e = 0; e < 6; e++) GQ[e] += A.part_r[c].g[e];
+    const double nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
+                       2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
+    double mu2 = A.mu2[b] + nu2;
+    for (int t = 0; t < ncross; t++) mu2 += 2.0 * fabs(cs[t]) * fabs(qs[t]);
     A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
     A.mu2[b] = mu2;
     A.nu2[b] = nu2;
@@ -874,71 +950,108 @@ __global__ __launch_bounds__(256) void aca_row_step_kernel(AcaStepArgs A, P *Cp,
   }
 }
 
-// Apply of stopped blocks (src/FarDot.cpp:1200-1388):
+// ---- chunked apply of stopped blocks (src/FarDot.cpp:1200-1388) -----------
 //   u_I += sum_l C_l (sum_j Q_l[j] f_j);  sym: u_J += sum_l Q_l^T (sum_i C_l[i]^T f_i).
+// Phase 1 (per source-side chunk): partial w_l over the chunk's items.
+// Phase 2 (per target-side chunk): w_l summed over the block's source chunks,
+// then u += sum_l (C_l or Q_l^T) w_l.  Side 0: sources J, targets I; side 1
+// (symmetry): sources I, targets J.
 struct AcaApplyArgs {
   const FarBlock *blk;
-  const int32_t *list;  // blocks to apply
+  const int32_t *list;       // apply slot -> block
   const int32_t *it;
   const float4 *fp;
   double *u;
-  int use_symmetry;
+  const AcaTile *chunks_i;   // (apply slot, first item) over I
+  const AcaTile *chunks_j;   // over J
+  const int32_t *first_i;    // apply slot -> first I chunk (nlist + 1)
+  const int32_t *first_j;
+  double *w_j;               // per J chunk: max_iter x 3 (side 0 partials)
+  double *w_i;               // per I chunk: max_iter x 3 (side 1 partials)
+  int max_iter;
+  int chunk;                 // items per chunk, 1..kAcaChunk
 };
 
-template <class P>
-__global__ __launch_bounds__(256) void aca_apply_kernel(AcaApplyArgs A, const P *Cp, const P *Qp) {
-  __shared__ double wsh[128][3];
-  const int b = A.list[blockIdx.x];
+template <class P, int SIDE>
+__global__ __launch_bounds__(256) void aca_apply_reduce_kernel(AcaApplyArgs A, const P *Cp,
+                                                               const P *Qp) {
+  __shared__ double sh[4 * 3];
+  const AcaTile ch = SIDE == 0 ? A.chunks_j[blockIdx.x] : A.chunks_i[blockIdx.x];
+  const int b = A.list[ch.slot];
   const FarBlock fb = A.blk[b];
-  const int K = A.it[b];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int pass = 0; pass < (A.use_symmetry ? 2 : 1); pass++) {
-    // pass 0: w_l = sum_j Q_l[j] f_j, then u_i += C_l[i] w_l
-    // pass 1: w_l = sum_i C_l[i]^T f_i, then u_j += Q_l[j]^T w_l
-    const P *Src = pass == 0 ? Qp + 9 * fb.qoff : Cp + 9 * fb.coff;
-    const P *Dst = pass == 0 ? Cp + 9 * fb.coff : Qp + 9 * fb.qoff;
-    const int ns = pass == 0 ? fb.n : fb.m, nd = pass == 0 ? fb.m : fb.n;
-    const int s0 = pass == 0 ? fb.js : fb.is, d0 = pass == 0 ? fb.is : fb.js;
-    for (int l = wv; l < K; l += nw) {
-      double a0 = 0, a1 = 0, a2 = 0;
-      for (int s = lane; s < ns; s += 64) {
-        const P *q = Src + 9 * ((int64_t)l * ns + s);
-        const float4 f = A.fp[s0 + s];
-        if (pass == 0) {
-          a0 += (double)q[0] * f.x + (double)q[1] * f.y + (double)q[2] * f.z;
-          a1 += (double)q[3] * f.x + (double)q[4] * f.y + (double)q[5] * f.z;
-          a2 += (double)q[6] * f.x + (double)q[7] * f.y + (double)q[8] * f.z;
-        } else {
-          a0 += (double)q[0] * f.x + (double)q[3] * f.y + (double)q[6] * f.z;
-          a1 += (double)q[1] * f.x + (double)q[4] * f.y + (double)q[7] * f.z;
-          a2 += (double)q[2] * f.x + (double)q[5] * f.y + (double)q[8] * f.z;
-        }
+  const int K = min(A.it[b], 128);
+  const int ns = SIDE == 0 ? fb.n : fb.m;
+  const int s0 = SIDE == 0 ? fb.js : fb.is;
+  const P *Src = SIDE == 0 ? Qp + 9 * fb.qoff : Cp + 9 * fb.coff;
+  const int send = min(ch.off + A.chunk, ns);
+  double *W = (SIDE == 0 ? A.w_j : A.w_i) + (size_t)blockIdx.x * A.max_iter * 3;
+  float4 f[kAcaIpt];
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int it = ch.off + threadIdx.x + 256 * s;
+    f[s] = it < send ? A.fp[s0 + it] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  for (int l = 0; l < K; l++) {
+    double a[3] = {0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kAcaIpt; s++) {
+      const int it = ch.off + threadIdx.x + 256 * s;
+      if (it >= send) break;
+      const P *q = Src + 9 * ((int64_t)l * ns + it);
+      if (SIDE == 0) {
+        a[0] += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
+        a[1] += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
+        a[2] += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
+      } else {
+        a[0] += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
+        a[1] += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
+        a[2] += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
       }
-      a0 = wave_sum(a0), a1 = wave_sum(a1), a2 = wave_sum(a2);
-      if (lane == 0 && l < 128) wsh[l][0] = a0, wsh[l][1] = a1, wsh[l][2] = a2;
     }
-    __syncthreads();
-    for (int d = threadIdx.x; d < nd; d += blockDim.x) {
-      double o0 = 0, o1 = 0, o2 = 0;
-      for (int l = 0; l < K && l < 128; l++) {
-        const P *c = Dst + 9 * ((int64_t)l * nd + d);
-        const double w0 = wsh[l][0], w1 = wsh[l][1], w2 = wsh[l][2];
-        if (pass == 0) {
-          o0 += (double)c[0] * w0 + (double)c[1] * w1 + (double)c[2] * w2;
-          o1 += (double)c[3] * w0 + (double)c[4] * w1 + (double)c[5] * w2;
-          o2 += (double)c[6] * w0 + (double)c[7] * w1 + (double)c[8] * w2;
-        } else {
-          o0 += (double)c[0] * w0 + (double)c[3] * w1 + (double)c[6] * w2;
-          o1 += (double)c[1] * w0 + (double)c[4] * w1 + (double)c[7] * w2;
-          o2 += (double)c[2] * w0 + (double)c[5] * w1 + (double)c[8] * w2;
-        }
+    block_sums_store<3>(a, sh, W + 3 * l, 3);
+  }
+}
+
+template <class P, int SIDE>
+__global__ __launch_bounds__(256) void aca_apply_update_kernel(AcaApplyArgs A, const P *Cp,
+                                                               const P *Qp) {
+  __shared__ double wsh[128 * 3];
+  const AcaTile ch = SIDE == 0 ? A.chunks_i[blockIdx.x] : A.chunks_j[blockIdx.x];
+  const int b = A.list[ch.slot];
+  const FarBlock fb = A.blk[b];
+  const int K = min(A.it[b], 128);
+  const int w0 = SIDE == 0 ? A.first_j[ch.slot] : A.first_i[ch.slot];
+  const int w1 = SIDE == 0 ? A.first_j[ch.slot + 1] : A.first_i[ch.slot + 1];
+  const double *W = SIDE == 0 ? A.w_j : A.w_i;
+  for (int t = threadIdx.x; t < 3 * K; t += blockDim.x) {
+    double s = 0.0;
+    for (int c = w0; c < w1; c++) s += W[(size_t)c * A.max_iter * 3 + t];
+    wsh[t] = s;
+  }
+  __syncthreads();
+  const int nd = SIDE == 0 ? fb.m : fb.n;
+  const int d0 = SIDE == 0 ? fb.is : fb.js;
+  const P *Dst = SIDE == 0 ? Cp + 9 * fb.coff : Qp + 9 * fb.qoff;
+  const int dend = min(ch.off + A.chunk, nd);
+  for (int d = ch.off + threadIdx.x; d < dend; d += blockDim.x) {
+    double o0 = 0, o1 = 0, o2 = 0;
+    for (int l = 0; l < K; l++) {
+      const P *c = Dst + 9 * ((int64_t)l * nd + d);
+      const double x0 = wsh[3 * l], x1 = wsh[3 * l + 1], x2 = wsh[3 * l + 2];
+      if (SIDE == 0) {
+        o0 += (double)c[0] * x0 + (double)c[1] * x1 + (double)c[2] * x2;
+        o1 += (double)c[3] * x0 + (double)c[4] * x1 + (double)c[5] * x2;
+        o2 += (double)c[6] * x0 + (double)c[7] * x1 + (double)c[8] * x2;
+      } else {
+        o0 += (double)c[0] * x0 + (double)c[3] * x1 + (double)c[6] * x2;
+        o1 += (double)c[1] * x0 + (double)c[4] * x1 + (double)c[7] * x2;
+        o2 += (double)c[2] * x0 + (double)c[5] * x1 + (double)c[8] * x2;
       }
-      double *u = A.u + 3 * (size_t)(d0 + d);
-      unsafeAtomicAdd(u + 0, o0);
-      unsafeAtomicAdd(u + 1, o1);
-      unsafeAtomicAdd(u + 2, o2);
     }
-    __syncthreads();
+    double *u = A.u + 3 * (size_t)(d0 + d);
+    unsafeAtomicAdd(u + 0, o0);
+    unsafeAtomicAdd(u + 1, o1);
+    unsafeAtomicAdd(u + 2, o2);
   }
 }
 
@@ -997,13 +1110,11 @@ __device__ __forceinline__ void pass_entries(const float4 *xp, int is, int js, i
     core(rx, ry, rz, y);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      if (item[t] >= nitems) continue;
-      const float nr = sqrtf(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+      if (item[t] >= nitems || g >= 3) continue;
+      float m[3];
+      block_row<float>(y[t], rx[t], ry[t], rz[t], g, m);
 #pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        const int e = 4 * g + rr;
-        if (e < 9) Y[9 * (int64_t)item[t] + e] = block_entry<float>(y[t][rr], e, nr);
-      }
+      for (int c = 0; c < 3; c++) Y[9 * (int64_t)item[t] + 3 * g + c] = m[c];
     }
   }
 }
@@ -1319,14 +1430,7 @@ __global__ __launch_bounds__(256) void persistent_aca_h_kernel(PAcaArgs A, WH W)
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
-  {
-    const uint4 *src2 = (const uint4 *)W.w2f;
-    const uint4 *src3 = (const uint4 *)W.w3f;
-    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
-    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
-    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
-    __syncthreads();
-  }
+  stage_fragments<H>(W, lw2, lw3);
   persistent_aca<float>(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
                                f32x4(&y)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y); });
 }
@@ -1353,13 +1457,11 @@ __device__ __forceinline__ void pair_body(const float *r, int64_t P, float *out,
     core(rx, ry, rz, y);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      if (idx[t] >= P) continue;
-      const R nr = sqrt(rx[t] * rx[t] + ry[t] * ry[t] + rz[t] * rz[t]);
+      if (idx[t] >= P || g >= 3) continue;
+      R m[3];
+      block_row<R>(y[t], rx[t], ry[t], rz[t], g, m);
 #pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        const int e = out_row<R>(g, rr);
-        if (e < 9) out[9 * idx[t] + e] = (float)block_entry<R>((R)y[t][rr], e, nr);
-      }
+      for (int c = 0; c < 3; c++) out[9 * idx[t] + 3 * g + c] = (float)m[c];
     }
   }
 }
@@ -1381,14 +1483,7 @@ __global__ __launch_bounds__(256) void pair_h_kernel(const float *r, int64_t P, 
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
-  {
-    const uint4 *src2 = (const uint4 *)W.w2f;
-    const uint4 *src3 = (const uint4 *)W.w3f;
-    uint4 *d2 = (uint4 *)lw2, *d3 = (uint4 *)lw3;
-    for (int i = threadIdx.x; i < kW2FragBytes / 16; i += blockDim.x) d2[i] = src2[i];
-    for (int i = threadIdx.x; i < kW3FragBytes / 16; i += blockDim.x) d3[i] = src3[i];
-    __syncthreads();
-  }
+  stage_fragments<H>(W, lw2, lw3);
   pair_body<float>(r, P, out, [&](const float(&x)[4], const float(&y)[4], const float(&z)[4],
                                   f32x4(&o)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, x, y, z, o); });
 }
@@ -1459,12 +1554,12 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
     allow_big_lds();
     NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
     WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
-    const int grid = grid_for(L.ntasks, 4, L.cus);
+    const int grid = grid_for(L.ntasks, kNearHThreads / 64, L.cus);
     const size_t lds = kW2FragBytes + kW3FragBytes;
     if (L.prec == PREC_BF16)
-      hipLaunchKernelGGL(near_h_kernel<bf16x8>, dim3(grid), dim3(256), lds, st, S, W);
+      hipLaunchKernelGGL(near_h_kernel<bf16x8>, dim3(grid), dim3(kNearHThreads), lds, st, S, W);
     else
-      hipLaunchKernelGGL(near_h_kernel<f16x8>, dim3(grid), dim3(256), lds, st, S, W);
+      hipLaunchKernelGGL(near_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, S, W);
   }
   return hipGetLastError();
 }
@@ -1494,35 +1589,55 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
 
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
   if (L.nactive <= 0) return hipSuccess;
-  AcaStepArgs A{L.blk, L.active, L.ybase, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2, L.stop,
-                L.max_iter, L.eps2};
-  if (L.pool_f64) {
-    if (which == 0)
-      hipLaunchKernelGGL(aca_col_step_kernel<double>, dim3(L.nactive), dim3(256), 0, st, A,
-                         (double *)L.C, (double *)L.Q);
-    else
-      hipLaunchKernelGGL(aca_row_step_kernel<double>, dim3(L.nactive), dim3(256), 0, st, A,
-                         (double *)L.C, (double *)L.Q);
+  AcaStepArgs A{L.blk, L.active, L.ybase_c, L.ybase_r, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2,
+                L.stop, L.max_iter, L.eps2, (const AcaTile *)L.chunks_c,
+                (const AcaTile *)L.chunks_r, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
+                (AcaPart *)L.part_r, L.x_c, L.x_r, L.inv, L.chunk};
+  const dim3 gc((unsigned)L.nchunks_c), gr((unsigned)L.nchunks_r), ga((unsigned)L.nactive);
+  if (which == 0) {
+    if (L.pool_f64) {
+      hipLaunchKernelGGL(aca_col_resid_kernel<double>, gc, dim3(256), 0, st, A, (double *)L.C,
+                         (const double *)L.Q);
+      hipLaunchKernelGGL(aca_col_pivot_kernel<double>, ga, dim3(64), 0, st, A,
+                         (const double *)L.C);
+      hipLaunchKernelGGL(aca_col_norm_kernel<double>, gc, dim3(256), 0, st, A, (double *)L.C);
+    } else {
+      hipLaunchKernelGGL(aca_col_resid_kernel<float>, gc, dim3(256), 0, st, A, (float *)L.C,
+                         (const float *)L.Q);
+      hipLaunchKernelGGL(aca_col_pivot_kernel<float>, ga, dim3(64), 0, st, A, (const float *)L.C);
+      hipLaunchKernelGGL(aca_col_norm_kernel<float>, gc, dim3(256), 0, st, A, (float *)L.C);
+    }
   } else {
-    if (which == 0)
-      hipLaunchKernelGGL(aca_col_step_kernel<float>, dim3(L.nactive), dim3(256), 0, st, A,
-                         (float *)L.C, (float *)L.Q);
+    if (L.pool_f64)
+      hipLaunchKernelGGL(aca_row_resid_kernel<double>, gr, dim3(256), 0, st, A,
+                         (const double *)L.C, (double *)L.Q);
     else
-      hipLaunchKernelGGL(aca_row_step_kernel<float>, dim3(L.nactive), dim3(256), 0, st, A,
-                         (float *)L.C, (float *)L.Q);
+      hipLaunchKernelGGL(aca_row_resid_kernel<float>, gr, dim3(256), 0, st, A, (const float *)L.C,
+                         (float *)L.Q);
+    hipLaunchKernelGGL(aca_finalize_kernel, ga, dim3(256), 0, st, A);
   }
   return hipGetLastError();
 }
 
+template <class P>
+static void apply_launch(const AcaApplyArgs &A, const AcaApplyLaunch &L, hipStream_t st) {
+  const P *C = (const P *)L.C, *Q = (const P *)L.Q;
+  const dim3 gi((unsigned)L.nchunks_i), gj((unsigned)L.nchunks_j);
+  hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 0>), gj, dim3(256), 0, st, A, C, Q);
+  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 1>), gi, dim3(256), 0, st, A, C, Q);
+  hipLaunchKernelGGL((aca_apply_update_kernel<P, 0>), gi, dim3(256), 0, st, A, C, Q);
+  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_update_kernel<P, 1>), gj, dim3(256), 0, st, A, C, Q);
+}
+
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st) {
   if (L.nlist <= 0) return hipSuccess;
-  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, L.use_symmetry};
+  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, (const AcaTile *)L.chunks_i,
+                 (const AcaTile *)L.chunks_j, L.first_i, L.first_j, L.w_j, L.w_i, L.max_iter,
+                 L.chunk};
   if (L.pool_f64)
-    hipLaunchKernelGGL(aca_apply_kernel<double>, dim3(L.nlist), dim3(256), 0, st, A,
-                       (const double *)L.C, (const double *)L.Q);
+    apply_launch<double>(A, L, st);
   else
-    hipLaunchKernelGGL(aca_apply_kernel<float>, dim3(L.nlist), dim3(256), 0, st, A,
-                       (const float *)L.C, (const float *)L.Q);
+    apply_launch<float>(A, L, st);
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -171,8 +172,15 @@ struct hh_model {
   DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r, tiles_c, tiles_r,
       applylist, Y, Cpool, Qpool;
   PBuf h_active, h_ybase_c, h_ybase_r, h_tiles_c, h_tiles_r, h_stop, h_list;
+  // chunked step / apply lists and per-chunk partials
+  DBuf chunks_c, chunks_r, cfirst_c, cfirst_r, part_c, part_r, x_c, x_r, invb;
+  DBuf a_chunks_i, a_chunks_j, a_first_i, a_first_j, w_i, w_j;
+  PBuf h_chunks_c, h_chunks_r, h_cfirst_c, h_cfirst_r, h_achunks_i, h_achunks_j, h_afirst_i,
+      h_afirst_j;
   // persistent far-field workspace
   int far_mode = 1;  // 1: batched iterations (default), 0: persistent kernel
+  bool verbose = false;  // HH_VERBOSE=1: far-field progress lines on stderr
+  int aca_chunk = kAcaChunk;  // HH_ACA_CHUNK: items per ACA step chunk (1..kAcaChunk)
   DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
   bool p_ready = false;
   int64_t p_slotC = 0, p_slotQ = 0, p_slotY = 0;
@@ -187,14 +195,18 @@ struct hh_model {
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
                     &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
                     &active_r, &ybase_c, &ybase_r, &tiles_c, &tiles_r, &applylist, &Y, &Cpool,
-                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u})
+                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &chunks_c, &chunks_r,
+                    &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r, &invb, &a_chunks_i,
+                    &a_chunks_j, &a_first_i, &a_first_j, &w_i, &w_j})
       b->release();
     for (auto &b : wf32) b.release();
     for (auto &b : wf64) b.release();
     for (auto &b : wh_f) b.release();
     for (auto &b : wh_bf16) b.release();
     for (auto &b : wh_f16) b.release();
-    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_tiles_c, &h_tiles_r, &h_stop, &h_list})
+    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_tiles_c, &h_tiles_r, &h_stop, &h_list,
+                    &h_chunks_c, &h_chunks_r, &h_cfirst_c, &h_cfirst_r, &h_achunks_i, &h_achunks_j,
+                    &h_afirst_i, &h_afirst_j})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -207,33 +219,50 @@ struct hh_model {
   }
 
   // ---- weights in fragment order ----------------------------------------
+  // Block entry e = 3 row + col carried by layer-3 output row d, or -1.  The
+  // kernels read row `row` of a pair's block from lane group g = row,
+  // register r = col, i.e. output row 4g + r (4r + g for the f64 MFMA).
+  static int entry_of_row(int d, bool f64) {
+    const int g = f64 ? d % 4 : d / 4, r = f64 ? d / 4 : d % 4;
+    return (g < 3 && r < 3) ? 3 * g + r : -1;
+  }
+  // The FP32 and 16-bit kernels evaluate tanh x as 1 - 2/(2^x' + 1) with
+  // x' = 2 x log2 e; scaling W1, b1, W2, b2 by 2 log2 e here saves the
+  // multiply per activation.  The FP64 kernels use unscaled weights.
+  static double act_scale(bool f64) { return f64 ? 1.0 : 2.0 * 1.4426950408889634; }
+
   template <class R>
   void prep_f(DBuf (&dst)[5]) {
     const float *W1 = w.data(), *B1 = W1 + 384, *W2 = B1 + 128, *B2 = W2 + 65536, *W3 = B2 + 512,
                 *B3 = W3 + 4608;
+    const bool f64 = sizeof(R) == 8;
+    const double sc = act_scale(f64);
     std::vector<R> w1b(512), w2f(32 * 32 * 64), b2(512), w3f(32 * 4 * 64), b3(16, (R)0);
     for (int k = 0; k < 128; k++) {
-      for (int d = 0; d < 3; d++) w1b[4 * k + d] = (R)W1[3 * k + d];
-      w1b[4 * k + 3] = (R)B1[k];
+      for (int d = 0; d < 3; d++) w1b[4 * k + d] = (R)(sc * W1[3 * k + d]);
+      w1b[4 * k + 3] = (R)(sc * B1[k]);
     }
     for (int mt = 0; mt < 32; mt++)
       for (int s = 0; s < 32; s++)
         for (int l = 0; l < 64; l++) {
           const int p = l & 15, g = l >> 4;
-          w2f[(mt * 32 + s) * 64 + l] = (R)W2[(16 * mt + p) * 128 + 4 * s + g];
+          w2f[(mt * 32 + s) * 64 + l] = (R)(sc * W2[(16 * mt + p) * 128 + 4 * s + g]);
         }
-    for (int k = 0; k < 512; k++) b2[k] = (R)B2[k];
+    for (int k = 0; k < 512; k++) b2[k] = (R)(sc * B2[k]);
     // layer-2 output register r of lane group g is feature 16mt + row(g, r):
     // 4g + r for the f32 MFMA, 4r + g for the f64 MFMA
-    const bool f64 = sizeof(R) == 8;
     for (int mt = 0; mt < 32; mt++)
       for (int r = 0; r < 4; r++)
         for (int l = 0; l < 64; l++) {
           const int p = l & 15, g = l >> 4;
           const int row = f64 ? 4 * r + g : 4 * g + r;
-          w3f[(mt * 4 + r) * 64 + l] = p < 9 ? (R)W3[p * 512 + 16 * mt + row] : (R)0;
+          const int e = entry_of_row(p, f64);
+          w3f[(mt * 4 + r) * 64 + l] = e >= 0 ? (R)W3[e * 512 + 16 * mt + row] : (R)0;
         }
-    for (int o = 0; o < 9; o++) b3[o] = (R)B3[o];
+    for (int d = 0; d < 16; d++) {
+      const int e = entry_of_row(d, f64);
+      b3[d] = e >= 0 ? (R)B3[e] : (R)0;
+    }
     upload(dst[0], w1b.data(), w1b.size() * sizeof(R));
     upload(dst[1], w2f.data(), w2f.size() * sizeof(R));
     upload(dst[2], b2.data(), b2.size() * sizeof(R));
@@ -243,20 +272,23 @@ struct hh_model {
 
   void prep_h(DBuf (&dst)[2], uint16_t (*cvt)(float)) {
     const float *W2 = w.data() + 512, *W3 = W2 + 65536 + 512;
+    const double sc = act_scale(false);
     std::vector<uint16_t> w2f(32 * 4 * 64 * 8), w3f(16 * 64 * 8);
     for (int mt = 0; mt < 32; mt++)
       for (int s = 0; s < 4; s++)
         for (int l = 0; l < 64; l++)
           for (int j = 0; j < 8; j++) {
             const int p = l & 15, g = l >> 4;
-            w2f[((mt * 4 + s) * 64 + l) * 8 + j] = cvt(W2[(16 * mt + p) * 128 + 32 * s + 8 * g + j]);
+            w2f[((mt * 4 + s) * 64 + l) * 8 + j] =
+                cvt((float)(sc * W2[(16 * mt + p) * 128 + 32 * s + 8 * g + j]));
           }
     for (int q = 0; q < 16; q++)
       for (int l = 0; l < 64; l++)
         for (int j = 0; j < 8; j++) {
           const int p = l & 15, g = l >> 4;
           const int feat = j < 4 ? 32 * q + 4 * g + j : 32 * q + 16 + 4 * g + (j - 4);
-          w3f[(q * 64 + l) * 8 + j] = p < 9 ? cvt(W3[p * 512 + feat]) : (uint16_t)0;
+          const int e = entry_of_row(p, false);
+          w3f[(q * 64 + l) * 8 + j] = e >= 0 ? cvt(W3[e * 512 + feat]) : (uint16_t)0;
         }
     upload(dst[0], w2f.data(), w2f.size() * 2);
     upload(dst[1], w3f.data(), w3f.size() * 2);
@@ -266,14 +298,19 @@ struct hh_model {
     w.assign(wts, wts + 71177);
     prep_f<float>(wf32);
     prep_f<double>(wf64);
-    // 16-bit path: FP32 layer 1 and biases shared with the FP32 layout
+    // 16-bit path: FP32 layer 1 and biases in the FP32 layout
     const float *W1 = w.data(), *B1 = W1 + 384, *B2 = B1 + 128 + 65536, *B3 = B2 + 512 + 4608;
-    std::vector<float> w1b(512), b2(B2, B2 + 512), b3(16, 0.0f);
+    const double sc = act_scale(false);
+    std::vector<float> w1b(512), b2(512), b3(16, 0.0f);
     for (int k = 0; k < 128; k++) {
-      for (int d = 0; d < 3; d++) w1b[4 * k + d] = W1[3 * k + d];
-      w1b[4 * k + 3] = B1[k];
+      for (int d = 0; d < 3; d++) w1b[4 * k + d] = (float)(sc * W1[3 * k + d]);
+      w1b[4 * k + 3] = (float)(sc * B1[k]);
     }
-    for (int o = 0; o < 9; o++) b3[o] = B3[o];
+    for (int k = 0; k < 512; k++) b2[k] = (float)(sc * B2[k]);
+    for (int d = 0; d < 16; d++) {
+      const int e = entry_of_row(d, false);
+      b3[d] = e >= 0 ? B3[e] : 0.0f;
+    }
     upload(wh_f[0], w1b.data(), 512 * 4);
     upload(wh_f[1], b2.data(), 512 * 4);
     upload(wh_f[2], b3.data(), 16 * 4);
@@ -411,6 +448,69 @@ struct hh_model {
   }
 
   // ---- far field: batched ACA -------------------------------------------
+  // Low-rank apply of the stopped blocks `done` (indices into hb), chunked
+  // over I and J like the steps.
+  void apply_blocks(const std::vector<FarBlock> &hb, const std::vector<int32_t> &done, int MI,
+                    bool f64, hipStream_t st) {
+    const int nl = (int)done.size();
+    int64_t nci = 0, ncj = 0;
+    for (int32_t b : done) {
+      nci += (hb[(size_t)b].m + aca_chunk - 1) / aca_chunk;
+      ncj += (hb[(size_t)b].n + aca_chunk - 1) / aca_chunk;
+    }
+    h_list.reserve((size_t)nl * 4);
+    applylist.reserve((size_t)nl * 4);
+    h_achunks_i.reserve((size_t)nci * 8);
+    a_chunks_i.reserve((size_t)nci * 8);
+    h_achunks_j.reserve((size_t)ncj * 8);
+    a_chunks_j.reserve((size_t)ncj * 8);
+    for (auto *b : {&h_afirst_i, &h_afirst_j}) b->reserve((size_t)(nl + 1) * 4);
+    for (auto *b : {&a_first_i, &a_first_j}) b->reserve((size_t)(nl + 1) * 4);
+    w_i.reserve((size_t)std::max<int64_t>(nci, 1) * MI * 3 * 8);
+    w_j.reserve((size_t)std::max<int64_t>(ncj, 1) * MI * 3 * 8);
+    int32_t *hl = h_list.as<int32_t>(), *ci = h_achunks_i.as<int32_t>(),
+            *cj = h_achunks_j.as<int32_t>(), *fi = h_afirst_i.as<int32_t>(),
+            *fj = h_afirst_j.as<int32_t>();
+    int64_t ki = 0, kj = 0;
+    for (int a = 0; a < nl; a++) {
+      const FarBlock &f = hb[(size_t)done[(size_t)a]];
+      hl[a] = done[(size_t)a];
+      fi[a] = (int32_t)ki;
+      fj[a] = (int32_t)kj;
+      for (int32_t o = 0; o < f.m; o += aca_chunk) ci[2 * ki] = a, ci[2 * ki + 1] = o, ki++;
+      for (int32_t o = 0; o < f.n; o += aca_chunk) cj[2 * kj] = a, cj[2 * kj + 1] = o, kj++;
+    }
+    fi[nl] = (int32_t)ki;
+    fj[nl] = (int32_t)kj;
+    HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_chunks_i.p, ci, (size_t)nci * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_chunks_j.p, cj, (size_t)ncj * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_first_i.p, fi, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_first_j.p, fj, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
+    AcaApplyLaunch al{};
+    al.blk = blk.as<FarBlock>();
+    al.list = applylist.as<int32_t>();
+    al.it = it.as<int32_t>();
+    al.fp = fp.as<float4>();
+    al.u = u.as<double>();
+    al.use_symmetry = use_symmetry;
+    al.nlist = nl;
+    al.pool_f64 = f64;
+    al.C = Cpool.p;
+    al.Q = Qpool.p;
+    al.chunks_i = a_chunks_i.p;
+    al.chunks_j = a_chunks_j.p;
+    al.first_i = a_first_i.as<int32_t>();
+    al.first_j = a_first_j.as<int32_t>();
+    al.nchunks_i = nci;
+    al.nchunks_j = ncj;
+    al.w_i = w_i.as<double>();
+    al.w_j = w_j.as<double>();
+    al.max_iter = MI;
+    al.chunk = aca_chunk;
+    HIPCHK(launch_aca_apply(al, st));
+  }
+
   void far_field(hipStream_t st) {
     if (far_mode == 0) {
       far_field_persistent(st);
@@ -472,22 +572,47 @@ struct hh_model {
       h_list.reserve((size_t)nb * 4);
       std::vector<int32_t> act((size_t)nb);
       for (int b = 0; b < nb; b++) act[(size_t)b] = b;
+      int iter = 0;
+      auto last = std::chrono::steady_clock::now();
       while (!act.empty()) {
         const int na = (int)act.size();
-        // tiles of 16 items for the column (i in I) and row (j in J) passes
-        int64_t ntc = 0, ntr = 0;
-        for (int a = 0; a < na; a++) {
-          ntc += (hb[(size_t)act[(size_t)a]].m + 15) / 16;
-          ntr += (hb[(size_t)act[(size_t)a]].n + 15) / 16;
+        if (verbose) {
+          const auto now = std::chrono::steady_clock::now();
+          if (iter == 0 || std::chrono::duration<double>(now - last).count() > 10.0) {
+            std::fprintf(stderr, "[hh] far blocks %lld..%lld of %lld: iteration %d, %d active\n",
+                         (long long)b0, (long long)b1, (long long)nfar, iter, na);
+            std::fflush(stderr);
+            last = now;
+          }
         }
-        h_tiles_c.reserve((size_t)ntc * 8);
-        h_tiles_r.reserve((size_t)ntr * 8);
-        tiles_c.reserve((size_t)ntc * 8);
-        tiles_r.reserve((size_t)ntr * 8);
+        iter++;
+        // tiles of 16 items (MLP passes) and chunks of aca_chunk items (steps)
+        // for the column (i in I) and row (j in J) sides
+        int64_t ntc = 0, ntr = 0, ncc = 0, ncr = 0;
+        for (int a = 0; a < na; a++) {
+          const FarBlock &f = hb[(size_t)act[(size_t)a]];
+          ntc += (f.m + 15) / 16;
+          ntr += (f.n + 15) / 16;
+          ncc += (f.m + aca_chunk - 1) / aca_chunk;
+          ncr += (f.n + aca_chunk - 1) / aca_chunk;
+        }
+        for (auto *b : {&h_tiles_c, &h_tiles_r, &h_chunks_c, &h_chunks_r})
+          b->reserve((size_t)std::max(std::max(ntc, ntr), std::max(ncc, ncr)) * 8);
+        for (auto *b : {&tiles_c, &tiles_r, &chunks_c, &chunks_r})
+          b->reserve((size_t)std::max(std::max(ntc, ntr), std::max(ncc, ncr)) * 8);
+        for (auto *b : {&h_cfirst_c, &h_cfirst_r}) b->reserve((size_t)(na + 1) * 4);
+        for (auto *b : {&cfirst_c, &cfirst_r}) b->reserve((size_t)(na + 1) * 4);
+        part_c.reserve((size_t)ncc * kAcaPartBytes);
+        part_r.reserve((size_t)ncr * kAcaPartBytes);
+        x_c.reserve((size_t)ncc * MI * 9 * 8);
+        x_r.reserve((size_t)ncr * MI * 9 * 8);
+        invb.reserve((size_t)na * 9 * 8);
         int32_t *tc = h_tiles_c.as<int32_t>(), *tr = h_tiles_r.as<int32_t>();
+        int32_t *cc = h_chunks_c.as<int32_t>(), *cr = h_chunks_r.as<int32_t>();
+        int32_t *fc = h_cfirst_c.as<int32_t>(), *fr = h_cfirst_r.as<int32_t>();
         int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
         int32_t *ha = h_active.as<int32_t>();
-        int64_t sc = 0, sr = 0, ic = 0, ir = 0;
+        int64_t sc = 0, sr = 0, ic = 0, ir = 0, kc = 0, kr = 0;
         for (int a = 0; a < na; a++) {
           const FarBlock &f = hb[(size_t)act[(size_t)a]];
           ha[a] = act[(size_t)a];
@@ -497,12 +622,22 @@ struct hh_model {
           sr += f.n;
           for (int32_t o = 0; o < f.m; o += 16) tc[2 * ic] = a, tc[2 * ic + 1] = o, ic++;
           for (int32_t o = 0; o < f.n; o += 16) tr[2 * ir] = a, tr[2 * ir + 1] = o, ir++;
+          fc[a] = (int32_t)kc;
+          fr[a] = (int32_t)kr;
+          for (int32_t o = 0; o < f.m; o += aca_chunk) cc[2 * kc] = a, cc[2 * kc + 1] = o, kc++;
+          for (int32_t o = 0; o < f.n; o += aca_chunk) cr[2 * kr] = a, cr[2 * kr + 1] = o, kr++;
         }
+        fc[na] = (int32_t)kc;
+        fr[na] = (int32_t)kr;
         HIPCHK(hipMemcpyAsync(active_c.p, ha, (size_t)na * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(ybase_c.p, yc, (size_t)na * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(ybase_r.p, yr, (size_t)na * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(tiles_c.p, tc, (size_t)ntc * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(tiles_r.p, tr, (size_t)ntr * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(chunks_c.p, cc, (size_t)ncc * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(chunks_r.p, cr, (size_t)ncr * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cfirst_c.p, fc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cfirst_r.p, fr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
         AcaMlpLaunch ml{};
         ml.prec = prec;
         ml.xp = xp.as<float4>();
@@ -520,6 +655,8 @@ struct hh_model {
         AcaStepLaunch sl{};
         sl.blk = blk.as<FarBlock>();
         sl.active = active_c.as<int32_t>();
+        sl.ybase_c = ybase_c.as<int64_t>();
+        sl.ybase_r = ybase_r.as<int64_t>();
         sl.Y = Y.as<float>();
         sl.it = it.as<int32_t>();
         sl.ipiv = ipiv.as<int32_t>();
@@ -529,7 +666,19 @@ struct hh_model {
         sl.stop = stop.as<int32_t>();
         sl.max_iter = MI;
         sl.eps2 = epsilon * epsilon;
+        sl.chunks_c = chunks_c.p;
+        sl.chunks_r = chunks_r.p;
+        sl.cfirst_c = cfirst_c.as<int32_t>();
+        sl.cfirst_r = cfirst_r.as<int32_t>();
+        sl.part_c = part_c.p;
+        sl.part_r = part_r.p;
+        sl.x_c = x_c.as<double>();
+        sl.x_r = x_r.as<double>();
+        sl.inv = invb.as<double>();
+        sl.chunk = aca_chunk;
         sl.nactive = na;
+        sl.nchunks_c = ncc;
+        sl.nchunks_r = ncr;
         sl.pool_f64 = f64;
         sl.C = Cpool.p;
         sl.Q = Qpool.p;
@@ -539,7 +688,6 @@ struct hh_model {
         ml.tiles = tiles_c.p;
         ml.ntiles = ntc;
         HIPCHK(launch_aca_mlp(ml, st));
-        sl.ybase = ybase_c.as<int64_t>();
         HIPCHK(launch_aca_steps(sl, 0, st));
         // row pass, residual, column pivot, norms, stop test
         ml.row_pass = 1;
@@ -547,35 +695,13 @@ struct hh_model {
         ml.tiles = tiles_r.p;
         ml.ntiles = ntr;
         HIPCHK(launch_aca_mlp(ml, st));
-        sl.ybase = ybase_r.as<int64_t>();
         HIPCHK(launch_aca_steps(sl, 1, st));
         HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         const int32_t *hs = h_stop.as<int32_t>();
-        int32_t *hl = h_list.as<int32_t>();
-        int nl = 0;
-        std::vector<int32_t> keep;
-        for (int a = 0; a < na; a++) {
-          if (hs[a])
-            hl[nl++] = act[(size_t)a];
-          else
-            keep.push_back(act[(size_t)a]);
-        }
-        if (nl) {
-          HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
-          AcaApplyLaunch al{};
-          al.blk = blk.as<FarBlock>();
-          al.list = applylist.as<int32_t>();
-          al.it = it.as<int32_t>();
-          al.fp = fp.as<float4>();
-          al.u = u.as<double>();
-          al.use_symmetry = use_symmetry;
-          al.nlist = nl;
-          al.pool_f64 = f64;
-          al.C = Cpool.p;
-          al.Q = Qpool.p;
-          HIPCHK(launch_aca_apply(al, st));
-        }
+        std::vector<int32_t> done, keep;
+        for (int a = 0; a < na; a++) (hs[a] ? done : keep).push_back(act[(size_t)a]);
+        if (!done.empty()) apply_blocks(hb, done, MI, f64, st);
         act.swap(keep);
       }
       std::vector<int32_t> ranks((size_t)nb);
@@ -780,6 +906,12 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
     if (n <= 0 || !coord) throw HHError(HH_ERR_ARG, "need n > 0 coordinates");
     if (n > (int64_t)INT32_MAX / 4) throw HHError(HH_ERR_ARG, "too many particles");
     if (block_size <= 0) throw HHError(HH_ERR_ARG, "block size must be positive");
+    int chunk = kAcaChunk;
+    if (const char *ch = std::getenv("HH_ACA_CHUNK")) {
+      chunk = std::atoi(ch);
+      if (chunk < 1 || chunk > kAcaChunk)
+        throw HHError(HH_ERR_ARG, "HH_ACA_CHUNK must be in 1.." + std::to_string(kAcaChunk));
+    }
     ensure_device();
     hh_model *m = new hh_model();
     m->n = n;
@@ -790,6 +922,8 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
     }
     if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
+    if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
+    m->aca_chunk = chunk;
     if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
       m->far_pool_bytes = (size_t)(std::atof(gb) * (double)(1ull << 30));
     *out = m;

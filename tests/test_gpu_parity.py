@@ -141,3 +141,29 @@ def test_chosen_form_accuracy(model_small, prec):
     e32 = O.rel_l2(u32[tg], ref)
     print(prec, "error vs fp64 blocks", e, "fp32 path", e32)
     assert e <= 1e-3
+
+
+
+@pytest.mark.parametrize("chunk", [16, 33])
+def test_dot_fp32_parity_multichunk_blocks(model_small, xavier_w, monkeypatch, chunk):
+    """ACA steps and apply cut far blocks into chunks of HH_ACA_CHUNK items;
+    with small chunks every large far block spans several chunks and the
+    per-chunk partials are combined across workgroups."""
+    monkeypatch.setenv("HH_ACA_CHUNK", str(chunk))
+    x, f, om = model_small["x"], model_small["f"], model_small["om"]
+    exp = om.export()
+    sz = exp["tree"][:, 3] - exp["tree"][:, 2]
+    assert sz[exp["far"]].max() > 2 * chunk
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50)
+    m.update_coord()
+    err = O.rel_l2(m.dot(f), om.dot(xavier_w, f))
+    print(f"chunk {chunk}: fp32 dot rel err vs oracle", err)
+    assert err <= 1e-5
+    np.testing.assert_array_equal(m.far_ranks(), om.far_ranks())
+    m.set(precision="fp16")
+    tg = np.arange(0, x.shape[0], 5)
+    e = O.rel_l2(m.dot(f)[tg], m.check_fp64(f, tg, 0))
+    print(f"chunk {chunk}: fp16 error vs fp64 blocks", e)
+    assert e <= 1e-3
