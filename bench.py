@@ -3,14 +3,16 @@
 Workload (default): 1,000,000 spheres of radius 1 in 200 Gaussian blobs
 (sigma = 15), blob centres uniform in the cube sized for a 10% mean volume
 fraction, minimum centre distance 2.05, seed 2; gravity forcing (0,0,-1);
-leaf 256, ACA epsilon 1e-4, at most 32 ACA iterations; pair-function
-weights: seeded Xavier initialisation of the 3-128-512-9 MLP.
+leaf 256, ACA epsilon 1e-4, at most 48 ACA iterations, far blocks of at most
+2048 particles (max_relative_coord); pair-function weights: seeded Xavier
+initialisation of the 3-128-512-9 MLP.
 
 A step is one matvec (HignnModel.dot) on a built tree with forces and
 velocities resident in device memory.  ``value`` is particle-updates/s of
-the chosen form (16-bit MLP on MFMA, FP32 accumulation, FP32 ACA); the plain
-FP32 path is timed in the same run, and both are checked against an FP64
-evaluation of the same block structure on a random subsample of targets.
+the chosen form (16-bit MLP on MFMA with FP32 accumulation, FP32 ACA
+factors, FP64 norms and velocity sums); the plain FP32 path is timed in the
+same run, and both are checked against an FP64 evaluation of the same block
+structure (dense far blocks) on a random subsample of targets.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
@@ -67,15 +69,17 @@ def cpu_baseline(x, w, f, leaf, eps, max_iter, mrc, exp, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--prec", default="bf16", choices=["bf16", "fp16"])
+    ap.add_argument("--prec", default="fp16", choices=["bf16", "fp16"])
+    ap.add_argument("--other-16bit", type=int, default=1,
+                    help="also time one step of the other 16-bit format (0: skip)")
     ap.add_argument("--check-targets", type=int, default=2000)
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--max-iter", type=int, default=48)
-    ap.add_argument("--max-relative-coord", type=int, default=512)
+    ap.add_argument("--max-relative-coord", type=int, default=2048)
     ap.add_argument("--epsilon", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -125,7 +129,7 @@ def main():
     for i in range(args.warmup):
         dt = run(args.prec)
         log(f"[bench] warmup {i} {args.prec}: {dt:.3f}s")
-    times, tcl, tfar = [], [], []
+    times, tcl, tfar, profs = [], [], [], []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
@@ -134,12 +138,23 @@ def main():
         times.append(dt)
         tcl.append(c)
         tfar.append(fa)
-        log(f"[bench] step {i} {args.prec}: {dt:.3f}s (close {c:.1f} ms, far {fa:.1f} ms)")
+        profs.append(model.last_profile())
+        log(f"[bench] step {i} {args.prec}: {dt:.3f}s (close {c:.1f} ms, far {fa:.1f} ms, "
+            f"far MLP {profs[-1]['far_mlp'][0]:.1f} ms)")
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     u_chosen = u_dev.cpu().numpy().copy()
     ranks = model.far_ranks()
     e_near, e_far = work_counts(exp, ranks)
+
+    # the other 16-bit format, one step
+    other = "bf16" if args.prec == "fp16" else "fp16"
+    u_other, other_rate = None, None
+    if args.other_16bit:
+        dt = run(other)
+        other_rate = n / dt
+        u_other = u_dev.cpu().numpy().copy()
+        log(f"[bench] {other} step: {dt:.3f}s")
 
     # plain FP32 path in the same run
     f32_times = []
@@ -148,11 +163,10 @@ def main():
         f32_times.append(dt)
         c, fa, _ = model.last_timings()
         log(f"[bench] fp32 step {i}: {dt:.3f}s (close {c:.1f} ms, far {fa:.1f} ms)")
-    u_fp32 = u_dev.cpu().numpy().copy()
-    ranks32 = model.far_ranks()
+    u_fp32 = u_dev.cpu().numpy().copy() if f32_times else None
 
     # accuracy: FP64 evaluation of the same blocks at random targets
-    err = err32 = None
+    err = err32 = err_other = None
     if args.check_targets > 0:
         g = np.random.default_rng(7)
         tg = np.sort(g.choice(n, size=min(args.check_targets, n), replace=False))
@@ -161,22 +175,36 @@ def main():
         t_chk = time.time() - t0
 
         def rel(u):
+            if u is None:
+                return None
             return float(np.sqrt(np.sum((u[tg] - ref) ** 2) / np.sum(ref ** 2)))
 
-        err, err32 = rel(u_chosen), rel(u_fp32)
-        log(f"[bench] fp64 check on {tg.size} targets ({t_chk:.1f}s): {args.prec} err {err:.3e}, "
-            f"fp32 err {err32:.3e}")
+        err, err32, err_other = rel(u_chosen), rel(u_fp32), rel(u_other)
+        log(f"[bench] fp64 check on {tg.size} targets ({t_chk:.1f}s): {args.prec} err {err}, "
+            f"{other} err {err_other}, fp32 err {err32}")
 
     ms = 1000.0 * t_total / args.steps
     value = n / (t_total / args.steps)
     fp32_rate = n / float(np.median(f32_times)) if f32_times else None
-    # roofline of the dominant kernel: the near-field MLP kernel
-    t_close_s = float(np.median(tcl)) / 1000.0
-    achieved = MLP_FLOP * e_near / t_close_s / 1e12
+    # roofline of the dominant pair-function kernel, from HIP events on the
+    # model's stream around each launch (near kernel; ACA column/row passes)
+    kern = {"near": ("near_h_kernel", 0), "far_mlp": ("aca_mlp_h_kernel", 1)}
+    tot = {k: sum(p[k][0] for p in profs) for k in kern}
+    dom = max(tot, key=tot.get)
+    ms_k = sum(p[dom][0] for p in profs)
+    pairs_k = sum(p[dom][1] for p in profs)
+    launches_k = sum(p[dom][2] for p in profs)
+    achieved = MLP_FLOP * pairs_k / (ms_k / 1000.0) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[args.prec],
             "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
-            "kernel": f"near_h_kernel<{args.prec}>", "flop_per_launch": MLP_FLOP * e_near,
-            "launch_ms": round(1000 * t_close_s, 2)}
+            "kernel": f"{kern[dom][0]}<{args.prec}>",
+            "flop_per_launch": int(MLP_FLOP * pairs_k / max(1, launches_k)),
+            "avg_launch_ms": round(ms_k / max(1, launches_k), 3),
+            "launches_per_step": int(launches_k / args.steps),
+            "near_kernel_tflops": round(MLP_FLOP * sum(p["near"][1] for p in profs)
+                                        / max(1e-9, tot["near"] / 1000.0) / 1e12, 2),
+            "far_mlp_tflops": round(MLP_FLOP * sum(p["far_mlp"][1] for p in profs)
+                                    / max(1e-9, tot["far_mlp"] / 1000.0) / 1e12, 2)}
 
     cpu = None
     if not args.no_cpu:
@@ -193,6 +221,7 @@ def main():
         except Exception as e:  # the baseline must not sink the GPU result
             log(f"[bench] cpu baseline failed: {e}")
 
+    far_mlp_ms = tot["far_mlp"] / args.steps
     line = {
         "metric": "matvec particle-updates/s @N=1M: BF16+surrogate vs plain FP32, rel err<=1e-3",
         "value": round(value, 2),
@@ -209,16 +238,21 @@ def main():
         "config": {
             "workload": f"gaussian blobs N={n} (200 blobs, sigma 15, 10% mean volume fraction, "
                         f"seed 2), gravity; leaf {leaf}, eps {eps}, max_iter {max_iter}, "
-                        f"max_relative_coord {args.max_relative_coord}",
-            "far_entries": f"{args.prec} MLP (surrogate not used)",
+                        f"max_relative_coord {args.max_relative_coord}; Xavier(seed 0) weights",
+            "chosen_form": f"{args.prec} MLP on MFMA (FP32 accumulate), FP32 ACA factors, "
+                           f"FP64 norms and sums; far entries from the full MLP (surrogate not used)",
             "fp32_value": round(fp32_rate, 2) if fp32_rate else None,
             "speedup_vs_fp32": round(value / fp32_rate, 3) if fp32_rate else None,
-            "rel_err_fp64": err, "rel_err_fp64_fp32_path": err32,
+            f"{other}_value": round(other_rate, 2) if other_rate else None,
+            "rel_err_fp64": err, f"rel_err_fp64_{other}": err_other,
+            "rel_err_fp64_fp32_path": err32,
             "check_targets": args.check_targets,
             "e_near": e_near, "e_far": e_far,
             "mean_far_rank": float(ranks.mean()) if ranks.size else 0.0,
             "update_coord_s": round(t_update, 3),
             "close_ms": round(float(np.median(tcl)), 2), "far_ms": round(float(np.median(tfar)), 2),
+            "far_mlp_ms": round(far_mlp_ms, 2),
+            "far_bookkeeping_ms": round(float(np.median(tfar)) - far_mlp_ms, 2),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -52,6 +52,7 @@ def lib():
             "hh_model_check_fp64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp],
             "hh_model_eval_pair": [c_vp, c_vp, c_i64, c_vp, c_int],
             "hh_model_last_timings": [c_vp, c_vp],
+            "hh_model_last_profile": [c_vp, c_vp, c_vp, c_vp],
             "hh_tree_build": [c_vp, c_i64, c_int, c_int, c_i64, c_vp],
             "hh_tree_counts": [c_vp, c_vp],
             "hh_tree_export": [c_vp] * 7,
@@ -187,6 +188,14 @@ class Model:
         t = np.zeros(3, np.float64)
         _chk(lib().hh_model_last_timings(self.h, _p(t)))
         return tuple(float(v) for v in t)
+
+    def profile(self):
+        """(ms, pair evaluations, launches) of [near kernel, ACA passes]."""
+        ms = np.zeros(2, np.float64)
+        pairs = np.zeros(2, np.int64)
+        launches = np.zeros(2, np.int64)
+        _chk(lib().hh_model_last_profile(self.h, _p(ms), _p(pairs), _p(launches)))
+        return ms, pairs, launches
 
 
 def header_symbols():

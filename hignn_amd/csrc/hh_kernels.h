@@ -42,7 +42,9 @@ struct AcaMlpLaunch {
   const int32_t *ipiv;
   const int32_t *jpiv;
   const int64_t *ybase;
-  const void *tiles;  // (slot, offset) pairs
+  const int32_t *tfirst;  // slot -> first 16-item tile (nactive + 1)
+  int nactive;
+  void *tiles;            // ntiles (slot, first item) pairs, filled from tfirst
   int64_t ntiles;
   int max_iter;
   int row_pass;
@@ -54,8 +56,9 @@ struct AcaMlpLaunch {
 };
 
 // Far-field ACA steps and apply work on chunks of at most kAcaChunk items of
-// a block (rows of I or columns of J); a chunk is an AcaTile (slot, first item).
-constexpr int kAcaChunk = 1024;
+// a block (rows of I or columns of J); kernels find a chunk's (slot, first
+// item) from per-slot prefix counts.
+constexpr int kAcaChunk = 256;
 constexpr size_t kAcaPartBytes = 64;  // per-chunk partials (arg-max + 6 Gram sums)
 
 struct AcaStepLaunch {
@@ -71,8 +74,8 @@ struct AcaStepLaunch {
   int32_t *stop;
   int max_iter;
   double eps2;
-  const void *chunks_c, *chunks_r;  // column-side (I) and row-side (J) chunks
-  const int32_t *cfirst_c, *cfirst_r;  // slot -> first chunk (nactive + 1)
+  const int32_t *cfirst_c, *cfirst_r;  // slot -> first column-side (I) / row-side (J) chunk
+                                       // (nactive + 1 entries)
   void *part_c, *part_r;            // kAcaPartBytes per chunk
   double *x_c, *x_r;                // max_iter x 9 per chunk
   double *inv;                      // 9 per slot
@@ -95,8 +98,7 @@ struct AcaApplyLaunch {
   int pool_f64;
   const void *C;
   const void *Q;
-  const void *chunks_i, *chunks_j;  // (apply slot, first item) over I and J
-  const int32_t *first_i, *first_j;  // apply slot -> first chunk (nlist + 1)
+  const int32_t *first_i, *first_j;  // apply slot -> first chunk over I / J (nlist + 1)
   int64_t nchunks_i, nchunks_j;
   double *w_i, *w_j;                // max_iter x 3 per chunk
   int max_iter;

@@ -399,8 +399,8 @@ __device__ __forceinline__ void stage_fragments(const WH &W, H *lw2, H *lw3) {
   __syncthreads();
 }
 
-// 16-bit near kernel: persistent workgroups of kNearHThreads threads (two
-// waves per SIMD) share the LDS-staged W2/W3 fragments.
+// 16-bit near and ACA MLP kernels: persistent workgroups of kNearHThreads
+// threads (two waves per SIMD) share the LDS-staged W2/W3 fragments.
 constexpr int kNearHThreads = 512;
 constexpr int kNearHTiles = 4;
 
@@ -423,6 +423,21 @@ struct AcaTile {
   int32_t off;   // first item (row or column index inside the block)
 };
 
+// Tile or chunk q of a list in which slot s owns the units first[s] ..
+// first[s + 1] - 1 (first ascending, every slot at least one unit) and unit
+// u of a slot starts at item u * items.
+__device__ __forceinline__ AcaTile locate(const int32_t *first, int nslots, int32_t q, int items) {
+  int lo = 0, hi = nslots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= q)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return AcaTile{lo, (q - first[lo]) * items};
+}
+
 struct AcaMlpArgs {
   const float4 *xp;
   const FarBlock *blk;
@@ -431,12 +446,21 @@ struct AcaMlpArgs {
   const int32_t *ipiv;    // per block [maxIter]
   const int32_t *jpiv;    // per block [maxIter+1]
   const int64_t *ybase;   // per slot: first row of Y
-  const AcaTile *tiles;
+  const AcaTile *tiles;    // (slot, first item) of each 16-item tile
   int64_t ntiles;
   int max_iter;
   int row_pass;           // 0: column pass (items i in I), 1: row pass (j in J)
   float *Y;               // (items, 9) block entries M (not symmetrised)
 };
+
+// Tile list of an ACA pass from the per-slot prefix counts: slot s owns
+// tiles first[s] .. first[s + 1] - 1, tile u of a slot starts at item 16u.
+__global__ void expand_tiles_kernel(const int32_t *first, int nslots, AcaTile *tiles) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int t0 = first[s], t1 = first[s + 1];
+  for (int t = t0; t < t1; t++) tiles[t] = AcaTile{s, 16 * (t - t0)};
+}
 
 template <class Core>
 __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
@@ -489,7 +513,7 @@ __global__ __launch_bounds__(256) void aca_mlp_fp32_kernel(AcaMlpArgs A, WF<floa
 }
 
 template <class H>
-__global__ __launch_bounds__(256) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
+__global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
@@ -501,11 +525,11 @@ __global__ __launch_bounds__(256) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
 // ---- chunked per-block steps ------------------------------------------------
 // The items of the active blocks (rows i of I for the column side, columns j
 // of J for the row side) are cut into chunks of at most kAcaChunk items and one
-// workgroup of 256 threads handles one chunk, so a block of 10^5 items spreads
-// over the whole chip instead of one CU.  Per-chunk partial results (pivot
-// arg-max, Gram sums, cross-term sums) are combined per block, in chunk order,
-// by one-workgroup kernels.
-constexpr int kAcaIpt = kAcaChunk / 256;  // items per thread
+// wave handles one chunk, so a block of 10^5 items spreads over the whole chip
+// instead of one CU.  Per-chunk partial results (pivot arg-max, Gram sums,
+// cross-term sums) are combined per block, in chunk order, by one-workgroup
+// kernels.
+constexpr int kAcaIpt = kAcaChunk / 64;  // items per lane
 constexpr double kNegHuge = -1.7976931348623157e308;
 
 struct AcaPart {  // per chunk
@@ -530,14 +554,13 @@ struct AcaStepArgs {
   int32_t *stop;            // per slot
   int max_iter;
   double eps2;
-  const AcaTile *chunks_c;  // (slot, first item) of the column-side chunks
-  const AcaTile *chunks_r;
+  int nactive;
   const int32_t *cfirst_c;  // slot -> first column-side chunk (nactive + 1 entries)
   const int32_t *cfirst_r;
   AcaPart *part_c;          // per column-side chunk
   AcaPart *part_r;
-  double *x_c;              // per chunk: max_iter x 9 cross-term sums
-  double *x_r;
+  double *x_c;              // per chunk: max_iter x 9 column cross-term sums
+  double *x_r;              // per chunk: max_iter x 9 row cross-term sums
   double *inv;              // per slot: inverse of the pivot block (9)
   int chunk;                // items per chunk, 1..kAcaChunk
 };
@@ -611,20 +634,22 @@ __device__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// Sum N per-thread values over a 256-thread workgroup and store the first
-// nvalid sums to dst (sh: 4 x N doubles of LDS).
-template <int N>
-__device__ __forceinline__ void block_sums_store(const double (&v)[N], double *sh, double *dst,
+// Wave sums of 32 per-lane values of a one-wave workgroup, the first nvalid
+// stored to dst.  The values are transposed through LDS (sh: 32 rows of 65
+// doubles, the odd stride spreads the rows over the banks); lanes e and e + 32
+// each add half of row e and one exchange completes the sum.
+__device__ __forceinline__ void wave_sum32_store(const double (&x)[32], double *sh, double *dst,
                                                  int nvalid) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int e = 0; e < N; e++) {
-    const double s = wave_sum(v[e]);
-    if (lane == 0) sh[wv * N + e] = s;
-  }
+  for (int e = 0; e < 32; e++) sh[e * 65 + lane] = x[e];
   __syncthreads();
-  for (int e = threadIdx.x; e < nvalid; e += blockDim.x)
-    dst[e] = ((sh[e] + sh[N + e]) + sh[2 * N + e]) + sh[3 * N + e];
+  const int e = lane & 31, h = lane >> 5;
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < 32; q++) t += sh[e * 65 + 32 * h + q];
+  t += __shfl_xor(t, 32, 64);
+  if (lane < nvalid) dst[lane] = t;
   __syncthreads();
 }
 
@@ -648,64 +673,149 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
   g[5] += q[6] * q[0] + q[7] * q[1] + q[8] * q[2];
 }
 
-// Residual of one item: d -= sum_{l<k} L_l R_l with the 3x3 products of the
-// reference's dot loops (src/FarDot.cpp:487-511, 812-836); side 0 (column):
-// L_l = C_l(i) from the pool, R_l = Q_l(jk) from LDS; side 1 (row):
-// L_l = C_l(ik) from LDS, R_l = Q_l(j) from the pool.
+// ---- residual passes with the cross terms of the previous iteration --------
+// The stop test of iteration k needs the cross terms C_k^T C_l and Q_k Q_l^T,
+// l < k-1 (src/FarDot.cpp:1041-1078), i.e. one more pass over all previous
+// factors.  The residual passes of iteration k+1 read exactly those factors,
+// so they take the cross-term sums of iteration k as well, and the soft stop
+// test of iteration k (nu2 < mu2 eps^2) is decided one iteration late.  A
+// block that stops there has computed one column and row pass too many; its
+// rank and every decision are the reference's.  The hard stops
+// (iteration count, block size) are taken at once.
+
+// Residual of a lane's kAcaIpt items, d -= sum_{l<k} L_l R_l with the 3x3
+// products of the reference's dot loops (src/FarDot.cpp:487-511, 812-836),
+// summed in the pool precision, plus the cross-term sums of iteration k-1
+// for l < k-2 against a = factor k-1 of the same item (per lane in the pool
+// precision over at most kAcaIpt items, across the wave in FP64):
+//   column side: L_l = C_l(i) from the pool, R_l = Q_l(jk) from LDS,
+//                x[9l + 3d1 + d2] = sum_i sum_row C_{k-1}(i)[row][d1] C_l(i)[row][d2]
+//   row side:    L_l = C_l(ik) from LDS, R_l = Q_l(j) from the pool,
+//                x[9l + 3d1 + d2] = sum_j sum_col Q_{k-1}(j)[d1][col] Q_l(j)[d2][col]
+// kCrossLC iterations per wave reduction.  pool points at the lane's first
+// item of C_0 (Q_0); the lane's items are 64 apart.
+constexpr int kCrossLC = 3;
+
 template <class P, int SIDE>
-__device__ __forceinline__ void residual(P (&d)[9], const P *pool, int64_t stride, int k,
-                                         const P (*piv)[9]) {
-  P sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int l = 0; l < k; l++) {
-    P v[9];
-    load9(pool + 9 * (int64_t)l * stride, v);
-    const P *cl = SIDE == 0 ? v : piv[l];
-    const P *ql = SIDE == 0 ? piv[l] : v;
+__device__ __forceinline__ void residual_cross(P (&d)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
+                                               const P *pool, int64_t stride, int k,
+                                               const P (*piv)[9], double *sh, double *X) {
+  const int ncross = k - 2;  // cross terms of iteration k-1: l < k-2
+  P a[kAcaIpt][9], sum[kAcaIpt][9];
 #pragma unroll
-    for (int r = 0; r < 3; r++)
+  for (int s = 0; s < kAcaIpt; s++)
 #pragma unroll
-      for (int cc = 0; cc < 3; cc++) {
-        P &s = sum[3 * r + cc];
-        s += cl[3 * r] * ql[cc];
-        s += cl[3 * r + 1] * ql[3 + cc];
-        s += cl[3 * r + 2] * ql[6 + cc];
+    for (int e = 0; e < 9; e++) {
+      sum[s][e] = (P)0;
+      a[s][e] = (valid[s] && ncross > 0) ? pool[9 * ((int64_t)(k - 1) * stride + 64 * s) + e] : (P)0;
+    }
+  for (int l0 = 0; l0 < k; l0 += kCrossLC) {
+    const int nl = min(kCrossLC, k - l0);
+    const bool cross = l0 < ncross;  // uniform
+    P x[32];
+#pragma unroll
+    for (int e = 0; e < 32; e++) x[e] = (P)0;
+#pragma unroll
+    for (int s = 0; s < kAcaIpt; s++) {
+#pragma unroll
+      for (int c = 0; c < kCrossLC; c++) {
+        if (!valid[s] || c >= nl) continue;
+        P v[9];
+        load9(pool + 9 * ((int64_t)(l0 + c) * stride + 64 * s), v);
+        const P *cl = SIDE == 0 ? v : piv[l0 + c];
+        const P *ql = SIDE == 0 ? piv[l0 + c] : v;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int cc = 0; cc < 3; cc++) {
+            P &t = sum[s][3 * r + cc];
+            if (sizeof(P) == 8) {  // FP64 pools: the oracle's rounding, no contraction
+              t += cl[3 * r] * ql[cc];
+              t += cl[3 * r + 1] * ql[3 + cc];
+              t += cl[3 * r + 2] * ql[6 + cc];
+            } else {
+              t = fma(cl[3 * r], ql[cc], t);
+              t = fma(cl[3 * r + 1], ql[3 + cc], t);
+              t = fma(cl[3 * r + 2], ql[6 + cc], t);
+            }
+          }
+        if (!cross) continue;
+#pragma unroll
+        for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; d2++) {
+            P &t = x[9 * c + 3 * d1 + d2];
+            if (SIDE == 0) {
+              t = fma(a[s][d1], v[d2], t);
+              t = fma(a[s][d1 + 3], v[d2 + 3], t);
+              t = fma(a[s][d1 + 6], v[d2 + 6], t);
+            } else {
+              t = fma(a[s][3 * d1], v[3 * d2], t);
+              t = fma(a[s][3 * d1 + 1], v[3 * d2 + 1], t);
+              t = fma(a[s][3 * d1 + 2], v[3 * d2 + 2], t);
+            }
+          }
       }
+    }
+    if (cross) {
+      double xd[32];
+#pragma unroll
+      for (int e = 0; e < 32; e++) xd[e] = (double)x[e];
+      wave_sum32_store(xd, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
+    }
   }
 #pragma unroll
-  for (int e = 0; e < 9; e++) d[e] -= sum[e];
+  for (int s = 0; s < kAcaIpt; s++)
+#pragma unroll
+    for (int e = 0; e < 9; e++) d[s][e] -= sum[s][e];
 }
 
-// Column pass residual and the row pivot candidates of one chunk
-// (src/FarDot.cpp:474-580).
+// Column pass residual R_k, the column cross-term sums of iteration k-1 and
+// the row pivot candidates of one chunk (src/FarDot.cpp:474-580).  One wave
+// per chunk.
 template <class P>
-__global__ __launch_bounds__(256) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
+__global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[128];
-  __shared__ ArgMax sh_am[4];
-  const AcaTile ch = A.chunks_c[blockIdx.x];
+  __shared__ double sh[32 * 65];
+  const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
   const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
-  for (int t = threadIdx.x; t < 9 * k; t += blockDim.x)
+  for (int t = threadIdx.x; t < 9 * k; t += 64)
     piv[t / 9][t % 9] = Qp[9 * (fb.qoff + (int64_t)(t / 9) * fb.n + jk) + t % 9];
-  for (int l = threadIdx.x; l < k; l += blockDim.x) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
+  for (int l = threadIdx.x; l < k; l += 64) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
   __syncthreads();
-  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
   const int iend = min(ch.off + A.chunk, fb.m);
-  ArgMax best{kNegHuge, -1};
-  for (int i = ch.off + threadIdx.x; i < iend; i += blockDim.x) {
-    P c[9];
-    sym_load<P>(A.Y + 9 * (A.ybase_c[ch.slot] + i), c);
-    residual<P, 0>(c, Cp + 9 * (fb.coff + i), fb.m, k, piv);
+  P c[kAcaIpt][9];
+  bool valid[kAcaIpt];
 #pragma unroll
-    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[e];
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int i = ch.off + threadIdx.x + 64 * s;
+    valid[s] = i < iend;
+    if (valid[s])
+      sym_load<P>(A.Y + 9 * (A.ybase_c[ch.slot] + i), c[s]);
+    else
+#pragma unroll
+      for (int e = 0; e < 9; e++) c[s][e] = (P)0;
+  }
+  residual_cross<P, 0>(c, valid, Cp + 9 * (fb.coff + ch.off + threadIdx.x), fb.m, k, piv, sh,
+                       A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
+  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  ArgMax best{kNegHuge, -1};
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int i = ch.off + threadIdx.x + 64 * s;
+    if (!valid[s]) continue;
+#pragma unroll
+    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[s][e];
     bool u = false;
     for (int l = 0; l < k; l++) u |= (used[l] == i);
-    const double d = (double)det3(c);
-    if (!u && d > best.val) best = ArgMax{d, i};
+    const double dt = (double)det3(c[s]);
+    if (!u && dt > best.val) best = ArgMax{dt, i};
   }
-  const ArgMax am = block_argmax(best, sh_am);
+  const ArgMax am = wave_argmax(best);
   if (threadIdx.x == 0) {
     A.part_c[blockIdx.x].val = am.val;
     A.part_c[blockIdx.x].idx = am.idx;
@@ -747,15 +857,11 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
   }
 }
 
-constexpr int kCrossLC = 4;  // cross-term iterations per pass over a chunk
-
-// Column normalisation C_k = R_k inv (src/FarDot.cpp:630-656), the Gram sums
-// of C_k and the column halves of the cross terms C_k^T C_l, l < k-1
-// (src/FarDot.cpp:1002-1078), for one chunk.
+// Column normalisation C_k = R_k inv (src/FarDot.cpp:630-656) and the Gram
+// sums of C_k for one chunk.  One wave per chunk.
 template <class P>
-__global__ __launch_bounds__(256) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) {
-  __shared__ double sh[4 * 9 * kCrossLC];
-  const AcaTile ch = A.chunks_c[blockIdx.x];
+__global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) {
+  const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -764,152 +870,102 @@ __global__ __launch_bounds__(256) void aca_col_norm_kernel(AcaStepArgs A, P *Cp)
   for (int e = 0; e < 9; e++) inv[e] = (P)A.inv[9 * (size_t)ch.slot + e];
   P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
   const int iend = min(ch.off + A.chunk, fb.m);
-  double a[kAcaIpt][9];
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
-    const int i = ch.off + threadIdx.x + 256 * s;
+    const int i = ch.off + threadIdx.x + 64 * s;
+    if (i >= iend) continue;
+    P c[9], o[9];
+    load9(Ck + 9 * (int64_t)i, c);
 #pragma unroll
-    for (int e = 0; e < 9; e++) a[s][e] = 0.0;
-    if (i < iend) {
-      P c[9], o[9];
-      load9(Ck + 9 * (int64_t)i, c);
+    for (int r = 0; r < 3; r++)
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 3; cc++)
+        o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
+    double a[9];
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++)
-          o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        Ck[9 * (int64_t)i + e] = o[e];
-        a[s][e] = (double)o[e];
-      }
-      gram_col(a[s], g);
+    for (int e = 0; e < 9; e++) {
+      Ck[9 * (int64_t)i + e] = o[e];
+      a[e] = (double)o[e];
     }
+    gram_col(a, g);
   }
-  {
-    AcaPart &pt = A.part_c[blockIdx.x];
 #pragma unroll
-    for (int e = 0; e < 6; e++) {
-      const double s = block_sum(g[e], sh);
-      if (threadIdx.x == 0) pt.g[e] = s;
-    }
-  }
-  double *X = A.x_c + (size_t)blockIdx.x * A.max_iter * 9;
-  for (int l0 = 0; l0 + 1 < k; l0 += kCrossLC) {
-    const int nl = min(kCrossLC, k - 1 - l0);
-    double cd[kCrossLC * 9];
-#pragma unroll
-    for (int e = 0; e < kCrossLC * 9; e++) cd[e] = 0.0;
-#pragma unroll
-    for (int s = 0; s < kAcaIpt; s++) {
-      const int i = ch.off + threadIdx.x + 256 * s;
-      if (i >= iend) break;
-#pragma unroll
-      for (int c = 0; c < kCrossLC; c++) {
-        if (c >= nl) break;
-        P v[9];
-        load9(Cp + 9 * (fb.coff + (int64_t)(l0 + c) * fb.m + i), v);
-#pragma unroll
-        for (int d1 = 0; d1 < 3; d1++)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; d2++)
-            cd[9 * c + 3 * d1 + d2] += a[s][d1] * (double)v[d2] + a[s][d1 + 3] * (double)v[d2 + 3] +
-                                       a[s][d1 + 6] * (double)v[d2 + 6];
-      }
-    }
-    block_sums_store<kCrossLC * 9>(cd, sh, X + 9 * l0, 9 * nl);
+  for (int e = 0; e < 6; e++) {
+    const double s = wave_sum(g[e]);
+    if (threadIdx.x == 0) A.part_c[blockIdx.x].g[e] = s;
   }
 }
 
-// Row pass residual Q_k, column pivot candidates, Gram sums of Q_k and the
-// row halves of the cross terms Q_k Q_l^T, l < k-1 (src/FarDot.cpp:797-1078),
-// for one chunk.
+// Row pass residual Q_k, the row cross-term sums of iteration k-1, column
+// pivot candidates and the Gram sums of Q_k for one chunk
+// (src/FarDot.cpp:797-1078).  One wave per chunk.
 template <class P>
-__global__ __launch_bounds__(256) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
+__global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[129];
-  __shared__ ArgMax sh_am[4];
-  __shared__ double sh[4 * 9 * kCrossLC];
-  const AcaTile ch = A.chunks_r[blockIdx.x];
+  __shared__ double sh[32 * 65];
+  const AcaTile ch = locate(A.cfirst_r, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
   const int ik = A.ipiv[(size_t)b * A.max_iter + k];
-  for (int t = threadIdx.x; t < 9 * k; t += blockDim.x)
+  for (int t = threadIdx.x; t < 9 * k; t += 64)
     piv[t / 9][t % 9] = Cp[9 * (fb.coff + (int64_t)(t / 9) * fb.m + ik) + t % 9];
-  for (int l = threadIdx.x; l <= k; l += blockDim.x)
-    used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
+  for (int l = threadIdx.x; l <= k; l += 64) used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
   __syncthreads();
-  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
   const int jend = min(ch.off + A.chunk, fb.n);
+  P q[kAcaIpt][9];
+  bool valid[kAcaIpt];
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    const int j = ch.off + threadIdx.x + 64 * s;
+    valid[s] = j < jend;
+    if (valid[s])
+      sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + j), q[s]);
+    else
+#pragma unroll
+      for (int e = 0; e < 9; e++) q[s][e] = (P)0;
+  }
+  residual_cross<P, 1>(q, valid, Qp + 9 * (fb.qoff + ch.off + threadIdx.x), fb.n, k, piv, sh,
+                       A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
+  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
   ArgMax best{kNegHuge, -1};
-  double a[kAcaIpt][9];
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
-    const int j = ch.off + threadIdx.x + 256 * s;
+    const int j = ch.off + threadIdx.x + 64 * s;
+    if (!valid[s]) continue;
+    double a[9];
 #pragma unroll
-    for (int e = 0; e < 9; e++) a[s][e] = 0.0;
-    if (j < jend) {
-      P q[9];
-      sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + j), q);
-      residual<P, 1>(q, Qp + 9 * (fb.qoff + j), fb.n, k, piv);
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        Qk[9 * (int64_t)j + e] = q[e];
-        a[s][e] = (double)q[e];
-      }
-      bool u = false;
-      for (int l = 0; l <= k; l++) u |= (used[l] == j);
-      const double d = (double)det3(q);
-      if (!u && d > best.val) best = ArgMax{d, j};
-      gram_row(a[s], g);
+    for (int e = 0; e < 9; e++) {
+      Qk[9 * (int64_t)j + e] = q[s][e];
+      a[e] = (double)q[s][e];
     }
+    bool u = false;
+    for (int l = 0; l <= k; l++) u |= (used[l] == j);
+    const double dt = (double)det3(q[s]);
+    if (!u && dt > best.val) best = ArgMax{dt, j};
+    gram_row(a, g);
   }
-  const ArgMax am = block_argmax(best, sh_am);
-  {
-    AcaPart &pt = A.part_r[blockIdx.x];
-    if (threadIdx.x == 0) pt.val = am.val, pt.idx = am.idx;
+  const ArgMax am = wave_argmax(best);
+  AcaPart &pt = A.part_r[blockIdx.x];
+  if (threadIdx.x == 0) pt.val = am.val, pt.idx = am.idx;
 #pragma unroll
-    for (int e = 0; e < 6; e++) {
-      const double s = block_sum(g[e], sh);
-      if (threadIdx.x == 0) pt.g[e] = s;
-    }
-  }
-  double *X = A.x_r + (size_t)blockIdx.x * A.max_iter * 9;
-  for (int l0 = 0; l0 + 1 < k; l0 += kCrossLC) {
-    const int nl = min(kCrossLC, k - 1 - l0);
-    double qd[kCrossLC * 9];
-#pragma unroll
-    for (int e = 0; e < kCrossLC * 9; e++) qd[e] = 0.0;
-#pragma unroll
-    for (int s = 0; s < kAcaIpt; s++) {
-      const int j = ch.off + threadIdx.x + 256 * s;
-      if (j >= jend) break;
-#pragma unroll
-      for (int c = 0; c < kCrossLC; c++) {
-        if (c >= nl) break;
-        P v[9];
-        load9(Qp + 9 * (fb.qoff + (int64_t)(l0 + c) * fb.n + j), v);
-#pragma unroll
-        for (int d1 = 0; d1 < 3; d1++)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; d2++)
-            qd[9 * c + 3 * d1 + d2] += a[s][3 * d1] * (double)v[3 * d2] +
-                                       a[s][3 * d1 + 1] * (double)v[3 * d2 + 1] +
-                                       a[s][3 * d1 + 2] * (double)v[3 * d2 + 2];
-      }
-    }
-    block_sums_store<kCrossLC * 9>(qd, sh, X + 9 * l0, 9 * nl);
+  for (int e = 0; e < 6; e++) {
+    const double s = wave_sum(g[e]);
+    if (threadIdx.x == 0) pt.g[e] = s;
   }
 }
 
-// Per block: next column pivot, nu2 = |C_k Q_k|_F^2, mu2 update with the
-// cross terms in the reference's order (l, d1, d2 ascending) and the stop
-// test (src/FarDot.cpp:1002-1108).
+// Per block, after the passes of iteration k: the deferred soft stop test of
+// iteration k-1 with its cross terms added to mu2 in the reference's order
+// (l, d1, d2 ascending); then the next column pivot, nu2 = |C_k Q_k|_F^2 and
+// the hard stop tests of iteration k (src/FarDot.cpp:1002-1108).  mu2[b]
+// holds mu2 of iteration k-1 without its cross terms, nu2[b] its nu2; it[b]
+// ends as the block's rank when it stops.
 __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
-  __shared__ double cs[127 * 9], qs[127 * 9];
+  __shared__ double cs[126 * 9], qs[126 * 9];
   __shared__ ArgMax sh_am[4];
   const int slot = blockIdx.x;
   const int b = A.active[slot];
@@ -921,7 +977,7 @@ __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
   for (int c = r0 + (int)threadIdx.x; c < r1; c += blockDim.x)
     best = better(best, ArgMax{A.part_r[c].val, A.part_r[c].idx});
   const ArgMax am = block_argmax(best, sh_am);
-  const int ncross = 9 * (k > 1 ? k - 1 : 0);
+  const int ncross = 9 * (k > 2 ? k - 2 : 0);
   for (int t = threadIdx.x; t < ncross; t += blockDim.x) {
     double sc = 0.0, sq = 0.0;
     for (int c = c0; c < c1; c++) sc += A.x_c[(size_t)c * A.max_iter * 9 + t];
@@ -931,6 +987,15 @@ __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    double mu2 = A.mu2[b];
+    if (k >= 1) {
+      for (int t = 0; t < ncross; t++) mu2 += 2.0 * fabs(cs[t]) * fabs(qs[t]);
+      if (A.nu2[b] < mu2 * A.eps2) {  // iteration k-1 met the tolerance: rank k
+        A.mu2[b] = mu2;
+        A.stop[slot] = 1;
+        return;
+      }
+    }
     double GC[6] = {0, 0, 0, 0, 0, 0}, GQ[6] = {0, 0, 0, 0, 0, 0};
     for (int c = c0; c < c1; c++)
       for (int e = 0; e < 6; e++) GC[e] += A.part_c[c].g[e];
@@ -938,15 +1003,13 @@ __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
       for (int e = 0; e < 6; e++) GQ[e] += A.part_r[c].g[e];
     const double nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
                        2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
-    double mu2 = A.mu2[b] + nu2;
-    for (int t = 0; t < ncross; t++) mu2 += 2.0 * fabs(cs[t]) * fabs(qs[t]);
     A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
-    A.mu2[b] = mu2;
+    A.mu2[b] = mu2 + nu2;
     A.nu2[b] = nu2;
     const int itn = k + 1;
     A.it[b] = itn;
     const int mn = fb.m < fb.n ? fb.m : fb.n;
-    A.stop[slot] = (nu2 < mu2 * A.eps2 || itn >= A.max_iter || itn >= mn) ? 1 : 0;
+    A.stop[slot] = (itn >= A.max_iter || itn >= mn) ? 1 : 0;
   }
 }
 
@@ -962,8 +1025,7 @@ struct AcaApplyArgs {
   const int32_t *it;
   const float4 *fp;
   double *u;
-  const AcaTile *chunks_i;   // (apply slot, first item) over I
-  const AcaTile *chunks_j;   // over J
+  int nlist;
   const int32_t *first_i;    // apply slot -> first I chunk (nlist + 1)
   const int32_t *first_j;
   double *w_j;               // per J chunk: max_iter x 3 (side 0 partials)
@@ -973,10 +1035,11 @@ struct AcaApplyArgs {
 };
 
 template <class P, int SIDE>
-__global__ __launch_bounds__(256) void aca_apply_reduce_kernel(AcaApplyArgs A, const P *Cp,
-                                                               const P *Qp) {
-  __shared__ double sh[4 * 3];
-  const AcaTile ch = SIDE == 0 ? A.chunks_j[blockIdx.x] : A.chunks_i[blockIdx.x];
+__global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const P *Cp,
+                                                              const P *Qp) {
+  __shared__ double sh[32 * 65];
+  const AcaTile ch = SIDE == 0 ? locate(A.first_j, A.nlist, blockIdx.x, A.chunk)
+                                : locate(A.first_i, A.nlist, blockIdx.x, A.chunk);
   const int b = A.list[ch.slot];
   const FarBlock fb = A.blk[b];
   const int K = min(A.it[b], 128);
@@ -988,35 +1051,43 @@ __global__ __launch_bounds__(256) void aca_apply_reduce_kernel(AcaApplyArgs A, c
   float4 f[kAcaIpt];
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
-    const int it = ch.off + threadIdx.x + 256 * s;
+    const int it = ch.off + threadIdx.x + 64 * s;
     f[s] = it < send ? A.fp[s0 + it] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  for (int l = 0; l < K; l++) {
-    double a[3] = {0, 0, 0};
+  // ten rank indices (30 sums, padded to 32) per wave reduction
+  for (int l0 = 0; l0 < K; l0 += 10) {
+    const int nl = min(10, K - l0);
+    double x[32];
 #pragma unroll
-    for (int s = 0; s < kAcaIpt; s++) {
-      const int it = ch.off + threadIdx.x + 256 * s;
-      if (it >= send) break;
-      const P *q = Src + 9 * ((int64_t)l * ns + it);
-      if (SIDE == 0) {
-        a[0] += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
-        a[1] += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
-        a[2] += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
-      } else {
-        a[0] += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
-        a[1] += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
-        a[2] += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+    for (int e = 0; e < 32; e++) x[e] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 10; c++) {
+#pragma unroll
+      for (int s = 0; s < kAcaIpt; s++) {
+        const int it = ch.off + threadIdx.x + 64 * s;
+        if (c >= nl || it >= send) continue;
+        const P *q = Src + 9 * ((int64_t)(l0 + c) * ns + it);
+        if (SIDE == 0) {
+          x[3 * c] += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
+          x[3 * c + 1] += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
+          x[3 * c + 2] += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
+        } else {
+          x[3 * c] += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
+          x[3 * c + 1] += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
+          x[3 * c + 2] += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+        }
       }
     }
-    block_sums_store<3>(a, sh, W + 3 * l, 3);
+    wave_sum32_store(x, sh, W + 3 * l0, 3 * nl);
   }
 }
 
 template <class P, int SIDE>
-__global__ __launch_bounds__(256) void aca_apply_update_kernel(AcaApplyArgs A, const P *Cp,
+__global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, const P *Cp,
                                                                const P *Qp) {
   __shared__ double wsh[128 * 3];
-  const AcaTile ch = SIDE == 0 ? A.chunks_i[blockIdx.x] : A.chunks_j[blockIdx.x];
+  const AcaTile ch = SIDE == 0 ? locate(A.first_i, A.nlist, blockIdx.x, A.chunk)
+                                : locate(A.first_j, A.nlist, blockIdx.x, A.chunk);
   const int b = A.list[ch.slot];
   const FarBlock fb = A.blk[b];
   const int K = min(A.it[b], 128);
@@ -1566,6 +1637,8 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
 
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)((L.nactive + 255) / 256)), dim3(256), 0,
+                     st, L.tfirst, L.nactive, (AcaTile *)L.tiles);
   AcaMlpArgs A{L.xp, L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, (const AcaTile *)L.tiles,
                L.ntiles, L.max_iter, L.row_pass, L.Y};
   const int64_t waves = (L.ntiles + 3) / 4;
@@ -1578,11 +1651,11 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
     allow_big_lds();
     WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
     const size_t lds = kW2FragBytes + kW3FragBytes;
-    const int grid = grid_for(waves, 4, L.cus);
+    const int grid = grid_for(waves, kNearHThreads / 64, L.cus);
     if (L.prec == PREC_BF16)
-      hipLaunchKernelGGL(aca_mlp_h_kernel<bf16x8>, dim3(grid), dim3(256), lds, st, A, W);
+      hipLaunchKernelGGL(aca_mlp_h_kernel<bf16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A, W);
     else
-      hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(256), lds, st, A, W);
+      hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A, W);
   }
   return hipGetLastError();
 }
@@ -1590,29 +1663,28 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
   if (L.nactive <= 0) return hipSuccess;
   AcaStepArgs A{L.blk, L.active, L.ybase_c, L.ybase_r, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2,
-                L.stop, L.max_iter, L.eps2, (const AcaTile *)L.chunks_c,
-                (const AcaTile *)L.chunks_r, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
+                L.stop, L.max_iter, L.eps2, L.nactive, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
                 (AcaPart *)L.part_r, L.x_c, L.x_r, L.inv, L.chunk};
   const dim3 gc((unsigned)L.nchunks_c), gr((unsigned)L.nchunks_r), ga((unsigned)L.nactive);
   if (which == 0) {
     if (L.pool_f64) {
-      hipLaunchKernelGGL(aca_col_resid_kernel<double>, gc, dim3(256), 0, st, A, (double *)L.C,
+      hipLaunchKernelGGL(aca_col_resid_kernel<double>, gc, dim3(64), 0, st, A, (double *)L.C,
                          (const double *)L.Q);
       hipLaunchKernelGGL(aca_col_pivot_kernel<double>, ga, dim3(64), 0, st, A,
                          (const double *)L.C);
-      hipLaunchKernelGGL(aca_col_norm_kernel<double>, gc, dim3(256), 0, st, A, (double *)L.C);
+      hipLaunchKernelGGL(aca_col_norm_kernel<double>, gc, dim3(64), 0, st, A, (double *)L.C);
     } else {
-      hipLaunchKernelGGL(aca_col_resid_kernel<float>, gc, dim3(256), 0, st, A, (float *)L.C,
+      hipLaunchKernelGGL(aca_col_resid_kernel<float>, gc, dim3(64), 0, st, A, (float *)L.C,
                          (const float *)L.Q);
       hipLaunchKernelGGL(aca_col_pivot_kernel<float>, ga, dim3(64), 0, st, A, (const float *)L.C);
-      hipLaunchKernelGGL(aca_col_norm_kernel<float>, gc, dim3(256), 0, st, A, (float *)L.C);
+      hipLaunchKernelGGL(aca_col_norm_kernel<float>, gc, dim3(64), 0, st, A, (float *)L.C);
     }
   } else {
     if (L.pool_f64)
-      hipLaunchKernelGGL(aca_row_resid_kernel<double>, gr, dim3(256), 0, st, A,
+      hipLaunchKernelGGL(aca_row_resid_kernel<double>, gr, dim3(64), 0, st, A,
                          (const double *)L.C, (double *)L.Q);
     else
-      hipLaunchKernelGGL(aca_row_resid_kernel<float>, gr, dim3(256), 0, st, A, (const float *)L.C,
+      hipLaunchKernelGGL(aca_row_resid_kernel<float>, gr, dim3(64), 0, st, A, (const float *)L.C,
                          (float *)L.Q);
     hipLaunchKernelGGL(aca_finalize_kernel, ga, dim3(256), 0, st, A);
   }
@@ -1623,17 +1695,16 @@ template <class P>
 static void apply_launch(const AcaApplyArgs &A, const AcaApplyLaunch &L, hipStream_t st) {
   const P *C = (const P *)L.C, *Q = (const P *)L.Q;
   const dim3 gi((unsigned)L.nchunks_i), gj((unsigned)L.nchunks_j);
-  hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 0>), gj, dim3(256), 0, st, A, C, Q);
-  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 1>), gi, dim3(256), 0, st, A, C, Q);
-  hipLaunchKernelGGL((aca_apply_update_kernel<P, 0>), gi, dim3(256), 0, st, A, C, Q);
-  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_update_kernel<P, 1>), gj, dim3(256), 0, st, A, C, Q);
+  hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 0>), gj, dim3(64), 0, st, A, C, Q);
+  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 1>), gi, dim3(64), 0, st, A, C, Q);
+  hipLaunchKernelGGL((aca_apply_update_kernel<P, 0>), gi, dim3(64), 0, st, A, C, Q);
+  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_update_kernel<P, 1>), gj, dim3(64), 0, st, A, C, Q);
 }
 
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st) {
   if (L.nlist <= 0) return hipSuccess;
-  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, (const AcaTile *)L.chunks_i,
-                 (const AcaTile *)L.chunks_j, L.first_i, L.first_j, L.w_j, L.w_i, L.max_iter,
-                 L.chunk};
+  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, L.nlist, L.first_i, L.first_j, L.w_j, L.w_i,
+                 L.max_iter, L.chunk};
   if (L.pool_f64)
     apply_launch<double>(A, L, st);
   else

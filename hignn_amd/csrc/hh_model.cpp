@@ -152,7 +152,9 @@ struct hh_model {
   int64_t max_rel_coord = 500000;
   bool post_check = false, use_symmetry = true;
   int prec = PREC_FP32;
-  size_t far_pool_bytes = (size_t)32 << 30;
+  // byte budget of one far-field batch (factor pools + Gram histories); 0:
+  // half the device memory free at the first far field, at most 192 GiB
+  size_t far_pool_bytes = 0;
   hipStream_t stream = nullptr;
   // weights
   bool has_weights = false;
@@ -164,19 +166,19 @@ struct hh_model {
   DBuf xp, perm, fp, u, fin, uout;
   DBuf near_blocks, near_tasks;
   int64_t n_near_tasks = 0;
+  int64_t near_pairs = 0;  // pair-function evaluations of the near field
   DBuf dense_blocks, dense_tasks;
   int64_t n_dense_tasks = 0;
   std::vector<FarBlock> far;  // is, m, js, n (offsets set per batch)
   std::vector<int32_t> far_rank;
   // far workspace
-  DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r, tiles_c, tiles_r,
+  DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r,
       applylist, Y, Cpool, Qpool;
-  PBuf h_active, h_ybase_c, h_ybase_r, h_tiles_c, h_tiles_r, h_stop, h_list;
+  PBuf h_active, h_ybase_c, h_ybase_r, h_stop, h_list;
   // chunked step / apply lists and per-chunk partials
-  DBuf chunks_c, chunks_r, cfirst_c, cfirst_r, part_c, part_r, x_c, x_r, invb;
-  DBuf a_chunks_i, a_chunks_j, a_first_i, a_first_j, w_i, w_j;
-  PBuf h_chunks_c, h_chunks_r, h_cfirst_c, h_cfirst_r, h_achunks_i, h_achunks_j, h_afirst_i,
-      h_afirst_j;
+  DBuf tfirst_c, tfirst_r, tiles_c, tiles_r, cfirst_c, cfirst_r, part_c, part_r, x_c, x_r, invb;
+  DBuf a_first_i, a_first_j, w_i, w_j;
+  PBuf h_tfirst_c, h_tfirst_r, h_cfirst_c, h_cfirst_r, h_afirst_i, h_afirst_j;
   // persistent far-field workspace
   int far_mode = 1;  // 1: batched iterations (default), 0: persistent kernel
   bool verbose = false;  // HH_VERBOSE=1: far-field progress lines on stderr
@@ -189,26 +191,31 @@ struct hh_model {
   DBuf chk_blocks, chk_tasks, chk_tlist, chk_u;
   double t_close = 0, t_far = 0, t_total = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // pair-function kernels of the last dot: [0] near field, [1] ACA passes
+  double prof_ms[2] = {0, 0};
+  int64_t prof_pairs[2] = {0, 0}, prof_launches[2] = {0, 0};
+  hipEvent_t evm[4] = {nullptr, nullptr, nullptr, nullptr};
 
   ~hh_model() {
     for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y}) b->release();
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
                     &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
-                    &active_r, &ybase_c, &ybase_r, &tiles_c, &tiles_r, &applylist, &Y, &Cpool,
-                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &chunks_c, &chunks_r,
-                    &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r, &invb, &a_chunks_i,
-                    &a_chunks_j, &a_first_i, &a_first_j, &w_i, &w_j})
+                    &active_r, &ybase_c, &ybase_r, &applylist, &Y, &Cpool,
+                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &tfirst_c, &tfirst_r, &tiles_c, &tiles_r,
+                    &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r, &invb, &a_first_i,
+                    &a_first_j, &w_i, &w_j})
       b->release();
     for (auto &b : wf32) b.release();
     for (auto &b : wf64) b.release();
     for (auto &b : wh_f) b.release();
     for (auto &b : wh_bf16) b.release();
     for (auto &b : wh_f16) b.release();
-    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_tiles_c, &h_tiles_r, &h_stop, &h_list,
-                    &h_chunks_c, &h_chunks_r, &h_cfirst_c, &h_cfirst_r, &h_achunks_i, &h_achunks_j,
-                    &h_afirst_i, &h_afirst_j})
+    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_stop, &h_list, &h_tfirst_c,
+                    &h_tfirst_r, &h_cfirst_c, &h_cfirst_r, &h_afirst_i, &h_afirst_j})
       b->release();
     for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : evm)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -346,11 +353,13 @@ struct hh_model {
     // near-field row blocks and tasks (src/CloseDot.cpp)
     std::vector<RowBlock> rb;
     std::vector<NearTask> tk;
+    near_pairs = 0;
     for (size_t b = 0; b < tl.closeI.size(); b++) {
       const int64_t I = tl.closeI[b], J = tl.closeJ[b];
       RowBlock r{(int32_t)tl.start(I), (int32_t)tl.size(I), (int32_t)tl.start(J),
                  (int32_t)tl.size(J), (use_symmetry && J > I) ? BF_REUSE : 0, 0, 0, 0};
       rb.push_back(r);
+      near_pairs += (int64_t)r.m * r.n;
       for (int32_t o = 0; o < r.m; o += 4) tk.push_back(NearTask{(int32_t)b, o});
     }
     upload(near_blocks, rb.data(), rb.size() * sizeof(RowBlock));
@@ -408,7 +417,7 @@ struct hh_model {
       p_rank.reserve((size_t)nfar * 4);
       const size_t per_slot = (size_t)(sc + sq) * esz + (size_t)sy * 36;
       int ns = g_cus;
-      while (ns > 1 && (size_t)ns * per_slot > far_pool_bytes) ns /= 2;
+      while (ns > 1 && (size_t)ns * per_slot > pool_budget()) ns /= 2;
       p_poolC.reserve((size_t)ns * sc * esz);
       p_poolQ.reserve((size_t)ns * sq * esz);
       p_Y.reserve((size_t)ns * sy * 36);
@@ -453,38 +462,26 @@ struct hh_model {
   void apply_blocks(const std::vector<FarBlock> &hb, const std::vector<int32_t> &done, int MI,
                     bool f64, hipStream_t st) {
     const int nl = (int)done.size();
-    int64_t nci = 0, ncj = 0;
-    for (int32_t b : done) {
-      nci += (hb[(size_t)b].m + aca_chunk - 1) / aca_chunk;
-      ncj += (hb[(size_t)b].n + aca_chunk - 1) / aca_chunk;
-    }
     h_list.reserve((size_t)nl * 4);
     applylist.reserve((size_t)nl * 4);
-    h_achunks_i.reserve((size_t)nci * 8);
-    a_chunks_i.reserve((size_t)nci * 8);
-    h_achunks_j.reserve((size_t)ncj * 8);
-    a_chunks_j.reserve((size_t)ncj * 8);
     for (auto *b : {&h_afirst_i, &h_afirst_j}) b->reserve((size_t)(nl + 1) * 4);
     for (auto *b : {&a_first_i, &a_first_j}) b->reserve((size_t)(nl + 1) * 4);
-    w_i.reserve((size_t)std::max<int64_t>(nci, 1) * MI * 3 * 8);
-    w_j.reserve((size_t)std::max<int64_t>(ncj, 1) * MI * 3 * 8);
-    int32_t *hl = h_list.as<int32_t>(), *ci = h_achunks_i.as<int32_t>(),
-            *cj = h_achunks_j.as<int32_t>(), *fi = h_afirst_i.as<int32_t>(),
+    int32_t *hl = h_list.as<int32_t>(), *fi = h_afirst_i.as<int32_t>(),
             *fj = h_afirst_j.as<int32_t>();
-    int64_t ki = 0, kj = 0;
+    int64_t nci = 0, ncj = 0;
     for (int a = 0; a < nl; a++) {
       const FarBlock &f = hb[(size_t)done[(size_t)a]];
       hl[a] = done[(size_t)a];
-      fi[a] = (int32_t)ki;
-      fj[a] = (int32_t)kj;
-      for (int32_t o = 0; o < f.m; o += aca_chunk) ci[2 * ki] = a, ci[2 * ki + 1] = o, ki++;
-      for (int32_t o = 0; o < f.n; o += aca_chunk) cj[2 * kj] = a, cj[2 * kj + 1] = o, kj++;
+      fi[a] = (int32_t)nci;
+      fj[a] = (int32_t)ncj;
+      nci += (f.m + aca_chunk - 1) / aca_chunk;
+      ncj += (f.n + aca_chunk - 1) / aca_chunk;
     }
-    fi[nl] = (int32_t)ki;
-    fj[nl] = (int32_t)kj;
+    fi[nl] = (int32_t)nci;
+    fj[nl] = (int32_t)ncj;
+    w_i.reserve((size_t)std::max<int64_t>(nci, 1) * MI * 3 * 8);
+    w_j.reserve((size_t)std::max<int64_t>(ncj, 1) * MI * 3 * 8);
     HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(a_chunks_i.p, ci, (size_t)nci * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(a_chunks_j.p, cj, (size_t)ncj * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(a_first_i.p, fi, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(a_first_j.p, fj, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
     AcaApplyLaunch al{};
@@ -498,8 +495,6 @@ struct hh_model {
     al.pool_f64 = f64;
     al.C = Cpool.p;
     al.Q = Qpool.p;
-    al.chunks_i = a_chunks_i.p;
-    al.chunks_j = a_chunks_j.p;
     al.first_i = a_first_i.as<int32_t>();
     al.first_j = a_first_j.as<int32_t>();
     al.nchunks_i = nci;
@@ -509,6 +504,16 @@ struct hh_model {
     al.max_iter = MI;
     al.chunk = aca_chunk;
     HIPCHK(launch_aca_apply(al, st));
+  }
+
+  size_t pool_budget() {
+    if (far_pool_bytes == 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      far_pool_bytes = std::min<size_t>(fr / 2, (size_t)192 << 30);
+      far_pool_bytes = std::max<size_t>(far_pool_bytes, (size_t)1 << 30);
+    }
+    return far_pool_bytes;
   }
 
   void far_field(hipStream_t st) {
@@ -521,18 +526,19 @@ struct hh_model {
     const bool f64 = prec == PREC_FP32;
     const size_t esz = (f64 ? 8 : 4) * 9;
     const int MI = max_iter;
-    // pool budget split between C and Q
-    const int64_t cap = std::max<int64_t>(1, (int64_t)(far_pool_bytes / (2 * esz)));
+    // batch budget: C and Q pools, kmax (m + n) entries of esz bytes
+    auto block_bytes = [&](const FarBlock &f) {
+      const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
+      return kmax * (int64_t)(f.m + f.n) * (int64_t)esz;
+    };
     int64_t b0 = 0;
     while (b0 < nfar) {
       // batch: blocks [b0, b1) whose reservations fit
-      int64_t b1 = b0, csum = 0, qsum = 0, ysum_c = 0, ysum_r = 0;
+      int64_t b1 = b0, bytes = 0, ysum_c = 0, ysum_r = 0;
       while (b1 < nfar) {
         const FarBlock &f = far[(size_t)b1];
-        const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
-        if (b1 > b0 && (csum + kmax * f.m > cap || qsum + kmax * f.n > cap)) break;
-        csum += kmax * f.m;
-        qsum += kmax * f.n;
+        if (b1 > b0 && bytes + block_bytes(f) > (int64_t)pool_budget()) break;
+        bytes += block_bytes(f);
         ysum_c += f.m;
         ysum_r += f.n;
         b1++;
@@ -586,56 +592,52 @@ struct hh_model {
           }
         }
         iter++;
-        // tiles of 16 items (MLP passes) and chunks of aca_chunk items (steps)
-        // for the column (i in I) and row (j in J) sides
-        int64_t ntc = 0, ntr = 0, ncc = 0, ncr = 0;
-        for (int a = 0; a < na; a++) {
-          const FarBlock &f = hb[(size_t)act[(size_t)a]];
-          ntc += (f.m + 15) / 16;
-          ntr += (f.n + 15) / 16;
-          ncc += (f.m + aca_chunk - 1) / aca_chunk;
-          ncr += (f.n + aca_chunk - 1) / aca_chunk;
-        }
-        for (auto *b : {&h_tiles_c, &h_tiles_r, &h_chunks_c, &h_chunks_r})
-          b->reserve((size_t)std::max(std::max(ntc, ntr), std::max(ncc, ncr)) * 8);
-        for (auto *b : {&tiles_c, &tiles_r, &chunks_c, &chunks_r})
-          b->reserve((size_t)std::max(std::max(ntc, ntr), std::max(ncc, ncr)) * 8);
-        for (auto *b : {&h_cfirst_c, &h_cfirst_r}) b->reserve((size_t)(na + 1) * 4);
-        for (auto *b : {&cfirst_c, &cfirst_r}) b->reserve((size_t)(na + 1) * 4);
-        part_c.reserve((size_t)ncc * kAcaPartBytes);
-        part_r.reserve((size_t)ncr * kAcaPartBytes);
-        x_c.reserve((size_t)ncc * MI * 9 * 8);
-        x_r.reserve((size_t)ncr * MI * 9 * 8);
-        invb.reserve((size_t)na * 9 * 8);
-        int32_t *tc = h_tiles_c.as<int32_t>(), *tr = h_tiles_r.as<int32_t>();
-        int32_t *cc = h_chunks_c.as<int32_t>(), *cr = h_chunks_r.as<int32_t>();
+        // per-slot prefix counts of the 16-item tiles (MLP passes) and the
+        // aca_chunk-item chunks (steps) on the column (i in I) and row
+        // (j in J) sides; kernels locate their tile or chunk from these
+        for (auto *b : {&h_tfirst_c, &h_tfirst_r, &h_cfirst_c, &h_cfirst_r})
+          b->reserve((size_t)(na + 1) * 4);
+        for (auto *b : {&tfirst_c, &tfirst_r, &cfirst_c, &cfirst_r})
+          b->reserve((size_t)(na + 1) * 4);
+        int32_t *tc = h_tfirst_c.as<int32_t>(), *tr = h_tfirst_r.as<int32_t>();
         int32_t *fc = h_cfirst_c.as<int32_t>(), *fr = h_cfirst_r.as<int32_t>();
         int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
         int32_t *ha = h_active.as<int32_t>();
-        int64_t sc = 0, sr = 0, ic = 0, ir = 0, kc = 0, kr = 0;
+        int64_t sc = 0, sr = 0, ntc = 0, ntr = 0, ncc = 0, ncr = 0;
         for (int a = 0; a < na; a++) {
           const FarBlock &f = hb[(size_t)act[(size_t)a]];
           ha[a] = act[(size_t)a];
           yc[a] = sc;
           yr[a] = sr;
+          tc[a] = (int32_t)ntc;
+          tr[a] = (int32_t)ntr;
+          fc[a] = (int32_t)ncc;
+          fr[a] = (int32_t)ncr;
           sc += f.m;
           sr += f.n;
-          for (int32_t o = 0; o < f.m; o += 16) tc[2 * ic] = a, tc[2 * ic + 1] = o, ic++;
-          for (int32_t o = 0; o < f.n; o += 16) tr[2 * ir] = a, tr[2 * ir + 1] = o, ir++;
-          fc[a] = (int32_t)kc;
-          fr[a] = (int32_t)kr;
-          for (int32_t o = 0; o < f.m; o += aca_chunk) cc[2 * kc] = a, cc[2 * kc + 1] = o, kc++;
-          for (int32_t o = 0; o < f.n; o += aca_chunk) cr[2 * kr] = a, cr[2 * kr + 1] = o, kr++;
+          ntc += (f.m + 15) / 16;
+          ntr += (f.n + 15) / 16;
+          ncc += (f.m + aca_chunk - 1) / aca_chunk;
+          ncr += (f.n + aca_chunk - 1) / aca_chunk;
         }
-        fc[na] = (int32_t)kc;
-        fr[na] = (int32_t)kr;
+        if (ntc > INT32_MAX || ntr > INT32_MAX)
+          throw HHError(HH_ERR_ARG, "far-field batch too large for 32-bit tile indices");
+        tc[na] = (int32_t)ntc;
+        tr[na] = (int32_t)ntr;
+        fc[na] = (int32_t)ncc;
+        fr[na] = (int32_t)ncr;
+        tiles_c.reserve((size_t)ntc * 8);
+        tiles_r.reserve((size_t)ntr * 8);
+        part_c.reserve((size_t)ncc * kAcaPartBytes);
+        part_r.reserve((size_t)ncr * kAcaPartBytes);
+        x_c.reserve((size_t)ncc * MI * 9 * 8);
+        x_r.reserve((size_t)ncr * MI * 9 * 8);
+        invb.reserve((size_t)na * 9 * 8);
         HIPCHK(hipMemcpyAsync(active_c.p, ha, (size_t)na * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(ybase_c.p, yc, (size_t)na * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(ybase_r.p, yr, (size_t)na * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(tiles_c.p, tc, (size_t)ntc * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(tiles_r.p, tr, (size_t)ntr * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(chunks_c.p, cc, (size_t)ncc * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(chunks_r.p, cr, (size_t)ncr * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(tfirst_c.p, tc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(tfirst_r.p, tr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(cfirst_c.p, fc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(cfirst_r.p, fr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
         AcaMlpLaunch ml{};
@@ -666,8 +668,6 @@ struct hh_model {
         sl.stop = stop.as<int32_t>();
         sl.max_iter = MI;
         sl.eps2 = epsilon * epsilon;
-        sl.chunks_c = chunks_c.p;
-        sl.chunks_r = chunks_r.p;
         sl.cfirst_c = cfirst_c.as<int32_t>();
         sl.cfirst_r = cfirst_r.as<int32_t>();
         sl.part_c = part_c.p;
@@ -685,19 +685,35 @@ struct hh_model {
         // column pass, residual, row pivot, normalisation
         ml.row_pass = 0;
         ml.ybase = ybase_c.as<int64_t>();
+        ml.tfirst = tfirst_c.as<int32_t>();
+        ml.nactive = na;
         ml.tiles = tiles_c.p;
         ml.ntiles = ntc;
+        HIPCHK(hipEventRecord(evm[0], st));
         HIPCHK(launch_aca_mlp(ml, st));
+        HIPCHK(hipEventRecord(evm[1], st));
         HIPCHK(launch_aca_steps(sl, 0, st));
         // row pass, residual, column pivot, norms, stop test
         ml.row_pass = 1;
         ml.ybase = ybase_r.as<int64_t>();
+        ml.tfirst = tfirst_r.as<int32_t>();
+        ml.nactive = na;
         ml.tiles = tiles_r.p;
         ml.ntiles = ntr;
+        HIPCHK(hipEventRecord(evm[2], st));
         HIPCHK(launch_aca_mlp(ml, st));
+        HIPCHK(hipEventRecord(evm[3], st));
         HIPCHK(launch_aca_steps(sl, 1, st));
         HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        {
+          float t0 = 0, t1 = 0;
+          HIPCHK(hipEventElapsedTime(&t0, evm[0], evm[1]));
+          HIPCHK(hipEventElapsedTime(&t1, evm[2], evm[3]));
+          prof_ms[1] += (double)t0 + (double)t1;
+          prof_pairs[1] += sc + sr;
+          prof_launches[1] += 2;
+        }
         const int32_t *hs = h_stop.as<int32_t>();
         std::vector<int32_t> done, keep;
         for (int a = 0; a < na; a++) (hs[a] ? done : keep).push_back(act[(size_t)a]);
@@ -734,14 +750,18 @@ struct hh_model {
     if (!has_weights) throw HHError(HH_ERR_STATE, "two-body model not loaded");
     if (!built) update();  // the reference would dereference empty lists here
     ensure_buffers();
-    if (!ev[0])
+    if (!ev[0]) {
       for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+      for (auto &e : evm) HIPCHK(hipEventCreate(&e));
+    }
   }
 
   void dot_device(float *u_dev, const float *f_dev, hipStream_t st) {
     check_ready();
     HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
     HIPCHK(hipMemsetAsync(u.p, 0, 24 * (size_t)n, st));
+    prof_ms[1] = 0;
+    prof_pairs[1] = prof_launches[1] = 0;
     HIPCHK(hipEventRecord(ev[0], st));
     near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
     HIPCHK(hipEventRecord(ev[1], st));
@@ -755,6 +775,9 @@ struct hh_model {
     t_close = a;
     t_far = b;
     t_total = (double)a + (double)b;
+    prof_ms[0] = a;  // the near field is one kernel launch
+    prof_pairs[0] = near_pairs;
+    prof_launches[0] = n_near_tasks > 0 ? 1 : 0;
   }
 
   void dot_host(float *uh, const float *fh, bool dense) {
@@ -1135,6 +1158,17 @@ int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int
 }
 
 void hh_tree_free(hh_tree *t) { delete t; }
+
+int hh_model_last_profile(hh_model *m, double ms[2], int64_t pairs[2], int64_t launches[2]) {
+  return guarded([&]() {
+    if (!m || !ms || !pairs || !launches) throw HHError(HH_ERR_ARG, "null argument");
+    for (int i = 0; i < 2; i++) {
+      ms[i] = m->prof_ms[i];
+      pairs[i] = m->prof_pairs[i];
+      launches[i] = m->prof_launches[i];
+    }
+  });
+}
 
 int hh_model_last_timings(hh_model *m, double t[3]) {
   return guarded([&]() {

@@ -154,6 +154,16 @@ class Model {
     return py::make_tuple(t[0], t[1], t[2]);
   }
 
+  py::dict last_profile() {
+    double ms[2];
+    int64_t pairs[2], launches[2];
+    check(hh_model_last_profile(m_, ms, pairs, launches));
+    py::dict d;
+    d["near"] = py::make_tuple(ms[0], pairs[0], launches[0]);
+    d["far_mlp"] = py::make_tuple(ms[1], pairs[1], launches[1]);
+    return d;
+  }
+
  private:
   void apply(py::array &u, py::array_t<float, py::array::forcecast> f, bool dense) {
     auto fv = rows3(f, "f");
@@ -418,7 +428,8 @@ PYBIND11_MODULE(hignn, m) {
       .def("far_ranks", &Model::far_ranks)
       .def("check_fp64", &Model::check_fp64, py::arg("f"), py::arg("targets"),
            py::arg("convention") = 0)
-      .def("last_timings", &Model::last_timings);
+      .def("last_timings", &Model::last_timings)
+      .def("last_profile", &Model::last_profile);
 
   py::class_<Euler>(m, "ExplicitEuler")
       .def(py::init())

@@ -118,6 +118,13 @@ void hh_tree_free(hh_tree *t);
 /* Per-phase device times (ms) of the last dot: [close, far, total]. */
 int hh_model_last_timings(hh_model *m, double t[3]);
 
+/* Pair-function kernels of the last dot (device time in ms from HIP events
+ * on the model's stream, pair-function evaluations, launches):
+ * [0] the near-field kernel, [1] the ACA column/row passes of the far field.
+ * No reference counterpart (the reference times phases with Kokkos timers,
+ * src/FarDot.cpp:74-79); used for roofline accounting. */
+int hh_model_last_profile(hh_model *m, double ms[2], int64_t pairs[2], int64_t launches[2]);
+
 #ifdef __cplusplus
 }
 #endif
