@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--max-relative-coord", type=int, default=2048)
     ap.add_argument("--epsilon", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this size before the timed region "
+                         "(known-size reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
@@ -124,6 +127,15 @@ def main():
         model.dot_device(u_dev.data_ptr(), f_dev.data_ptr(), stream)
         torch.cuda.synchronize()
         return time.perf_counter() - t
+
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {nb} bytes read, {nb} bytes written")
+        del src, dst
 
     # chosen form: warmup, then K timed steps
     for i in range(args.warmup):

@@ -44,7 +44,7 @@ struct AcaMlpLaunch {
   const int64_t *ybase;
   const int32_t *tfirst;  // slot -> first 16-item tile (nactive + 1)
   int nactive;
-  void *tiles;            // ntiles (slot, first item) pairs, filled from tfirst
+  void *tiles;            // ntiles resolved tiles (24 B each), filled from tfirst
   int64_t ntiles;
   int max_iter;
   int row_pass;

@@ -438,28 +438,60 @@ __device__ __forceinline__ AcaTile locate(const int32_t *first, int nslots, int3
   return AcaTile{lo, (q - first[lo]) * items};
 }
 
+// One 16-item tile of an ACA column or row pass, resolved on the device
+// before the pass: the pivot particle, the first item particle, the number
+// of valid items and the first row of the entry buffer Y.
+struct AcaPassTile {
+  int32_t piv;    // tree-order index of the pivot particle
+  int32_t item0;  // tree-order index of the tile's first item
+  int32_t nvalid; // items in the tile (1..16)
+  int32_t pad;
+  int64_t yrow0;  // row of Y of the tile's first item
+};
+static_assert(sizeof(AcaPassTile) == 24, "AcaPassTile layout (24 B per tile on the host)");
+
 struct AcaMlpArgs {
   const float4 *xp;
-  const FarBlock *blk;
-  const int32_t *active;  // slot -> block
-  const int32_t *it;      // per block: current iteration k
-  const int32_t *ipiv;    // per block [maxIter]
-  const int32_t *jpiv;    // per block [maxIter+1]
-  const int64_t *ybase;   // per slot: first row of Y
-  const AcaTile *tiles;    // (slot, first item) of each 16-item tile
+  const AcaPassTile *tiles;
   int64_t ntiles;
-  int max_iter;
   int row_pass;           // 0: column pass (items i in I), 1: row pass (j in J)
   float *Y;               // (items, 9) block entries M (not symmetrised)
 };
 
 // Tile list of an ACA pass from the per-slot prefix counts: slot s owns
 // tiles first[s] .. first[s + 1] - 1, tile u of a slot starts at item 16u.
-__global__ void expand_tiles_kernel(const int32_t *first, int nslots, AcaTile *tiles) {
+// Column pass: pivot x[js + jpiv[k]], items x[is + i]; row pass: pivot
+// x[is + ipiv[k]], items x[js + j] (src/FarDot.cpp:346-458, 667-782).
+struct AcaExpandArgs {
+  const FarBlock *blk;
+  const int32_t *active;
+  const int32_t *it;
+  const int32_t *ipiv;
+  const int32_t *jpiv;
+  const int64_t *ybase;
+  const int32_t *first;
+  int nslots;
+  int max_iter;
+  int row_pass;
+  AcaPassTile *tiles;
+};
+
+__global__ void expand_tiles_kernel(AcaExpandArgs A) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  const int t0 = first[s], t1 = first[s + 1];
-  for (int t = t0; t < t1; t++) tiles[t] = AcaTile{s, 16 * (t - t0)};
+  if (s >= A.nslots) return;
+  const int b = A.active[s];
+  const FarBlock fb = A.blk[b];
+  const int k = A.it[b];
+  const int piv = A.row_pass ? fb.is + A.ipiv[(size_t)b * A.max_iter + k]
+                             : fb.js + A.jpiv[(size_t)b * (A.max_iter + 1) + k];
+  const int base = A.row_pass ? fb.js : fb.is;
+  const int lim = A.row_pass ? fb.n : fb.m;
+  const int64_t y0 = A.ybase[s];
+  const int t0 = A.first[s], t1 = A.first[s + 1];
+  for (int t = t0; t < t1; t++) {
+    const int off = 16 * (t - t0);
+    A.tiles[t] = AcaPassTile{piv, base + off, min(16, lim - off), 0, y0 + off};
+  }
 }
 
 template <class Core>
@@ -475,24 +507,17 @@ __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       const int64_t q = q0 + t < A.ntiles ? q0 + t : A.ntiles - 1;
-      const AcaTile tl = A.tiles[q];
-      const int b = A.active[tl.slot];
-      const FarBlock fb = A.blk[b];
-      const int k = A.it[b];
-      const int item = tl.off + p;
-      const int lim = A.row_pass ? fb.n : fb.m;
-      valid[t] = (q0 + t < A.ntiles) && item < lim;
-      const int itc = valid[t] ? item : 0;
-      float4 xa, xb;  // displacement xb - xa
-      if (!A.row_pass) {
-        xb = A.xp[fb.js + A.jpiv[(size_t)b * (A.max_iter + 1) + k]];
-        xa = A.xp[fb.is + itc];
-      } else {
-        xb = A.xp[fb.js + itc];
-        xa = A.xp[fb.is + A.ipiv[(size_t)b * A.max_iter + k]];
+      const AcaPassTile T = A.tiles[q];
+      valid[t] = (q0 + t < A.ntiles) && p < T.nvalid;
+      const int pc = valid[t] ? p : 0;
+      const float4 xi = A.xp[T.item0 + pc];
+      const float4 xv = A.xp[T.piv];
+      if (!A.row_pass) {  // pivot - item
+        rx[t] = xv.x - xi.x, ry[t] = xv.y - xi.y, rz[t] = xv.z - xi.z;
+      } else {  // item - pivot
+        rx[t] = xi.x - xv.x, ry[t] = xi.y - xv.y, rz[t] = xi.z - xv.z;
       }
-      rx[t] = xb.x - xa.x, ry[t] = xb.y - xa.y, rz[t] = xb.z - xa.z;
-      yrow[t] = A.ybase[tl.slot] + itc;
+      yrow[t] = T.yrow0 + pc;
     }
     f32x4 y[4];
     core(rx, ry, rz, y);
@@ -634,12 +659,13 @@ __device__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// Wave sums of 32 per-lane values of a one-wave workgroup, the first nvalid
-// stored to dst.  The values are transposed through LDS (sh: 32 rows of 65
-// doubles, the odd stride spreads the rows over the banks); lanes e and e + 32
-// each add half of row e and one exchange completes the sum.
-__device__ __forceinline__ void wave_sum32_store(const double (&x)[32], double *sh, double *dst,
-                                                 int nvalid) {
+// Wave sums (in FP64) of 32 per-lane values of a one-wave workgroup, the
+// first nvalid stored to dst.  The values are transposed through LDS (sh: 32
+// rows of 65 values of type T, the odd stride spreads the rows over the
+// banks); lanes e and e + 32 each add half of row e and one exchange
+// completes the sum.
+template <class T>
+__device__ __forceinline__ void wave_sum32_store(const T (&x)[32], T *sh, double *dst, int nvalid) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int e = 0; e < 32; e++) sh[e * 65 + lane] = x[e];
@@ -647,7 +673,7 @@ __device__ __forceinline__ void wave_sum32_store(const double (&x)[32], double *
   const int e = lane & 31, h = lane >> 5;
   double t = 0.0;
 #pragma unroll
-  for (int q = 0; q < 32; q++) t += sh[e * 65 + 32 * h + q];
+  for (int q = 0; q < 32; q++) t += (double)sh[e * 65 + 32 * h + q];
   t += __shfl_xor(t, 32, 64);
   if (lane < nvalid) dst[lane] = t;
   __syncthreads();
@@ -683,8 +709,9 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 // rank and every decision are the reference's.  The hard stops
 // (iteration count, block size) are taken at once.
 
-// Residual of a lane's kAcaIpt items, d -= sum_{l<k} L_l R_l with the 3x3
-// products of the reference's dot loops (src/FarDot.cpp:487-511, 812-836),
+// Residual sums of a lane's kAcaIpt items, sum = sum_{l<k} L_l R_l with the
+// 3x3 products of the reference's dot loops (src/FarDot.cpp:487-511, 812-836;
+// the caller subtracts them from the new column or row),
 // summed in the pool precision, plus the cross-term sums of iteration k-1
 // for l < k-2 against a = factor k-1 of the same item (per lane in the pool
 // precision over at most kAcaIpt items, across the wave in FP64):
@@ -697,11 +724,11 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 constexpr int kCrossLC = 3;
 
 template <class P, int SIDE>
-__device__ __forceinline__ void residual_cross(P (&d)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
+__device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
                                                const P *pool, int64_t stride, int k,
-                                               const P (*piv)[9], double *sh, double *X) {
+                                               const P (*piv)[9], P *sh, double *X) {
   const int ncross = k - 2;  // cross terms of iteration k-1: l < k-2
-  P a[kAcaIpt][9], sum[kAcaIpt][9];
+  P a[kAcaIpt][9];
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++)
 #pragma unroll
@@ -757,17 +784,8 @@ __device__ __forceinline__ void residual_cross(P (&d)[kAcaIpt][9], const bool (&
           }
       }
     }
-    if (cross) {
-      double xd[32];
-#pragma unroll
-      for (int e = 0; e < 32; e++) xd[e] = (double)x[e];
-      wave_sum32_store(xd, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
-    }
+    if (cross) wave_sum32_store<P>(x, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
   }
-#pragma unroll
-  for (int s = 0; s < kAcaIpt; s++)
-#pragma unroll
-    for (int e = 0; e < 9; e++) d[s][e] -= sum[s][e];
 }
 
 // Column pass residual R_k, the column cross-term sums of iteration k-1 and
@@ -777,7 +795,7 @@ template <class P>
 __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[128];
-  __shared__ double sh[32 * 65];
+  __shared__ P sh[32 * 65];
   const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
@@ -788,20 +806,20 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp,
   for (int l = threadIdx.x; l < k; l += 64) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
   __syncthreads();
   const int iend = min(ch.off + A.chunk, fb.m);
-  P c[kAcaIpt][9];
   bool valid[kAcaIpt];
 #pragma unroll
-  for (int s = 0; s < kAcaIpt; s++) {
-    const int i = ch.off + threadIdx.x + 64 * s;
-    valid[s] = i < iend;
-    if (valid[s])
-      sym_load<P>(A.Y + 9 * (A.ybase_c[ch.slot] + i), c[s]);
-    else
-#pragma unroll
-      for (int e = 0; e < 9; e++) c[s][e] = (P)0;
-  }
+  for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < iend;
+  P c[kAcaIpt][9];
   residual_cross<P, 0>(c, valid, Cp + 9 * (fb.coff + ch.off + threadIdx.x), fb.m, k, piv, sh,
                        A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    if (!valid[s]) continue;
+    P raw[9];
+    sym_load<P>(A.Y + 9 * (A.ybase_c[ch.slot] + ch.off + threadIdx.x + 64 * s), raw);
+#pragma unroll
+    for (int e = 0; e < 9; e++) c[s][e] = raw[e] - c[s][e];
+  }
   P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
   ArgMax best{kNegHuge, -1};
 #pragma unroll
@@ -904,7 +922,7 @@ template <class P>
 __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[129];
-  __shared__ double sh[32 * 65];
+  __shared__ P sh[32 * 65];
   const AcaTile ch = locate(A.cfirst_r, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
@@ -915,20 +933,20 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
   for (int l = threadIdx.x; l <= k; l += 64) used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
   __syncthreads();
   const int jend = min(ch.off + A.chunk, fb.n);
-  P q[kAcaIpt][9];
   bool valid[kAcaIpt];
 #pragma unroll
-  for (int s = 0; s < kAcaIpt; s++) {
-    const int j = ch.off + threadIdx.x + 64 * s;
-    valid[s] = j < jend;
-    if (valid[s])
-      sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + j), q[s]);
-    else
-#pragma unroll
-      for (int e = 0; e < 9; e++) q[s][e] = (P)0;
-  }
+  for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < jend;
+  P q[kAcaIpt][9];
   residual_cross<P, 1>(q, valid, Qp + 9 * (fb.qoff + ch.off + threadIdx.x), fb.n, k, piv, sh,
                        A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) {
+    if (!valid[s]) continue;
+    P raw[9];
+    sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + ch.off + threadIdx.x + 64 * s), raw);
+#pragma unroll
+    for (int e = 0; e < 9; e++) q[s][e] = raw[e] - q[s][e];
+  }
   P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
   ArgMax best{kNegHuge, -1};
   double g[6] = {0, 0, 0, 0, 0, 0};
@@ -1078,7 +1096,7 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
         }
       }
     }
-    wave_sum32_store(x, sh, W + 3 * l0, 3 * nl);
+    wave_sum32_store<double>(x, sh, W + 3 * l0, 3 * nl);
   }
 }
 
@@ -1637,10 +1655,11 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
 
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
+  AcaExpandArgs E{L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, L.tfirst, L.nactive,
+                  L.max_iter, L.row_pass, (AcaPassTile *)L.tiles};
   hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)((L.nactive + 255) / 256)), dim3(256), 0,
-                     st, L.tfirst, L.nactive, (AcaTile *)L.tiles);
-  AcaMlpArgs A{L.xp, L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, (const AcaTile *)L.tiles,
-               L.ntiles, L.max_iter, L.row_pass, L.Y};
+                     st, E);
+  AcaMlpArgs A{L.xp, (const AcaPassTile *)L.tiles, L.ntiles, L.row_pass, L.Y};
   const int64_t waves = (L.ntiles + 3) / 4;
   if (L.prec == PREC_FP32) {
     WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,

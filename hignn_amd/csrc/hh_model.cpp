@@ -626,8 +626,8 @@ struct hh_model {
         tr[na] = (int32_t)ntr;
         fc[na] = (int32_t)ncc;
         fr[na] = (int32_t)ncr;
-        tiles_c.reserve((size_t)ntc * 8);
-        tiles_r.reserve((size_t)ntr * 8);
+        tiles_c.reserve((size_t)ntc * 24);
+        tiles_r.reserve((size_t)ntr * 24);
         part_c.reserve((size_t)ncc * kAcaPartBytes);
         part_r.reserve((size_t)ncr * kAcaPartBytes);
         x_c.reserve((size_t)ncc * MI * 9 * 8);
