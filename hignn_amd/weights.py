@@ -4,13 +4,18 @@ The pair function is the TorchScript ``Net`` of python/convert.py:6-20 around
 ``Sequential(Linear(3,128), Tanh, Linear(128,512), Tanh, Linear(512,9))``
 (nn/two_body_unbounded.pkl).  Weights travel as one flat float32 vector in
 the order ``W1(128,3) | b1(128) | W2(512,128) | b2(512) | W3(9,512) | b3(9)``
-(71,177 floats), row-major, exactly the tensor layout of the pickle.
+(71,177 floats), row-major, exactly the tensor layout of the archives.
 
-``load_pkl`` reads the pickle as what it is on disk -- a zip archive whose
-``<root>/data/<k>`` members hold the raw little-endian float32 storages --
-without unpickling anything.  The storage keys 0..5 map to
-W1, b1, W2, b2, W3, b3 (the order in which ``torch.save`` wrote the
-Sequential's parameters).
+Both model files of the reference are zip archives: ``torch.save`` of the
+Sequential (``.pkl``) and ``torch.jit.save`` of ``Net`` (``.pt``,
+``<base>_<device>.pt`` for GPU runs, python/convert.py:41-57,
+src/HignnModel.cpp:404-411).  Each holds ``<root>/data.pkl`` describing the
+objects and ``<root>/data/<key>`` raw little-endian storages.  The loaders
+here never unpickle: ``data.pkl`` is read opcode by opcode
+(``pickletools.genops``) by a small stack machine that only records which
+names, dictionaries and tensor references the stream builds -- no global is
+imported and no callable is invoked -- and the tensors are then read from
+the raw storages.
 """
 from __future__ import annotations
 
@@ -66,38 +71,207 @@ def xavier(seed: int = 0) -> np.ndarray:
     return pack(parts)
 
 
-def load_pkl(path: str) -> np.ndarray:
-    """Raw weights from a torch-saved Sequential (zip of float32 storages)."""
+class _Ref:
+    """A global (module, name) named by the opcode stream; never imported."""
+
+    def __init__(self, module, name):
+        self.module, self.name = module, name
+
+
+class _Obj:
+    """An object the stream would build: constructor, arguments, state."""
+
+    def __init__(self, fn, args):
+        self.fn, self.args, self.state, self.items = fn, args, None, {}
+
+
+class _Pers:
+    def __init__(self, pid):
+        self.pid = pid
+
+
+def _walk_pickle(data: bytes):
+    """Replays the opcodes of a pickle symbolically; returns the root."""
+    import pickletools
+
+    stack, memo, marks = [], {}, []
+
+    def pop_mark():
+        i = marks.pop()
+        items = stack[i:]
+        del stack[i:]
+        return items
+
+    for op, arg, _ in pickletools.genops(data):
+        n = op.name
+        if n in ("PROTO", "FRAME", "STOP"):
+            continue
+        if n == "GLOBAL":
+            mod, _, name = arg.partition(" ")
+            stack.append(_Ref(mod, name))
+        elif n == "STACK_GLOBAL":
+            name = stack.pop()
+            mod = stack.pop()
+            stack.append(_Ref(mod, name))
+        elif n in ("BINPUT", "LONG_BINPUT", "PUT"):
+            memo[arg] = stack[-1]
+        elif n == "MEMOIZE":
+            memo[len(memo)] = stack[-1]
+        elif n in ("BINGET", "LONG_BINGET", "GET"):
+            stack.append(memo[arg])
+        elif n == "MARK":
+            marks.append(len(stack))
+        elif n == "EMPTY_TUPLE":
+            stack.append(())
+        elif n in ("TUPLE1", "TUPLE2", "TUPLE3"):
+            k = int(n[-1])
+            t = tuple(stack[-k:])
+            del stack[-k:]
+            stack.append(t)
+        elif n == "TUPLE":
+            stack.append(tuple(pop_mark()))
+        elif n == "EMPTY_LIST":
+            stack.append([])
+        elif n == "LIST":
+            stack.append(list(pop_mark()))
+        elif n == "APPEND":
+            v = stack.pop()
+            stack[-1].append(v)
+        elif n == "APPENDS":
+            vs = pop_mark()
+            stack[-1].extend(vs)
+        elif n == "EMPTY_DICT":
+            stack.append({})
+        elif n == "DICT":
+            vs = pop_mark()
+            stack.append(dict(zip(vs[0::2], vs[1::2])))
+        elif n in ("SETITEM", "SETITEMS"):
+            vs = [stack.pop(), stack.pop()][::-1] if n == "SETITEM" else pop_mark()
+            tgt = stack[-1]
+            d = tgt.items if isinstance(tgt, _Obj) else tgt
+            for k, v in zip(vs[0::2], vs[1::2]):
+                d[k] = v
+        elif n in ("EMPTY_SET", "FROZENSET"):
+            stack.append(set() if n == "EMPTY_SET" else frozenset(pop_mark()))
+        elif n == "ADDITEMS":
+            pop_mark()
+        elif n in ("REDUCE", "NEWOBJ"):
+            args = stack.pop()
+            fn = stack.pop()
+            stack.append(_Obj(fn, args))
+        elif n == "BUILD":
+            state = stack.pop()
+            if isinstance(stack[-1], _Obj):
+                stack[-1].state = state
+        elif n == "BINPERSID":
+            stack.append(_Pers(stack.pop()))
+        elif n == "NONE":
+            stack.append(None)
+        elif n in ("NEWTRUE", "NEWFALSE"):
+            stack.append(n == "NEWTRUE")
+        elif n in ("BININT", "BININT1", "BININT2", "LONG1", "LONG4", "INT", "LONG", "BINFLOAT",
+                   "FLOAT", "BINUNICODE", "SHORT_BINUNICODE", "BINUNICODE8", "UNICODE",
+                   "BINSTRING", "SHORT_BINSTRING", "STRING", "BINBYTES", "SHORT_BINBYTES",
+                   "BINBYTES8"):
+            stack.append(arg)
+        elif n == "POP":
+            stack.pop()
+        elif n == "POP_MARK":
+            pop_mark()
+        elif n == "DUP":
+            stack.append(stack[-1])
+        else:
+            raise ValueError(f"unsupported pickle opcode {n}")
+    return stack[-1] if stack else None
+
+
+def _tensors(node, path="", out=None, seen=None):
+    """(path, storage key, dtype name, offset, shape, stride) of every tensor
+    reference below node, in stream order."""
+    if out is None:
+        out, seen = [], set()
+    if id(node) in seen:
+        return out
+    if isinstance(node, (_Obj, dict, list, tuple)):
+        seen.add(id(node))
+    if isinstance(node, _Obj):
+        fn = node.fn
+        if isinstance(fn, _Ref) and fn.module == "torch._utils" and fn.name == "_rebuild_tensor_v2":
+            pers, offset, size, stride = node.args[:4]
+            pid = pers.pid
+            stype = pid[1]
+            dtype = stype.name if isinstance(stype, _Ref) else str(stype)
+            out.append((path, str(pid[2]), dtype, int(offset), tuple(size), tuple(stride)))
+            return out
+        if isinstance(fn, _Ref) and fn.module == "torch._utils" and fn.name == "_rebuild_parameter":
+            return _tensors(node.args[0], path, out, seen)
+        for k, v in node.items.items():
+            _tensors(v, f"{path}.{k}", out, seen)
+        if node.state is not None:
+            _tensors(node.state, path, out, seen)
+        if isinstance(node.args, tuple):
+            for v in node.args:
+                _tensors(v, path, out, seen)
+    elif isinstance(node, dict):
+        for k, v in node.items():
+            _tensors(v, f"{path}.{k}", out, seen)
+    elif isinstance(node, (list, tuple)):
+        for v in node:
+            _tensors(v, path, out, seen)
+    return out
+
+
+def load_torch_zip(path: str) -> np.ndarray:
+    """Raw weights from a torch-saved Sequential (.pkl) or a TorchScript Net
+    (.pt) archive, without unpickling: the six tensors of shapes W1, b1, W2,
+    b2, W3, b3, taken in the stream's order."""
     with zipfile.ZipFile(path) as z:
         names = z.namelist()
-        blobs = {}
-        for n in names:
-            head, _, tail = n.rpartition("/data/")
-            if head and tail.isdigit():
-                blobs[int(tail)] = z.read(n)
-    if sorted(blobs) != list(range(6)):
-        raise ValueError(f"{path}: expected storages 0..5, found {sorted(blobs)}")
-    parts = {}
-    for k, (name, shape) in enumerate(SHAPES):
-        a = np.frombuffer(blobs[k], dtype="<f4")
-        if a.size != int(np.prod(shape)):
-            raise ValueError(f"{path}: storage {k} has {a.size} floats, {name} needs {shape}")
-        parts[name] = a.reshape(shape)
+        pk = [n for n in names if n.endswith("/data.pkl")]
+        if len(pk) != 1:
+            raise ValueError(f"{path}: expected one data.pkl, found {pk}")
+        root = pk[0][: -len("data.pkl")]
+        refs = _tensors(_walk_pickle(z.read(pk[0])))
+        want = [shape for _, shape in SHAPES]
+        parts, k = {}, 0
+        for p_, key, dtype, offset, size, stride in refs:
+            if k < len(want) and size == want[k]:
+                if dtype != "FloatStorage":
+                    raise ValueError(f"{path}: tensor {p_} is {dtype}, expected float32")
+                n = int(np.prod(size))
+                expect = tuple(int(np.prod(size[i + 1:])) for i in range(len(size)))
+                if stride != expect:
+                    raise ValueError(f"{path}: tensor {p_} is not contiguous")
+                raw = np.frombuffer(z.read(f"{root}data/{key}"), dtype="<f4")
+                parts[SHAPES[k][0]] = raw[offset:offset + n].reshape(size)
+                k += 1
+        if k != len(want):
+            raise ValueError(f"{path}: found {k} of the 6 MLP tensors ({[r[4] for r in refs]})")
     return pack(parts)
 
 
-def resolve_model_path(base: str) -> str:
-    """load_two_body_model(base) resolution (src/HignnModel.cpp:404-411 loads
-    ``base.pt`` / ``base_<dev>.pt``): here ``base.pkl`` (raw weights) or a
-    ``.npy`` flat vector; TorchScript archives are not parsed."""
-    for cand in (base, base + ".pkl", base + ".npy"):
+def load_pkl(path: str) -> np.ndarray:
+    """Raw weights from a torch-saved Sequential (zip of float32 storages)."""
+    return load_torch_zip(path)
+
+
+def resolve_model_path(base: str, device: int | None = None) -> str:
+    """load_two_body_model(base) resolution: the reference loads
+    ``base_<device>.pt`` on GPU builds and ``base.pt`` otherwise
+    (src/HignnModel.cpp:404-411).  Tried here in this order: ``base`` itself,
+    ``base_<device>.pt`` (device from LOCAL_RANK, default 0), ``base.pt``,
+    ``base.pkl`` (the torch-saved Sequential) and ``base.npy`` (a flat float32
+    vector)."""
+    if device is None:
+        device = int(os.environ.get("LOCAL_RANK", "0"))
+    for cand in (base, f"{base}_{device}.pt", base + ".pt", base + ".pkl", base + ".npy"):
         if os.path.isfile(cand):
             return cand
-    raise FileNotFoundError(f"no two-body model at {base}(.pkl|.npy)")
+    raise FileNotFoundError(f"no two-body model at {base}(_<device>.pt|.pt|.pkl|.npy)")
 
 
 def load(base: str) -> np.ndarray:
     path = resolve_model_path(base)
     if path.endswith(".npy"):
         return pack(unpack(np.load(path, allow_pickle=False)))
-    return load_pkl(path)
+    return load_torch_zip(path)
