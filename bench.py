@@ -30,6 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
+# HBM bytes per pair-function evaluation (read + written), measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this benchmark at N=1M, each
+# scaled by the counter's reading for a 1 GiB device copy (x2.0 and x1.0):
+# the ACA passes write 36 B of entries and read 9.5 B; the near kernel's
+# traffic is mostly its FP64 velocity atomics.
+TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": 7.25}
 PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # dense MFMA TFLOP/s
 
 
@@ -208,7 +214,8 @@ def main():
     launches_k = sum(p[dom][2] for p in profs)
     achieved = MLP_FLOP * pairs_k / (ms_k / 1000.0) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[args.prec],
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4),
+            "traffic": int(TRAFFIC_PER_PAIR[dom] * pairs_k / max(1, launches_k)),
             "kernel": f"{kern[dom][0]}<{args.prec}>",
             "flop_per_launch": int(MLP_FLOP * pairs_k / max(1, launches_k)),
             "avg_launch_ms": round(ms_k / max(1, launches_k), 3),

@@ -43,6 +43,7 @@ def lib():
             "hh_model_set_max_far_field_distance": [c_vp, c_dbl],
             "hh_model_set_precision": [c_vp, c_int],
             "hh_model_update_coord": [c_vp, c_vp, c_i64],
+            "hh_model_set_tree_reuse": [c_vp, c_int],
             "hh_model_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dense_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dot_device": [c_vp, c_vp, c_vp, c_i64, c_vp],
@@ -127,8 +128,10 @@ class Model:
         _chk(lib().hh_model_load_two_body(self.h, _p(w), w.size))
 
     def set(self, epsilon=None, max_iter=None, use_symmetry=None, max_relative_coord=None,
-            precision=None):
+            precision=None, tree_reuse=None):
         L = lib()
+        if tree_reuse is not None:
+            _chk(L.hh_model_set_tree_reuse(self.h, int(bool(tree_reuse))))
         if epsilon is not None:
             _chk(L.hh_model_set_epsilon(self.h, float(epsilon)))
         if max_iter is not None:

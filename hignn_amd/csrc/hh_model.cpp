@@ -151,6 +151,7 @@ struct hh_model {
   int max_iter = 100, pool_factor = 40, far_work = 5000;
   int64_t max_rel_coord = 500000;
   bool post_check = false, use_symmetry = true;
+  bool tree_reuse = false;  // update_coord keeps the last tree and lists (opt-in)
   int prec = PREC_FP32;
   // byte budget of one far-field batch (factor pools + Gram histories); 0:
   // half the device memory free at the first far field, at most 192 GiB
@@ -338,6 +339,18 @@ struct hh_model {
 
   // ---- update_coord ------------------------------------------------------
   void update() {
+    if (tree_reuse && built) {
+      // keep the permutation, tree and block lists of the last build; only
+      // the positions (in that tree order) are refreshed
+      std::vector<float> xph(4 * (size_t)n);
+      for (int64_t i = 0; i < n; i++) {
+        const int64_t s = tl.perm[(size_t)i];
+        for (int d = 0; d < 3; d++) xph[4 * (size_t)i + d] = x[3 * (size_t)s + d];
+        xph[4 * (size_t)i + 3] = 0.0f;
+      }
+      upload(xp, xph.data(), xph.size() * 4);
+      return;
+    }
     build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
     // positions in tree order
     std::vector<float> xph(4 * (size_t)n);
@@ -1010,6 +1023,13 @@ int hh_model_set_precision(hh_model *m, int p) {
     if (p != HH_PREC_FP32 && p != HH_PREC_BF16 && p != HH_PREC_FP16)
       throw HHError(HH_ERR_ARG, "unknown precision");
     m->prec = p;
+  });
+}
+
+int hh_model_set_tree_reuse(hh_model *m, int flag) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    m->tree_reuse = flag != 0;
   });
 }
 

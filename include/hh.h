@@ -66,6 +66,13 @@ int hh_model_set_max_far_field_distance(hh_model *m, double distance); /* stored
  * tree + close/far lists (Build + CloseFarCheck). */
 int hh_model_update_coord(hh_model *m, const float *coord, int64_t n);
 
+/* Opt-in tree reuse (no counterpart in the reference, which rebuilds on every
+ * update_coord): with flag != 0, update_coord after a first build keeps the
+ * permutation, tree and close/far lists and only refreshes the positions --
+ * for the stages of one integrator step, where particles move little.  The
+ * caller decides when to rebuild (flag 0, then update_coord). */
+int hh_model_set_tree_reuse(hh_model *m, int flag);
+
 /* dot(u, f) -- src/HignnModel.cpp:545-634: u = M(x) f, host arrays. */
 int hh_model_dot(hh_model *m, float *u, const float *f, int64_t n);
 

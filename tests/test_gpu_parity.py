@@ -210,3 +210,34 @@ def test_dot_fp32_parity_tolerance_stops(model_small, xavier_w, monkeypatch, eps
         np.testing.assert_array_equal(ranks, oranks)
     else:
         assert same >= 0.97 and np.abs(ranks - oranks).max() <= 8
+
+
+def test_tree_reuse_keeps_lists_and_tracks_positions(xavier_w):
+    """Opt-in tree reuse: after a first build, update_coord keeps the lists
+    and refreshes the positions.  The matvec at the moved positions then
+    equals the FP64 evaluation of the kept block structure at those
+    positions, to the ACA tolerance, and moving back restores the first
+    result (up to the order of the FP64 atomic sums)."""
+    x0 = workloads.uniform_cube(1200, 0.05, 0)
+    g = np.random.default_rng(11)
+    x1 = (x0 + 0.05 * g.standard_normal(x0.shape)).astype(np.float32)
+    f = workloads.gravity(x0.shape[0])
+    m = abi.Model(x0, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50, tree_reuse=True)
+    m.update_coord()
+    e0 = m.export()
+    u0 = m.dot(f)
+    m.update_coord(x1)
+    e1 = m.export()
+    for k in e0:
+        if k != "aux":
+            np.testing.assert_array_equal(e0[k], e1[k], err_msg=k)
+    u1 = m.dot(f)
+    tg = np.arange(0, x0.shape[0], 7)
+    err = O.rel_l2(u1[tg], m.check_fp64(f, tg, 0))
+    moved = O.rel_l2(u1, u0)
+    print("tree reuse: error vs fp64 of the kept blocks", err, "change from x0", moved)
+    assert err <= 1e-3 and moved > 0.0
+    m.update_coord(x0)
+    np.testing.assert_allclose(m.dot(f), u0, rtol=1e-5, atol=1e-6 * np.abs(u0).max())
