@@ -494,31 +494,56 @@ __global__ void expand_tiles_kernel(AcaExpandArgs A) {
   }
 }
 
+// Displacements of the four tiles q0..q0+3 of a wave (lane p: item p of each
+// tile), loaded one iteration ahead of their use.
+struct AcaPassIn {
+  float4 xi[4], xv[4];
+  int64_t yrow[4];
+  bool valid[4];
+};
+
+__device__ __forceinline__ void aca_pass_load(const AcaMlpArgs &A, int64_t q0, int p,
+                                              AcaPassIn &in) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int64_t q = q0 + t < A.ntiles ? q0 + t : A.ntiles - 1;
+    const AcaPassTile T = A.tiles[q];
+    in.valid[t] = (q0 + t < A.ntiles) && p < T.nvalid;
+    const int pc = in.valid[t] ? p : 0;
+    in.xi[t] = A.xp[T.item0 + pc];
+    in.xv[t] = A.xp[T.piv];
+    in.yrow[t] = T.yrow0 + pc;
+  }
+}
+
 template <class Core>
 __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, p = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t q0 = 4 * wave; q0 < A.ntiles; q0 += 4 * nwaves) {
+  int64_t q0 = 4 * wave;
+  if (q0 >= A.ntiles) return;
+  AcaPassIn cur;
+  aca_pass_load(A, q0, p, cur);
+  for (; q0 < A.ntiles; q0 += 4 * nwaves) {
     float rx[4], ry[4], rz[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (!A.row_pass) {  // pivot - item
+        rx[t] = cur.xv[t].x - cur.xi[t].x, ry[t] = cur.xv[t].y - cur.xi[t].y,
+        rz[t] = cur.xv[t].z - cur.xi[t].z;
+      } else {  // item - pivot
+        rx[t] = cur.xi[t].x - cur.xv[t].x, ry[t] = cur.xi[t].y - cur.xv[t].y,
+        rz[t] = cur.xi[t].z - cur.xv[t].z;
+      }
+    }
     bool valid[4];
     int64_t yrow[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int64_t q = q0 + t < A.ntiles ? q0 + t : A.ntiles - 1;
-      const AcaPassTile T = A.tiles[q];
-      valid[t] = (q0 + t < A.ntiles) && p < T.nvalid;
-      const int pc = valid[t] ? p : 0;
-      const float4 xi = A.xp[T.item0 + pc];
-      const float4 xv = A.xp[T.piv];
-      if (!A.row_pass) {  // pivot - item
-        rx[t] = xv.x - xi.x, ry[t] = xv.y - xi.y, rz[t] = xv.z - xi.z;
-      } else {  // item - pivot
-        rx[t] = xi.x - xv.x, ry[t] = xi.y - xv.y, rz[t] = xi.z - xv.z;
-      }
-      yrow[t] = T.yrow0 + pc;
-    }
+    for (int t = 0; t < 4; t++) valid[t] = cur.valid[t], yrow[t] = cur.yrow[t];
+    // the next tiles' loads are in flight while this MLP runs
+    if (q0 + 4 * nwaves < A.ntiles) aca_pass_load(A, q0 + 4 * nwaves, p, cur);
     f32x4 y[4];
     core(rx, ry, rz, y);
 #pragma unroll
@@ -532,7 +557,7 @@ __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
   }
 }
 
-__global__ __launch_bounds__(256) void aca_mlp_fp32_kernel(AcaMlpArgs A, WF<float> W) {
+__global__ __launch_bounds__(256, 2) void aca_mlp_fp32_kernel(AcaMlpArgs A, WF<float> W) {
   aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
                       f32x4(&y)[4]) { mlp_tiles_f<float, 4>(W, rx, ry, rz, y); });
 }
