@@ -684,21 +684,24 @@ __device__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// Wave sums (in FP64) of 32 per-lane values of a one-wave workgroup, the
-// first nvalid stored to dst.  The values are transposed through LDS (sh: 32
-// rows of 65 values of type T, the odd stride spreads the rows over the
+// Wave sums (in FP64) of N <= 32 per-lane values of a one-wave workgroup,
+// the first nvalid stored to dst.  The values are transposed through LDS (sh:
+// N rows of 65 values of type T, the odd stride spreads the rows over the
 // banks); lanes e and e + 32 each add half of row e and one exchange
 // completes the sum.
-template <class T>
-__device__ __forceinline__ void wave_sum32_store(const T (&x)[32], T *sh, double *dst, int nvalid) {
+template <class T, int N>
+__device__ __forceinline__ void wave_sum_store(const T (&x)[N], T *sh, double *dst, int nvalid) {
+  static_assert(N <= 32, "at most 32 values");
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int e = 0; e < 32; e++) sh[e * 65 + lane] = x[e];
+  for (int e = 0; e < N; e++) sh[e * 65 + lane] = x[e];
   __syncthreads();
   const int e = lane & 31, h = lane >> 5;
   double t = 0.0;
+  if (e < N) {
 #pragma unroll
-  for (int q = 0; q < 32; q++) t += (double)sh[e * 65 + 32 * h + q];
+    for (int q = 0; q < 32; q++) t += (double)sh[e * 65 + 32 * h + q];
+  }
   t += __shfl_xor(t, 32, 64);
   if (lane < nvalid) dst[lane] = t;
   __syncthreads();
@@ -746,7 +749,8 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 //                x[9l + 3d1 + d2] = sum_j sum_col Q_{k-1}(j)[d1][col] Q_l(j)[d2][col]
 // kCrossLC iterations per wave reduction.  pool points at the lane's first
 // item of C_0 (Q_0); the lane's items are 64 apart.
-constexpr int kCrossLC = 3;
+constexpr int kCrossLC = 1;
+constexpr int kCrossN = 9 * kCrossLC;  // cross sums per wave reduction
 
 template <class P, int SIDE>
 __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
@@ -764,9 +768,9 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
   for (int l0 = 0; l0 < k; l0 += kCrossLC) {
     const int nl = min(kCrossLC, k - l0);
     const bool cross = l0 < ncross;  // uniform
-    P x[32];
+    P x[kCrossN];
 #pragma unroll
-    for (int e = 0; e < 32; e++) x[e] = (P)0;
+    for (int e = 0; e < kCrossN; e++) x[e] = (P)0;
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++) {
 #pragma unroll
@@ -809,7 +813,7 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
           }
       }
     }
-    if (cross) wave_sum32_store<P>(x, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
+    if (cross) wave_sum_store<P, kCrossN>(x, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
   }
 }
 
@@ -820,7 +824,7 @@ template <class P>
 __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[128];
-  __shared__ P sh[32 * 65];
+  __shared__ P sh[kCrossN * 65];
   const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
@@ -947,7 +951,7 @@ template <class P>
 __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[129];
-  __shared__ P sh[32 * 65];
+  __shared__ P sh[kCrossN * 65];
   const AcaTile ch = locate(A.cfirst_r, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
@@ -1121,7 +1125,7 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
         }
       }
     }
-    wave_sum32_store<double>(x, sh, W + 3 * l0, 3 * nl);
+    wave_sum_store<double, 32>(x, sh, W + 3 * l0, 3 * nl);
   }
 }
 
