@@ -202,7 +202,8 @@ class Model:
 
 
 def header_symbols():
-    """Function names declared in include/hh.h."""
+    """Function names declared in the headers of include/ (hh.h, hh_dynamics.h)."""
+    import glob
     import re
-    txt = open(HEADER).read()
+    txt = "".join(open(p).read() for p in sorted(glob.glob(os.path.join(os.path.dirname(HEADER), "*.h"))))
     return sorted(set(re.findall(r"\b(hh_[a-z0-9_]+)\s*\(", txt)))

@@ -37,7 +37,8 @@ def _run(cmd):
 
 
 def build_libhh():
-    srcs = [os.path.join(CSRC, f) for f in ("hh_kernels.hip", "hh_model.cpp", "hh_tree.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("hh_kernels.hip", "hh_forcing.hip", "hh_model.cpp", "hh_tree.cpp",
+                                             "hh_dynamics.cpp")]
     tmp = LIBHH + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", *srcs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIBHH)

@@ -20,45 +20,18 @@
 #include <vector>
 
 #include "../../include/hh.h"
+#include "../../include/hh_dynamics.h"
+#include "hh_host.h"
 #include "hh_internal.h"
 #include "hh_kernels.h"
 
 using namespace hh;
 
-namespace {
+namespace hh {
 
 thread_local std::string g_last_error;
 int g_device = -1;
 int g_cus = 256;
-
-struct HHError : std::runtime_error {
-  int code;
-  HHError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t _e = (expr);                                                            \
-    if (_e != hipSuccess)                                                              \
-      throw HHError(HH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-  } while (0)
-
-template <class F>
-int guarded(F &&f) {
-  try {
-    f();
-    return HH_OK;
-  } catch (const HHError &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return HH_ERR_ALLOC;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return HH_ERR_ARG;
-  }
-}
 
 void ensure_device() {
   if (g_device >= 0) return;
@@ -74,56 +47,9 @@ void ensure_device() {
   g_device = dev;
 }
 
-// Device buffer that only grows.
-struct DBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  void reserve(size_t b) {
-    if (b <= bytes) return;
-    if (p) HIPCHK(hipFree(p));
-    p = nullptr;
-    bytes = 0;
-    if (b == 0) return;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-      p = nullptr;
-      throw HHError(HH_ERR_ALLOC, "hipMalloc(" + std::to_string(b) + "): " + hipGetErrorString(e));
-    }
-    bytes = b;
-  }
-  template <class T>
-  T *as() const {
-    return (T *)p;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
+}  // namespace hh
 
-// Pinned host staging buffer that only grows.
-struct PBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  void reserve(size_t b) {
-    if (b <= bytes) return;
-    if (p) HIPCHK(hipHostFree(p));
-    p = nullptr;
-    bytes = 0;
-    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
-    bytes = b;
-  }
-  template <class T>
-  T *as() const {
-    return (T *)p;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
+namespace {
 
 uint16_t to_bf16(float f) {
   uint32_t u;
@@ -1038,6 +964,23 @@ int hh_model_update_coord(hh_model *m, const float *coord, int64_t n) {
     if (!m || !coord) throw HHError(HH_ERR_ARG, "NULL argument");
     if (n != m->n) throw HHError(HH_ERR_ARG, "particle count differs from the model's");
     m->x.assign(coord, coord + 3 * n);
+    m->update();
+  });
+}
+
+int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n, void *stream) {
+  return guarded([&]() {
+    if (!m || !coord_dev) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "particle count differs from the model's");
+    const hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+    if (m->tree_reuse && m->built) {
+      // kept tree: positions in tree order straight from the device array
+      HIPCHK(launch_gather_forces(coord_dev, n, m->perm.as<int32_t>(), m->xp.as<float4>(), st));
+      return;
+    }
+    m->x.resize(3 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(m->x.data(), coord_dev, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     m->update();
   });
 }

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/hh.h"
+#include "../../include/hh_dynamics.h"
 
 namespace py = pybind11;
 
@@ -59,6 +60,8 @@ class Model {
     check(hh_model_create(x.data(), n_, block_size, &m_));
   }
   ~Model() { hh_model_destroy(m_); }
+  hh_model *handle() const { return m_; }
+  int64_t size() const { return n_; }
 
   // load_two_body_model(base): base_<device>.pt or base.pt (TorchScript, as
   // src/HignnModel.cpp:404-411), base.pkl (torch-saved Sequential), base.npy,
@@ -185,6 +188,51 @@ class Model {
 };
 
 // ---------------------------------------------------------------------------
+// Forcing on the device (include/hh_dynamics.h).  PotentialForce keeps the
+// reference's class (interface/HignnInterface.cpp:66-71, PotentialForce.hpp):
+// get_potential_force(coord) returns the Morse force of every particle.
+// DeviceForcing adds the body force and filament terms of
+// python/simulate.py:90-144 and is what set_device_velocity runs per stage.
+// ---------------------------------------------------------------------------
+class Forcing {
+ public:
+  Forcing() { check(hh_forcing_create(&f_)); }
+  ~Forcing() { hh_forcing_destroy(f_); }
+  hh_forcing *handle() const { return f_; }
+  void set_periodic(bool flag) { check(hh_forcing_set_periodic(f_, flag)); }
+  void set_domain(py::array_t<float, py::array::forcecast> domain) {
+    if (domain.ndim() != 2) throw std::runtime_error("Rank of domain must be two");
+    if (domain.shape(0) < 2 || domain.shape(1) < 3)
+      throw std::invalid_argument("domain must have shape (2, 3)");
+    auto d = domain.unchecked<2>();
+    const float lo[3] = {d(0, 0), d(0, 1), d(0, 2)}, hi[3] = {d(1, 0), d(1, 1), d(1, 2)};
+    check(hh_forcing_set_domain(f_, lo, hi));
+  }
+  void set_two_body_epsilon(float eps) { check(hh_forcing_set_morse(f_, eps)); }
+  void set_body_force(std::vector<float> b) {
+    if (b.size() != 3) throw std::invalid_argument("body force must have 3 components");
+    check(hh_forcing_set_body(f_, b.data()));
+  }
+  void set_filaments(int64_t n_filament, int64_t n_chain, float rest_length, float k_t,
+                     double k_b) {
+    check(hh_forcing_set_filaments(f_, n_filament, n_chain, rest_length, k_t, k_b));
+  }
+  py::array_t<float> eval(py::array_t<float, py::array::forcecast> coord) {
+    if (coord.ndim() != 2) throw std::runtime_error("Rank of target sites must be two");
+    auto x = rows3(coord, "coord");
+    const py::ssize_t n = coord.shape(0);
+    py::array_t<float> out({n, (py::ssize_t)3});
+    check(hh_forcing_eval(f_, x.data(), out.mutable_data(), (int64_t)n));
+    return out;
+  }
+
+ private:
+  hh_forcing *f_ = nullptr;
+};
+
+class PotentialForceCls : public Forcing {};
+
+// ---------------------------------------------------------------------------
 // Time integrators (include/TimeIntegrator.hpp).  Positions are float
 // triples updated with float arithmetic in the reference's order.  The
 // reference never sets angular velocities (TimeIntegrator.hpp:137 is
@@ -199,7 +247,9 @@ struct V3 {
 
 class Integrator {
  public:
-  virtual ~Integrator() = default;
+  virtual ~Integrator() {
+    if (stepper_) hh_stepper_destroy(stepper_);
+  }
   void set_time_step(float dt) { dt_ = dt; }
   void set_final_time(float t) { tf_ = t; }
   virtual void set_num_rigid_body(size_t n) {
@@ -229,6 +279,35 @@ class Integrator {
     }
   }
   int func_count() const { return count_; }
+
+  // Device-resident velocity: every stage evaluates `forcing` on the device,
+  // updates the model's coordinates and applies M(x) on the device; no
+  // Python callback and no host copy of the particle arrays per stage.
+  void set_device_velocity(Model &model, Forcing &forcing) {
+    if ((size_t)model.size() != n_ && n_ != 0)
+      throw std::runtime_error("model and integrator sizes differ");
+    dm_ = &model;
+    df_ = &forcing;
+    if (stepper_) {
+      hh_stepper_destroy(stepper_);
+      stepper_ = nullptr;
+    }
+  }
+  void set_graphs(bool flag) { graphs_ = flag; }
+  py::array_t<float> positions() const {
+    py::array_t<float> p({(py::ssize_t)n_, (py::ssize_t)3});
+    auto a = p.mutable_unchecked<2>();
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) a(i, j) = pos_[i].v[j];
+    return p;
+  }
+  py::dict device_stats() const {
+    py::dict d;
+    d["velocity_evaluations"] = stats_[0];
+    d["accepted"] = stats_[1];
+    d["rejected"] = stats_[2];
+    return d;
+  }
 
  protected:
   void set_lim(int d, py::list l) {
@@ -287,12 +366,52 @@ class Integrator {
   std::vector<float> ang_;
   std::function<py::array_t<float>(float, py::array_t<float>)> vf_;
   std::function<void(py::array_t<float>, py::array_t<float>)> of_;
+
+  // ---- device-resident mode (set_device_velocity) ----
+  Model *dm_ = nullptr;
+  Forcing *df_ = nullptr;
+  hh_stepper *stepper_ = nullptr;
+  bool graphs_ = true;
+  int64_t stats_[3] = {0, 0, 0};
+
+  bool device_mode() const { return dm_ != nullptr; }
+
+  static void device_output(void *user, int64_t, double, const float *, const float *) {
+    auto *self = static_cast<Integrator *>(user);
+    py::array_t<float> p({(py::ssize_t)self->n_, (py::ssize_t)3}),
+        v({(py::ssize_t)self->n_, (py::ssize_t)3});
+    check(hh_stepper_state(self->stepper_, p.mutable_data(), v.mutable_data(), nullptr, nullptr));
+    self->of_(p, v);
+  }
+
+  // Runs the integration on the device from the current positions (a run
+  // starts at t = 0, as the host loops do).
+  void run_device(int method, double threshold) {
+    if ((size_t)dm_->size() != n_) throw std::runtime_error("model and integrator sizes differ");
+    if (!stepper_) check(hh_stepper_create(dm_->handle(), df_->handle(), method, &stepper_));
+    check(hh_stepper_set(stepper_, dt_, tf_, threshold));
+    if (periodic_)
+      for (int d = 0; d < 3; d++) check(hh_stepper_set_lim(stepper_, d, lim_[0][d], lim_[1][d]));
+    check(hh_stepper_set_graphs(stepper_, graphs_));
+    std::vector<float> x(3 * n_);
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) x[3 * i + j] = pos_[i].v[j];
+    check(hh_stepper_initialize(stepper_, x.data(), (int64_t)n_));
+    check(hh_stepper_set_output(stepper_, out_step_, custom_out_ ? &device_output : nullptr, this));
+    check(hh_stepper_run(stepper_, 0));
+    std::vector<float> v(3 * n_);
+    check(hh_stepper_state(stepper_, x.data(), v.data(), nullptr, stats_));
+    for (size_t i = 0; i < n_; i++)
+      for (int j = 0; j < 3; j++) pos_[i].v[j] = x[3 * i + j], vel_[i].v[j] = v[3 * i + j];
+    count_ = (int)stats_[1];
+  }
 };
 
 // ExplicitEuler::Run (TimeIntegrator.hpp:431-488).
 class Euler : public Integrator {
  public:
   void run() {
+    if (device_mode()) return run_device(HH_STEP_EULER, 0.0);
     count_ = 0;
     double t = 0.0;
     const double h0 = dt_;
@@ -338,6 +457,7 @@ class Rk4 : public Integrator {
                 dc4 = b4 - 393.0f / 640.0f, dc5 = b5 - (-92097.0f / 339200.0f),
                 dc6 = b6 - 187.0f / 2100.0f, dc7 = -1.0f / 40.0f;
     const float c[7] = {0.0f, 1.0f / 5.0f, 3.0f / 10.0f, 4.0f / 5.0f, 8.0f / 9.0f, 1.0f, 1.0f};
+    if (device_mode()) return run_device(HH_STEP_DOPRI, thr_);
     const float A[6][5] = {{a21, 0, 0, 0, 0},       {a31, a32, 0, 0, 0},
                            {a41, a42, a43, 0, 0},   {a51, a52, a53, a54, 0},
                            {a61, a62, a63, a64, a65}, {b1, 0, b3, b4, b5}};
@@ -448,7 +568,12 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_output_step", &Euler::set_output_step)
       .def("initialize", &Euler::initialize)
       .def("run", &Euler::run)
-      .def("func_count", &Euler::func_count);
+      .def("func_count", &Euler::func_count)
+      .def("set_device_velocity", &Euler::set_device_velocity, py::keep_alive<1, 2>(),
+           py::keep_alive<1, 3>())
+      .def("set_graphs", &Euler::set_graphs)
+      .def("positions", &Euler::positions)
+      .def("device_stats", &Euler::device_stats);
 
   py::class_<Rk4>(m, "ExplicitRk4")
       .def(py::init())
@@ -464,5 +589,24 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_output_step", &Rk4::set_output_step)
       .def("initialize", &Rk4::initialize)
       .def("run", &Rk4::run)
-      .def("func_count", &Rk4::func_count);
+      .def("func_count", &Rk4::func_count)
+      .def("set_device_velocity", &Rk4::set_device_velocity, py::keep_alive<1, 2>(),
+           py::keep_alive<1, 3>())
+      .def("set_graphs", &Rk4::set_graphs)
+      .def("positions", &Rk4::positions)
+      .def("device_stats", &Rk4::device_stats);
+
+  py::class_<Forcing>(m, "DeviceForcing")
+      .def(py::init())
+      .def("set_body_force", &Forcing::set_body_force)
+      .def("set_filaments", &Forcing::set_filaments, py::arg("n_filament"), py::arg("n_chain"),
+           py::arg("rest_length"), py::arg("k_t"), py::arg("k_b"))
+      .def("set_two_body_epsilon", &Forcing::set_two_body_epsilon)
+      .def("set_periodic", &Forcing::set_periodic, py::arg("is_periodic") = false)
+      .def("set_domain", &Forcing::set_domain)
+      .def("eval", &Forcing::eval);
+
+  py::class_<PotentialForceCls, Forcing>(m, "PotentialForce")
+      .def(py::init())
+      .def("get_potential_force", &Forcing::eval);
 }

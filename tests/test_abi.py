@@ -1,4 +1,4 @@
-"""libhh loads and exports every symbol that include/hh.h declares; the
+"""libhh loads and exports every symbol that include/*.h declare; the
 pybind11 module exposes the reference's surface (no device calls here)."""
 import ctypes
 

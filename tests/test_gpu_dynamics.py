@@ -1,0 +1,190 @@
+"""Device forcing and the device-resident integrators (GPU).
+
+The forcing kernels are checked against oracle/forcing.py; the device
+integrators against the host integrators of the pybind11 module driven by a
+Python velocity callback that performs the same stage (forcing, update_coord,
+dot) through the host API, so both sides see the same matvec kernels.
+"""
+import numpy as np
+import pytest
+
+from oracle import forcing as F
+
+pytestmark = pytest.mark.gpu
+
+f32 = np.float32
+
+
+def hignn():
+    from hignn_amd import hignn as h
+    h.Init()
+    return h
+
+
+def suspension(seed, n_fil=8, n_chain=40, n_free=400, box=60.0, step=2.0):
+    """n_fil random-walk chains (bond length `step`) then free particles."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _ in range(n_fil):
+        d = rng.normal(size=(n_chain, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        parts.append(rng.uniform(0.2 * box, 0.8 * box, 3) + np.cumsum(step * d, axis=0))
+    parts.append(rng.uniform(0, box, size=(n_free, 3)))
+    return np.concatenate(parts).astype(f32)
+
+
+FIL = dict(n_fil=8, n_chain=40, rest=2.0, k_t=5.0, k_b=3.0)
+
+
+def forcing(h, body=(0.0, 0.0, -1.0), morse=5.0):
+    fc = h.DeviceForcing()
+    fc.set_body_force(list(body))
+    fc.set_filaments(FIL["n_fil"], FIL["n_chain"], FIL["rest"], FIL["k_t"], FIL["k_b"])
+    if morse:
+        fc.set_two_body_epsilon(morse)
+    return fc
+
+
+def test_device_forcing_matches_oracle():
+    h = hignn()
+    x = suspension(0)
+    got = forcing(h).eval(x)
+    want = F.velocity_force(x, (0, 0, -1), FIL["n_fil"], FIL["n_chain"], FIL["rest"], FIL["k_t"],
+                            FIL["k_b"], morse=F.morse_force(x, 5.0))
+    # same float32 terms on both sides; the sums differ only in order (double)
+    scale = np.abs(want).max()
+    err = np.abs(got - want).max() / scale
+    print(f"forcing: max |f| {scale:.3g}, max rel diff {err:.2e}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("eps", [3.0, 7.5])
+def test_potential_force_matches_oracle(eps):
+    h = hignn()
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-20, 20, size=(1500, 3)).astype(f32)
+    pf = h.PotentialForce()
+    pf.set_two_body_epsilon(eps)
+    got = pf.get_potential_force(x)
+    want = F.morse_force(x, eps)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
+def test_potential_force_periodic_matches_oracle():
+    h = hignn()
+    rng = np.random.default_rng(2)
+    lo, hi = np.array([-10, -12, 0], f32), np.array([14, 10, 25], f32)
+    x = rng.uniform(lo, hi, size=(900, 3)).astype(f32)
+    x[0] = lo + 0.5  # a corner particle: 8 images
+    pf = h.PotentialForce()
+    pf.set_two_body_epsilon(4.0)
+    pf.set_domain(np.stack([lo, hi]))
+    got = pf.get_potential_force(x)
+    want = F.morse_force(x, 4.0, periodic=True, lo=lo, hi=hi)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+    # set_periodic(False) returns to the plain search
+    pf.set_periodic(False)
+    np.testing.assert_allclose(pf.get_potential_force(x), F.morse_force(x, 4.0),
+                               atol=1e-6 * np.abs(want).max())
+
+
+def test_periodic_without_domain_is_rejected():
+    h = hignn()
+    pf = h.PotentialForce()
+    pf.set_two_body_epsilon(2.0)
+    pf.set_periodic(True)
+    with pytest.raises(RuntimeError, match="domain"):
+        pf.get_potential_force(np.zeros((4, 3), f32))
+
+
+def model(h, x, w, reuse=False):
+    m = h.HignnModel(x, 64)
+    m.load_two_body_weights(w)
+    m.set_epsilon(1e-3)
+    m.set_max_iter(20)
+    m.set_tree_reuse(reuse)
+    return m
+
+
+def integrate(h, cls, x, w, device, dt=0.02, tf=0.1, thr=None, lims=None, reuse=False,
+              graphs=True, morse=5.0):
+    m = model(h, x, w, reuse)
+    fc = forcing(h, morse=morse)
+    it = cls()
+    it.set_time_step(dt)
+    it.set_final_time(tf)
+    it.set_num_rigid_body(x.shape[0])
+    if thr is not None:
+        it.set_threshold(thr)
+    if lims is not None:
+        it.set_x_lim(lims[0])
+        it.set_y_lim(lims[1])
+        it.set_z_lim(lims[2])
+    outs = []
+    it.set_output_func(lambda p, v: outs.append(np.array(p)))
+    if device:
+        it.set_device_velocity(m, fc)
+        it.set_graphs(graphs)
+    else:
+        def vel(t, pos):
+            xx = np.ascontiguousarray(pos[:, :3])
+            m.update_coord(xx)
+            f = fc.eval(xx)
+            u = np.zeros_like(f)
+            m.dot(u, f)
+            return u
+        it.set_velocity_func(vel)
+    state = np.zeros((x.shape[0], 6), f32)
+    state[:, :3] = x
+    it.initialize(state)
+    it.run()
+    return it.positions(), it.func_count(), outs
+
+
+def close(a, b, rel=1e-5):
+    return np.abs(a - b).max() <= rel * np.abs(b).max()
+
+
+@pytest.mark.parametrize("reuse", [False, True])
+def test_device_euler_matches_host(xavier_w, reuse):
+    h = hignn()
+    x = suspension(3)
+    xh, ch, oh = integrate(h, h.ExplicitEuler, x, xavier_w, device=False, reuse=reuse)
+    xd, cd, od = integrate(h, h.ExplicitEuler, x, xavier_w, device=True, reuse=reuse)
+    assert ch == cd == 5
+    assert len(oh) == len(od) == 5
+    print("euler max |x_dev - x_host|:", np.abs(xd - xh).max(), "moved:", np.abs(xh - x).max())
+    assert np.abs(xh - x).max() > 1e-3  # the particles moved
+    assert close(xd, xh)
+    for a, b in zip(od, oh):
+        assert close(a, b)
+
+
+def test_device_euler_graphs_match_eager(xavier_w):
+    h = hignn()
+    x = suspension(4)
+    xg, _, _ = integrate(h, h.ExplicitEuler, x, xavier_w, device=True, graphs=True)
+    xe, _, _ = integrate(h, h.ExplicitEuler, x, xavier_w, device=True, graphs=False)
+    assert close(xg, xe, 1e-6)
+
+
+def test_device_euler_periodic_matches_host(xavier_w):
+    h = hignn()
+    x = suspension(5)
+    lo, hi = x.min(axis=0) + 1.0, x.max(axis=0) - 1.0  # some particles start outside: wrapped
+    lims = [[float(lo[j]), float(hi[j])] for j in range(3)]
+    xh, _, _ = integrate(h, h.ExplicitEuler, x, xavier_w, device=False, lims=lims, morse=None)
+    xd, _, _ = integrate(h, h.ExplicitEuler, x, xavier_w, device=True, lims=lims, morse=None)
+    assert close(xd, xh)
+
+
+def test_device_dopri_matches_host(xavier_w):
+    h = hignn()
+    x = suspension(6)
+    # stiff bonds: the first step is rejected and shrunk, then a few accepted
+    kw = dict(dt=0.002, tf=0.004, thr=1e-2)
+    xh, ch, oh = integrate(h, h.ExplicitRk4, x, xavier_w, device=False, **kw)
+    xd, cd, od = integrate(h, h.ExplicitRk4, x, xavier_w, device=True, **kw)
+    assert ch == cd and len(oh) == len(od)
+    print("dopri steps:", ch, "max |x_dev - x_host|:", np.abs(xd - xh).max())
+    assert close(xd, xh)
