@@ -72,6 +72,197 @@ def cpu_baseline(x, w, f, leaf, eps, max_iter, mrc, exp, budget_s):
     return t, wn, wf, cs, fs
 
 
+THREE_BODY_FLOP = {"three_body": 2 * (6 * 256 + 256 * 1024 + 1024 * 512 + 512 * 9),
+                   "self": 2 * (3 * 256 + 256 * 1024 + 1024 * 512 + 512 * 9)}
+
+
+def three_body_bench(args):
+    """velocity_update_without_2body (python/gravity_field.py:34-59) on its
+    own workload: a 50^3 lattice of spacing 3 (python/gravity_field.py:85-95),
+    three-body radius 5, force (0,0,-1); seeded Xavier nets of the shapes of
+    HIGNN_nn_3body.pkl / HIGNN_nn_self.pkl.  A step is one evaluation: lists
+    (neighbours, triplets) and both nets over every edge, velocities summed."""
+    from hignn_amd import hignn, weights
+
+    hignn.Init()
+    m = args.lattice
+    g = np.arange(m, dtype=np.float64) * 3.0
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3).astype(np.float32)
+    n = x.shape[0]
+    f = np.zeros((n, 3), np.float32)
+    f[:, 2] = -1.0
+    w3, ws = weights.xavier(1, weights.net_shapes(6)), weights.xavier(2, weights.net_shapes(3))
+    out = {}
+    for prec in ("fp16", "fp32"):
+        tb = hignn.ThreeBodyModel()
+        tb.load_weights(w3, ws)
+        tb.set_three_body_epsilon(5.0)
+        tb.set_precision(prec)
+        steps = args.steps if prec == args.prec else 1
+        for _ in range(args.warmup):
+            tb.velocity(x, f)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            v = tb.velocity(x, f)
+        el = (time.perf_counter() - t0) / steps
+        prof = tb.last_profile()
+        out[prec] = (el, prof, v, steps)
+        log(f"[three-body] {prec}: {1e3 * el:.1f} ms per evaluation, {prof}")
+    el, prof, v, steps = out[args.prec]
+    e3, es = prof["three_body_edges"], prof["self_edges"]
+    fl3, fls = THREE_BODY_FLOP["three_body"] * e3, THREE_BODY_FLOP["self"] * es
+    mlp_ms = prof["three_body_ms"] + prof["self_ms"]
+    achieved = (fl3 + fls) / (mlp_ms * 1e-3) / 1e12
+    v32 = out["fp32"][2]
+    rel = float(np.linalg.norm(v - v32) / np.linalg.norm(v32))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import three_body as OT
+        k = 12  # a 12^3 sub-lattice: same neighbourhoods, ~1/70 of the edges
+        sub = x.reshape(m, m, m, 3)[:k, :k, :k].reshape(-1, 3)
+        t0 = time.perf_counter()
+        OT.velocity(sub, f[:sub.shape[0]], w3, ws, 5.0)
+        ct = time.perf_counter() - t0
+        lists, _ = OT.neighbour_lists(sub, 5.0)
+        trip, self_e = OT.three_body_edges(lists)
+        ce = trip.shape[1] + self_e.shape[1]
+        cpu = {"value": round(ce / ct, 1), "unit": "edges/s", "cores": os.cpu_count() or 1,
+               "kind": "port",
+               "sample": f"oracle (numpy float64, BLAS threads) on the 12^3 corner of the lattice: "
+                         f"{ce} edges in {ct:.1f}s"}
+    line = {
+        "metric": "three-body + self corrections: edges/s (gravity_field lattice)",
+        "value": round((e3 + es) / el, 1),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.prec,
+        "data": "synthetic",
+        "config": {
+            "workload": f"gravity_field lattice {m}^3 (N={n}), spacing 3, three-body radius 5, "
+                        "force (0,0,-1); Xavier nets (seeds 1, 2)",
+            "three_body_edges": e3, "self_edges": es,
+            "mlp_ms": round(mlp_ms, 3),
+            "lists_and_copies_ms": round(1e3 * el - mlp_ms, 3),
+            "fp32_ms_per_step": round(1e3 * out["fp32"][0], 3),
+            "rel_l2_vs_fp32_path": rel,
+        },
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
+                     "peak": PEAK[args.prec], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
+                     "kernel": "layer1 + gemm_act (x2) + layer4 over all edges"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def stepper_bench(args):
+    """Integrator steps on the reference's filament_cloud configuration
+    (python/config/filament_cloud.json: 1024 filaments x 21 beads, rest length
+    3, k_t 20, k_b 50, body force (0,0,-1), Explicit Euler dt 0.005; model
+    settings of python/simulate.py:41-51).  Each step is one full velocity
+    evaluation (forcing, update_coord with a tree rebuild, matvec) plus the
+    position update.  Timed four ways on the same model settings: the Python
+    velocity callback over host arrays (as the reference's scripts drive the
+    integrator), the device-resident integrator with per-stage HIP graphs,
+    the same without graphs, and the device-resident integrator with the tree
+    kept across steps (opt-in)."""
+    from hignn_amd import hignn, weights, workloads
+
+    hignn.Init()
+    nchain, rest, k_t, k_b, dt = 21, 3.0, 20.0, 50.0, 0.005
+    x = workloads.filament_cloud(args.filaments, nchain, rest, 240.0, seed=0)
+    n = x.shape[0]
+    w = weights.xavier(0)
+    prec = args.prec
+
+    def setup(reuse):
+        m = hignn.HignnModel(x, 100 if n > 100 else n // 2)
+        m.load_two_body_weights(w)
+        m.set_epsilon(0.01)
+        m.set_max_iter(50)
+        m.set_mat_pool_size_factor(200)
+        m.set_post_check_flag(False)
+        m.set_use_symmetry_flag(True)
+        m.set_max_far_dot_work_node_size(10000)
+        m.set_max_relative_coord(100000)
+        m.set_precision(prec)
+        m.set_tree_reuse(reuse)
+        fc = hignn.DeviceForcing()
+        fc.set_body_force([0.0, 0.0, -1.0])
+        fc.set_filaments(args.filaments, nchain, rest, k_t, k_b)
+        return m, fc
+
+    def timed(mode, steps):
+        m, fc = setup(mode == "device_reuse")
+        it = hignn.ExplicitEuler()
+        it.set_time_step(dt)
+        it.set_num_rigid_body(n)
+        if mode == "callback":
+            def vel(t, pos):
+                xx = np.ascontiguousarray(pos[:, :3])
+                m.update_coord(xx)
+                f = fc.eval(xx)
+                u = np.zeros_like(f)
+                m.dot(u, f)
+                return u
+            it.set_velocity_func(vel)
+        else:
+            it.set_device_velocity(m, fc)
+            it.set_graphs(mode != "device_eager")
+        state = np.zeros((n, 6), np.float32)
+        state[:, :3] = x
+        res = []
+        for k, s in ((0, args.warmup), (1, steps)):
+            it.set_final_time(dt * max(s, 1) * (1 - 1e-3))
+            it.initialize(state)
+            t0 = time.perf_counter()
+            it.run()
+            el = time.perf_counter() - t0
+            res.append((it.func_count(), el))
+        cnt, el = res[1]
+        log(f"[stepper] {mode}: {cnt} steps in {el:.3f}s ({1e3 * el / cnt:.2f} ms/step)")
+        return cnt, el, it.positions()
+
+    out = {}
+    for mode in ("callback", "device_graphs", "device_eager", "device_reuse"):
+        out[mode] = timed(mode, args.steps)
+    xs = {k: v[2] for k, v in out.items()}
+    diff = float(np.abs(xs["device_graphs"] - xs["callback"]).max())
+    rate = {k: v[0] / v[1] for k, v in out.items()}
+    steps, el = out["device_graphs"][0], out["device_graphs"][1]
+    line = {
+        "metric": "Explicit Euler steps/s, filament_cloud suspension (device-resident integrator)",
+        "value": round(rate["device_graphs"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic",
+        "config": {
+            "workload": f"filament_cloud: {args.filaments} straight filaments x {nchain} beads "
+                        f"(N={n}), rest 3, k_t 20, k_b 50, body force (0,0,-1), dt 0.005; leaf "
+                        "100, eps 0.01, max_iter 50, max_relative_coord 100000; Xavier(seed 0)",
+            "callback_steps_per_s": round(rate["callback"], 3),
+            "device_eager_steps_per_s": round(rate["device_eager"], 3),
+            "device_tree_reuse_steps_per_s": round(rate["device_reuse"], 3),
+            "speedup_vs_callback": round(rate["device_graphs"] / rate["callback"], 3),
+            "max_abs_position_diff_vs_callback": diff,
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,11 +282,25 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size before the timed region "
                          "(known-size reference for HBM byte counters)")
+    ap.add_argument("--stepper", action="store_true",
+                    help="time Explicit Euler steps of the filament_cloud suspension instead "
+                         "(Python velocity callback vs the device-resident integrator)")
+    ap.add_argument("--filaments", type=int, default=1024,
+                    help="filaments of 21 beads in the --stepper workload")
+    ap.add_argument("--three-body", action="store_true",
+                    help="time the three-body and self corrections on the gravity_field lattice "
+                         "instead")
+    ap.add_argument("--lattice", type=int, default=50,
+                    help="lattice points per side in the --three-body workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
               "not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.stepper:
+        return stepper_bench(args)
+    if args.three_body:
+        return three_body_bench(args)
 
     import torch
 

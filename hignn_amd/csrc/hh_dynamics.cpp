@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "../../include/hh_dynamics.h"
+#include "hh_cells.h"
 #include "hh_forcing.h"
 #include "hh_host.h"
 
@@ -28,7 +29,8 @@ struct hh_forcing {
   bool periodic = false, has_domain = false;
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   int version = 0;  // bumped by every setter (integrators re-capture their graphs)
-  DBuf src, src_tgt, src_cell, cnt, off, bcnt, bstart, bsum, sidx, spos, scell, xh, fh;
+  CellBufs cells;
+  DBuf xh, fh;
 
   // Reserves the buffers for n particles and fills the launch descriptor.
   ForcingLaunch prepare(const float *x, float *f, int64_t n) {
@@ -49,50 +51,16 @@ struct hh_forcing {
     L.morse = eps > 0 ? 1 : 0;
     L.periodic = periodic ? 1 : 0;
     L.eps = eps;
-    L.eps2 = (double)(eps * eps);
-    L.inv_h = eps > 0 ? 1.0 / ((double)eps * (1.0 + 1e-6)) : 0.0;
+    L.margin = eps;
     if (!L.morse || n <= 0) return L;
-    if (periodic)
-      for (int j = 0; j < 3; j++)
-        if (!(hi[j] - lo[j] > 2.0f * eps))
-          throw HHError(HH_ERR_ARG, "periodic domain must be wider than twice the radius");
-    const int64_t cap = periodic ? 8 * n : n;
-    if (cap >= (int64_t)1 << 30) throw HHError(HH_ERR_ARG, "too many particles for the cell list");
-    uint32_t T = 1024;
-    while ((int64_t)T < 2 * cap) T <<= 1;
-    src.reserve(16 * (size_t)cap);
-    src_tgt.reserve(4 * (size_t)cap);
-    src_cell.reserve(16 * (size_t)cap);
-    sidx.reserve(4 * (size_t)cap);
-    spos.reserve(16 * (size_t)cap);
-    scell.reserve(16 * (size_t)cap);
-    cnt.reserve(4 * (size_t)n);
-    off.reserve(4 * (size_t)(n + 1));
-    bcnt.reserve(4 * (size_t)T);
-    bstart.reserve(4 * ((size_t)T + 1));
-    bsum.reserve(4 * (size_t)scan_partials_needed(std::max<int64_t>(n, T)));
-    L.src_cap = cap;
-    L.src = src.as<float4>();
-    L.src_tgt = src_tgt.as<int32_t>();
-    L.src_cell = src_cell.as<int4>();
-    L.cnt = cnt.as<uint32_t>();
-    L.off = off.as<uint32_t>();
-    L.nsrc = off.as<uint32_t>() + n;
-    L.tsize = T;
-    L.tmask = T - 1;
-    L.bcnt = bcnt.as<uint32_t>();
-    L.bstart = bstart.as<uint32_t>();
-    L.bsum = bsum.as<uint32_t>();
-    L.sidx = sidx.as<int32_t>();
-    L.spos = spos.as<float4>();
-    L.scell = scell.as<int4>();
+    cells.setup(L, n);
     return L;
   }
 
   ~hh_forcing() {
-    for (DBuf *d : {&src, &src_tgt, &src_cell, &cnt, &off, &bcnt, &bstart, &bsum, &sidx, &spos,
-                    &scell, &xh, &fh})
-      d->release();
+    cells.release();
+    xh.release();
+    fh.release();
   }
 };
 

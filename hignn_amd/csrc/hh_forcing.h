@@ -23,6 +23,7 @@ struct ForcingLaunch {
   // Morse neighbours within eps
   int morse, periodic;
   float eps;
+  float margin;  // periodic image margin (the two-body radius)
   double eps2;   // (double)(eps * eps), the float square of the radius
   double inv_h;  // 1 / cell size; cell size = eps (1 + 1e-6)
   float lo[3], hi[3];
@@ -63,7 +64,19 @@ struct ErrLaunch {
   int64_t n;
 };
 
+// Exclusive scan of n 32-bit counts into out[0..n] (out[n] = total); bsum
+// holds scan_partials_needed(n) entries.
+hipError_t exclusive_scan(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *bsum,
+                          hipStream_t st);
 hipError_t launch_forcing(const ForcingLaunch &L, hipStream_t st);
+// Sources and hashed cell table of L (the first half of launch_forcing).
+hipError_t launch_cell_list(const ForcingLaunch &L, hipStream_t st);
+// Neighbour lists within L.eps: counts, then off = exclusive scan (n + 1
+// entries, off[n] = total); then the lists, each sorted ascending.
+hipError_t launch_neighbour_count(const ForcingLaunch &L, uint32_t *cnt, uint32_t *off,
+                                  uint32_t *bsum, hipStream_t st);
+hipError_t launch_neighbour_fill(const ForcingLaunch &L, const uint32_t *off, int32_t *idx,
+                                 hipStream_t st);
 int64_t scan_partials_needed(int64_t n);
 hipError_t launch_stage(const StageLaunch &S, hipStream_t st);
 hipError_t launch_euler_update(float *x, const float *v, float *off, int64_t n, const float *fdt,

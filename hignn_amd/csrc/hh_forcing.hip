@@ -45,15 +45,15 @@ __device__ __forceinline__ int cell_of(float x, double inv_h) {
 
 // ---------------------------------------------------------------------------
 // Periodic source images (PotentialForce.hpp:247-392).  Axis j of target i is
-// duplicated when x_j < lo_j + eps (image shifted by +L_j) or x_j > hi_j - eps
-// (shifted by -L_j).  Image n of a target with `num` images carries the shift
+// duplicated when x_j < lo_j + margin (image shifted by +L_j) or
+// x_j > hi_j - margin (shifted by -L_j); the margin is the two-body radius.  Image n of a target with `num` images carries the shift
 // of its k-th duplicated axis when bit (num >> (k+1)) of n is set: the
 // reference fills halves of blocks of size num, num/2, ... in axis order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int image_axes(const float *x, const ForcingLaunch &L, int dup[3]) {
   int num = 1;
   for (int j = 0; j < 3; j++) {
-    const float clo = L.lo[j] + L.eps, chi = L.hi[j] - L.eps;
+    const float clo = L.lo[j] + L.margin, chi = L.hi[j] - L.margin;
     dup[j] = x[j] < clo ? -2 : (x[j] > chi ? 2 : 1);
     if (dup[j] != 1) num *= 2;
   }
@@ -203,6 +203,8 @@ __global__ void scan_add_kernel(uint32_t *out, int64_t n, const uint32_t *bsum) 
   if (i < n) out[i] += bsum[i / kScanBlock];
 }
 
+}  // namespace
+
 hipError_t exclusive_scan(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *bsum,
                           hipStream_t st) {
   const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
@@ -212,6 +214,68 @@ hipError_t exclusive_scan(const uint32_t *in, int64_t n, uint32_t *out, uint32_t
   hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, st, out, n, bsum);
   return hipGetLastError();
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Neighbour lists (NeighborLists::BuildTargetNeighborLists,
+// include/NeighborLists.hpp:223-251): for every target, the targets of the
+// sources within the radius, the target's own entry dropped once; each list
+// sorted ascending.  Two passes: counts, then (after a scan) the lists.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__device__ __forceinline__ uint32_t visit_neighbours(const ForcingLaunch &L, int64_t i,
+                                                     int32_t *out) {
+  const float xi[3] = {L.x[3 * i], L.x[3 * i + 1], L.x[3 * i + 2]};
+  const int cx = cell_of(xi[0], L.inv_h), cy = cell_of(xi[1], L.inv_h), cz = cell_of(xi[2], L.inv_h);
+  bool self_seen = false;
+  uint32_t c = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+        const uint32_t b = cell_hash(ux, uy, uz, L.tmask);
+        const uint32_t e = L.bstart[b + 1];
+        for (uint32_t k = L.bstart[b]; k < e; k++) {
+          const int4 q = L.scell[k];
+          if (q.x != ux || q.y != uy || q.z != uz) continue;
+          const float4 s = L.spos[k];
+          const double ddx = (double)s.x - (double)xi[0], ddy = (double)s.y - (double)xi[1],
+                       ddz = (double)s.z - (double)xi[2];
+          const double d = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+          if (!(d * d < L.eps2)) continue;
+          if (q.w == (int32_t)i && !self_seen) {
+            self_seen = true;
+            continue;
+          }
+          if (FILL) out[c] = q.w;
+          c++;
+        }
+      }
+  return c;
+}
+
+__global__ void neighbour_count_kernel(ForcingLaunch L, uint32_t *cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  cnt[i] = visit_neighbours<false>(L, i, nullptr);
+}
+
+__global__ void neighbour_fill_kernel(ForcingLaunch L, const uint32_t *off, int32_t *idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  int32_t *o = idx + off[i];
+  const uint32_t c = visit_neighbours<true>(L, i, o);
+  for (uint32_t k = 1; k < c; k++) {  // insertion sort, ascending
+    const int32_t v = o[k];
+    uint32_t q = k;
+    while (q > 0 && o[q - 1] > v) {
+      o[q] = o[q - 1];
+      q--;
+    }
+    o[q] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -441,28 +505,51 @@ __global__ void sum_partials_kernel(const double *part, int64_t np, double *out)
 
 }  // namespace
 
+hipError_t launch_cell_list(const ForcingLaunch &L, hipStream_t st) {
+  if (L.n <= 0) return hipSuccess;
+  if (L.periodic) {
+    hipLaunchKernelGGL(image_count_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
+    hipError_t e = exclusive_scan(L.cnt, L.n, L.off, L.bsum, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(image_fill_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
+  } else {
+    hipLaunchKernelGGL(plain_sources_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
+  }
+  hipLaunchKernelGGL(fill_u32_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L.bcnt,
+                     (int64_t)L.tsize, 0u);
+  hipLaunchKernelGGL(cell_count_kernel, grid_for(L.src_cap), dim3(kThreads), 0, st, L);
+  hipError_t e = exclusive_scan(L.bcnt, (int64_t)L.tsize, L.bstart, L.bsum, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fill_u32_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L.bcnt,
+                     (int64_t)L.tsize, 0u);
+  hipLaunchKernelGGL(cell_scatter_kernel, grid_for(L.src_cap), dim3(kThreads), 0, st, L);
+  hipLaunchKernelGGL(bucket_sort_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L);
+  return hipGetLastError();
+}
+
 hipError_t launch_forcing(const ForcingLaunch &L, hipStream_t st) {
   if (L.n <= 0) return hipSuccess;
   if (L.morse) {
-    if (L.periodic) {
-      hipLaunchKernelGGL(image_count_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
-      hipError_t e = exclusive_scan(L.cnt, L.n, L.off, L.bsum, st);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(image_fill_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
-    } else {
-      hipLaunchKernelGGL(plain_sources_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
-    }
-    hipLaunchKernelGGL(fill_u32_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L.bcnt,
-                       (int64_t)L.tsize, 0u);
-    hipLaunchKernelGGL(cell_count_kernel, grid_for(L.src_cap), dim3(kThreads), 0, st, L);
-    hipError_t e = exclusive_scan(L.bcnt, (int64_t)L.tsize, L.bstart, L.bsum, st);
+    hipError_t e = launch_cell_list(L, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fill_u32_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L.bcnt,
-                       (int64_t)L.tsize, 0u);
-    hipLaunchKernelGGL(cell_scatter_kernel, grid_for(L.src_cap), dim3(kThreads), 0, st, L);
-    hipLaunchKernelGGL(bucket_sort_kernel, grid_for(L.tsize), dim3(kThreads), 0, st, L);
   }
   hipLaunchKernelGGL(force_kernel, grid_for(L.n), dim3(kThreads), 0, st, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbour_count(const ForcingLaunch &L, uint32_t *cnt, uint32_t *off,
+                                  uint32_t *bsum, hipStream_t st) {
+  if (L.n <= 0) return hipSuccess;
+  hipError_t e = launch_cell_list(L, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(neighbour_count_kernel, grid_for(L.n), dim3(kThreads), 0, st, L, cnt);
+  return exclusive_scan(cnt, L.n, off, bsum, st);
+}
+
+hipError_t launch_neighbour_fill(const ForcingLaunch &L, const uint32_t *off, int32_t *idx,
+                                 hipStream_t st) {
+  if (L.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(neighbour_fill_kernel, grid_for(L.n), dim3(kThreads), 0, st, L, off, idx);
   return hipGetLastError();
 }
 

@@ -1,0 +1,385 @@
+// Three-body and self corrections of the H-HIGNN velocity on the device.
+//
+// The reference evaluates them with torch over edge lists
+// (python/HIGNN/model_structure.py:74-140, called from
+// python/gravity_field.py:34-59 with the lists of
+// include/NeighborLists.hpp:512-588):
+//   three-body edge (i, n, p), n a neighbour of i, p a neighbour of n, p != i:
+//     v_p += net3(x_n - x_i, x_p - x_n) / |x_n - x_i| / |x_p - x_n| . f_i
+//   self edge (i, n), n a neighbour of i:
+//     v_n += net_self(x_i - x_n) / |x_i - x_n|^2 . f_n
+// with net = Linear(d,256) Tanh Linear(256,1024) Tanh Linear(1024,512) Tanh
+// Linear(512,9), the 9 outputs read as a row-major 3x3 block.
+//
+// Edges are processed in batches: layer 1 on the VALU (d = 6 or 3 inputs),
+// layers 2 and 3 as tiled MFMA GEMMs with bias and tanh fused into the
+// epilogue (activations stay in HBM between layers: 1.8 KB per edge in 16 bit
+// against 1.6 MFLOP), layer 4 and the 3x3 product on the VALU with FP64
+// atomic accumulation of the velocities.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hh_three_body.h"
+
+namespace hh {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+
+// tanh from weights pre-scaled by 2 log2(e): tanh(x) = 1 - 2 / (2^(2 x log2 e) + 1)
+__device__ __forceinline__ float act(float xs) {
+  return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(xs) + 1.0f), 1.0f);
+}
+
+template <class T>
+__device__ __forceinline__ T to_t(float v);
+template <>
+__device__ __forceinline__ float to_t<float>(float v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ _Float16 to_t<_Float16>(float v) {
+  return (_Float16)v;
+}
+
+// ---------------------------------------------------------------------------
+// Triplet lists (NeighborLists::BuildThreeBodyInfo, NeighborLists.hpp:512-588):
+// per target i, for each neighbour n (ascending), each neighbour p of n
+// (ascending) other than i.
+// ---------------------------------------------------------------------------
+__global__ void triplet_count_kernel(const uint32_t *off, const int32_t *idx, int64_t n,
+                                     uint32_t *cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = 0;
+  for (uint32_t a = off[i]; a < off[i + 1]; a++) {
+    const int32_t nb = idx[a];
+    for (uint32_t b = off[nb]; b < off[nb + 1]; b++) c += idx[b] != (int32_t)i;
+  }
+  cnt[i] = c;
+}
+
+__global__ void triplet_fill_kernel(const uint32_t *off, const int32_t *idx, int64_t n,
+                                    const uint32_t *toff, int32_t *trip) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t o = toff[i];
+  for (uint32_t a = off[i]; a < off[i + 1]; a++) {
+    const int32_t nb = idx[a];
+    for (uint32_t b = off[nb]; b < off[nb + 1]; b++) {
+      const int32_t p = idx[b];
+      if (p == (int32_t)i) continue;
+      trip[3 * o] = (int32_t)i;
+      trip[3 * o + 1] = nb;
+      trip[3 * o + 2] = p;
+      o++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layer 1 (VALU) and the per-edge record of the epilogue.
+// Block: 256 threads = the 256 outputs; 32 edges per block.
+// ---------------------------------------------------------------------------
+constexpr int kL1Edges = 32;
+
+template <int DIN, class T>
+__global__ __launch_bounds__(kThreads) void layer1_kernel(EdgeBatch B, const float *w1s,
+                                                          const float *b1s, T *h1) {
+  __shared__ float xin[kL1Edges][DIN];
+  const int t = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * kL1Edges;
+  if (t < kL1Edges) {
+    const int64_t e = e0 + t;
+    float r1 = 1.0f, r2 = 1.0f, f[3] = {0, 0, 0};
+    int32_t tgt = -1;
+    float d[6] = {0, 0, 0, 0, 0, 0};
+    if (e < B.count) {
+      const int64_t g = B.first + e;
+      if (DIN == 6) {
+        const int32_t i = B.edges[3 * g], nb = B.edges[3 * g + 1], p = B.edges[3 * g + 2];
+        for (int j = 0; j < 3; j++) {
+          d[j] = B.x[3 * nb + j] - B.x[3 * i + j];
+          d[3 + j] = B.x[3 * p + j] - B.x[3 * nb + j];
+          f[j] = B.f[3 * i + j];
+        }
+        r1 = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        r2 = sqrtf(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+        tgt = p;
+      } else {
+        const int32_t i = B.edges[2 * g], nb = B.edges[2 * g + 1];
+        for (int j = 0; j < 3; j++) {
+          d[j] = B.x[3 * i + j] - B.x[3 * nb + j];
+          f[j] = B.f[3 * nb + j];
+        }
+        r1 = r2 = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        tgt = nb;
+      }
+      B.rec[e] = make_float4(r1, r2, 0.0f, 0.0f);
+      B.frc[e] = make_float4(f[0], f[1], f[2], __int_as_float(tgt));
+    }
+    for (int j = 0; j < DIN; j++) xin[t][j] = d[j];
+  }
+  __syncthreads();
+  float w[DIN];
+  for (int j = 0; j < DIN; j++) w[j] = w1s[t * DIN + j];
+  const float b = b1s[t];
+  for (int k = 0; k < kL1Edges; k++) {
+    float a = b;
+    for (int j = 0; j < DIN; j++) a = __builtin_fmaf(w[j], xin[k][j], a);
+    h1[(e0 + k) * 256 + t] = to_t<T>(act(a));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C[e][o] = tanh(sum_k A[e][k] W[o][k] + b[o]) for a batch of edges:
+// 128 x 128 tiles, 256 threads in 2 x 2 waves of 64 x 64.  K is staged
+// through two LDS buffers: while the waves multiply one K step, the next is
+// in flight from global memory into registers and then stored to the other
+// buffer, one barrier per step.  MFMA 16x16x32 f16 (FP16 path, K steps of 64)
+// or 16x16x4 f32 (FP32 path, K steps of 32); lane l holds
+// D[4 (l/16) + r][l % 16] of a 16 x 16 tile.
+// Requires E a multiple of 128 (padded batch), O a multiple of 128 and K a
+// multiple of the K step.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 128;
+
+template <class T>
+struct Gemm;
+
+template <>
+struct Gemm<_Float16> {
+  static constexpr int kKS = 64;   // K per stage
+  static constexpr int kPad = 8;   // row stride 72 halves = 144 B
+  static constexpr int kSub = 2;   // MFMAs (K = 32) per stage
+  typedef f16x8 Frag;
+  __device__ static Frag frag(const _Float16 *row, int sub, int lane) {
+    return *(const f16x8 *)(row + 32 * sub + 8 * (lane >> 4));
+  }
+  __device__ static f32x4 mfma(Frag a, Frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Gemm<float> {
+  static constexpr int kKS = 32;
+  static constexpr int kPad = 4;   // row stride 36 floats
+  static constexpr int kSub = 8;   // MFMAs (K = 4) per stage
+  typedef float Frag;
+  __device__ static Frag frag(const float *row, int sub, int lane) { return row[4 * sub + (lane >> 4)]; }
+  __device__ static f32x4 mfma(Frag a, Frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <class T>
+__global__ __launch_bounds__(kThreads, 2) void gemm_act_kernel(const T *A, const T *W,
+                                                               const float *bias, T *C, int K,
+                                                               int O) {
+  typedef Gemm<T> G;
+  constexpr int KS = G::kKS;
+  constexpr int RS = KS + G::kPad;                  // LDS row stride (elements)
+  constexpr int CPR = KS * (int)sizeof(T) / 16;     // 16-byte chunks per row and stage
+  constexpr int PER = kTile * CPR / kThreads;       // chunks per thread per operand
+  constexpr int EPC = 16 / (int)sizeof(T);          // elements per chunk
+  constexpr int STAGE = kTile * RS;                 // elements of one operand stage
+  __shared__ __attribute__((aligned(16))) T lds[2 * 2 * STAGE];  // [stage][operand]
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t e0 = (int64_t)blockIdx.y * kTile;
+  const int o0 = blockIdx.x * kTile;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const T *ga = A + e0 * K, *gw = W + (int64_t)o0 * K;
+
+  i32x4 ra[PER], rw[PER];
+  int goff[PER], soff[PER];
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int c = t + kThreads * j, row = c / CPR, q = c % CPR;
+    goff[j] = row * K + q * EPC;
+    soff[j] = row * RS + q * EPC;
+  }
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    ra[j] = *(const i32x4 *)(ga + goff[j]);
+    rw[j] = *(const i32x4 *)(gw + goff[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    *(i32x4 *)(lds + soff[j]) = ra[j];
+    *(i32x4 *)(lds + STAGE + soff[j]) = rw[j];
+  }
+  __syncthreads();
+
+  const int nsteps = K / KS;
+  for (int step = 0; step < nsteps; step++) {
+    const int cur = step & 1;
+    // next stage (the last step reloads itself into the idle buffer)
+    const int k1 = min(step + 1, nsteps - 1) * KS;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      ra[j] = *(const i32x4 *)(ga + goff[j] + k1);
+      rw[j] = *(const i32x4 *)(gw + goff[j] + k1);
+    }
+    const T *sa = lds + cur * 2 * STAGE, *sw = sa + STAGE;
+#pragma unroll
+    for (int s = 0; s < G::kSub; s++) {
+      typename G::Frag fa[4], fb[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) fa[m] = G::frag(sa + (wr + 16 * m + (lane & 15)) * RS, s, lane);
+#pragma unroll
+      for (int m = 0; m < 4; m++) fb[m] = G::frag(sw + (wc + 16 * m + (lane & 15)) * RS, s, lane);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = G::mfma(fa[a], fb[b], acc[a][b]);
+    }
+    T *da = lds + (cur ^ 1) * 2 * STAGE;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      *(i32x4 *)(da + soff[j]) = ra[j];
+      *(i32x4 *)(da + STAGE + soff[j]) = rw[j];
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int o = o0 + wc + 16 * b + (lane & 15);
+    const float bo = bias[o];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t e = e0 + wr + 16 * a + 4 * (lane >> 4) + r;
+        C[e * O + o] = to_t<T>(act(acc[a][b][r] + bo));
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layer 4 (512 -> 9), the 3x3 block over the radii and its product with the
+// force; one wave per edge, 8 inputs per lane, wave sums, FP64 atomics.
+// ---------------------------------------------------------------------------
+constexpr int kL4Edges = 64;  // edges per block (16 per wave)
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void layer4_kernel(EdgeBatch B, const T *h3,
+                                                          const float *w4, const float *b4,
+                                                          double *u) {
+  __shared__ float sw[9 * 512];
+  for (int k = threadIdx.x; k < 9 * 512; k += kThreads) sw[k] = w4[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < kL4Edges / 4; q++) {
+    const int64_t e = (int64_t)blockIdx.x * kL4Edges + wave * (kL4Edges / 4) + q;
+    if (e >= B.count) break;  // e is uniform across the wave
+    float hv[8];
+    for (int j = 0; j < 8; j++) hv[j] = (float)h3[e * 512 + 8 * lane + j];
+    float y[9];
+    for (int c = 0; c < 9; c++) {
+      float s = 0.0f;
+      for (int j = 0; j < 8; j++) s = __builtin_fmaf(sw[c * 512 + 8 * lane + j], hv[j], s);
+      for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+      y[c] = s;
+    }
+    if (lane < 3) {
+      const float4 rr = B.rec[e];
+      const float4 fr = B.frc[e];
+      const int32_t tgt = __float_as_int(fr.w);
+      float v = 0.0f;
+      const float f[3] = {fr.x, fr.y, fr.z};
+      for (int c = 0; c < 3; c++) {
+        const float m = ((y[3 * lane + c] + b4[3 * lane + c]) / rr.x) / rr.y;
+        v = __builtin_fmaf(m, f[c], v);
+      }
+      atomicAdd(&u[3 * (int64_t)tgt + lane], (double)v);
+    }
+  }
+}
+
+__global__ void to_float_kernel(const double *u, float *out, int64_t n3, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  out[i] = accumulate ? (float)((double)out[i] + u[i]) : (float)u[i];
+}
+
+inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+}  // namespace
+
+hipError_t launch_triplet_count(const uint32_t *off, const int32_t *idx, int64_t n, uint32_t *cnt,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(triplet_count_kernel, grid_for(n), dim3(kThreads), 0, st, off, idx, n, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_triplet_fill(const uint32_t *off, const int32_t *idx, int64_t n,
+                               const uint32_t *toff, int32_t *trip, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(triplet_fill_kernel, grid_for(n), dim3(kThreads), 0, st, off, idx, n, toff,
+                     trip);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int half, void *h1,
+                           void *h2, void *h3, double *u, hipStream_t st) {
+  if (B.count <= 0) return hipSuccess;
+  const int64_t epad = (B.count + kTile - 1) / kTile * kTile;
+  const dim3 g1((unsigned)((B.count + kL1Edges - 1) / kL1Edges));
+  const dim3 g2(1024 / kTile, (unsigned)(epad / kTile)), g3(512 / kTile, (unsigned)(epad / kTile));
+  const dim3 g4((unsigned)((B.count + kL4Edges - 1) / kL4Edges));
+  if (half) {
+    typedef _Float16 H;
+    if (din == 6)
+      hipLaunchKernelGGL((layer1_kernel<6, H>), g1, dim3(kThreads), 0, st, B, net.w1s, net.b1s,
+                         (H *)h1);
+    else
+      hipLaunchKernelGGL((layer1_kernel<3, H>), g1, dim3(kThreads), 0, st, B, net.w1s, net.b1s,
+                         (H *)h1);
+    hipLaunchKernelGGL(gemm_act_kernel<H>, g2, dim3(kThreads), 0, st, (const H *)h1,
+                       (const H *)net.w2h, net.b2s, (H *)h2, 256, 1024);
+    hipLaunchKernelGGL(gemm_act_kernel<H>, g3, dim3(kThreads), 0, st, (const H *)h2,
+                       (const H *)net.w3h, net.b3s, (H *)h3, 1024, 512);
+    hipLaunchKernelGGL(layer4_kernel<H>, g4, dim3(kThreads), 0, st, B, (const H *)h3, net.w4,
+                       net.b4, u);
+  } else {
+    if (din == 6)
+      hipLaunchKernelGGL((layer1_kernel<6, float>), g1, dim3(kThreads), 0, st, B, net.w1s,
+                         net.b1s, (float *)h1);
+    else
+      hipLaunchKernelGGL((layer1_kernel<3, float>), g1, dim3(kThreads), 0, st, B, net.w1s,
+                         net.b1s, (float *)h1);
+    hipLaunchKernelGGL(gemm_act_kernel<float>, g2, dim3(kThreads), 0, st, (const float *)h1,
+                       net.w2s, net.b2s, (float *)h2, 256, 1024);
+    hipLaunchKernelGGL(gemm_act_kernel<float>, g3, dim3(kThreads), 0, st, (const float *)h2,
+                       net.w3s, net.b3s, (float *)h3, 1024, 512);
+    hipLaunchKernelGGL(layer4_kernel<float>, g4, dim3(kThreads), 0, st, B, (const float *)h3,
+                       net.w4, net.b4, u);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double *u, float *out, int64_t n3, int accumulate,
+                           hipStream_t st) {
+  if (n3 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(to_float_kernel, grid_for(n3), dim3(kThreads), 0, st, u, out, n3, accumulate);
+  return hipGetLastError();
+}
+
+}  // namespace hh

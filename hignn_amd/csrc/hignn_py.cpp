@@ -20,6 +20,7 @@
 
 #include "../../include/hh.h"
 #include "../../include/hh_dynamics.h"
+#include "../../include/hh_three_body.h"
 
 namespace py = pybind11;
 
@@ -231,6 +232,160 @@ class Forcing {
 };
 
 class PotentialForceCls : public Forcing {};
+
+// ---------------------------------------------------------------------------
+// NeighborLists (interface/HignnInterface.cpp:73-91, include/NeighborLists.hpp)
+// on the device lists of include/hh_three_body.h.  Edge arrays are uint64
+// (the reference's size_t), (2,E) or (3,E).  Inside one target the edges come
+// ascending by neighbour; the reference's order is its kd-tree's.
+// ---------------------------------------------------------------------------
+class Neighbors {
+ public:
+  Neighbors() { check(hh_neighbors_create(&nb_)); }
+  ~Neighbors() { hh_neighbors_destroy(nb_); }
+  void set_periodic(bool flag) { check(hh_neighbors_set_periodic(nb_, flag)); }
+  void set_domain(py::array_t<float, py::array::forcecast> domain) {
+    if (domain.ndim() != 2) throw std::runtime_error("Rank of domain must be two");
+    if (domain.shape(0) < 2 || domain.shape(1) < 3)
+      throw std::invalid_argument("domain must have shape (2, 3)");
+    auto d = domain.unchecked<2>();
+    const float lo[3] = {d(0, 0), d(0, 1), d(0, 2)}, hi[3] = {d(1, 0), d(1, 1), d(1, 2)};
+    check(hh_neighbors_set_domain(nb_, lo, hi));
+  }
+  void update_coord(py::array_t<float, py::array::forcecast> coord) {
+    if (coord.ndim() != 2) throw std::runtime_error("Rank of target sites must be two");
+    auto x = rows3(coord, "coord");
+    n_ = coord.shape(0);
+    check(hh_neighbors_update_coord(nb_, x.data(), n_));
+  }
+  void set_two_body_epsilon(float e) { check(hh_neighbors_set_two_body_epsilon(nb_, e)); }
+  void set_three_body_epsilon(float e) { check(hh_neighbors_set_three_body_epsilon(nb_, e)); }
+  void build_two_body_info() { check(hh_neighbors_build_two_body(nb_, &e2_)); }
+  void build_three_body_info() { check(hh_neighbors_build_three_body(nb_, &e3_, &es_)); }
+
+  py::array_t<uint64_t> two_body() { return fetch(2, e2_, hh_neighbors_two_body_edges); }
+  py::array_t<uint64_t> three_body() { return fetch(3, e3_, hh_neighbors_three_body_edges); }
+  py::array_t<uint64_t> three_body_self() {
+    return fetch(2, es_, hh_neighbors_three_body_self_edges);
+  }
+
+  // get_two_body_edge_info_by_index (NeighborLists.hpp:309-340): every pair
+  // (j, index), j != index, over all particles.
+  py::array_t<uint64_t> two_body_by_index(py::array_t<uint64_t, py::array::forcecast> idx) {
+    auto ix = idx.unchecked<1>();
+    const int64_t nt = idx.shape(0), per = n_ > 0 ? n_ - 1 : 0, ne = nt * per;
+    py::array_t<uint64_t> out({(py::ssize_t)2, (py::ssize_t)ne});
+    uint64_t *o = out.mutable_data();
+    for (int64_t i = 0; i < nt; i++) {
+      int64_t c = i * per;
+      for (int64_t j = 0; j < n_; j++)
+        if ((uint64_t)j != ix(i)) {
+          o[c] = (uint64_t)j;
+          o[ne + c] = ix(i);
+          c++;
+        }
+    }
+    return out;
+  }
+
+  // get_three_body_edge_info_by_index (:362-403): the edges of each given
+  // target, rows (p, n, i); get_three_body_edge_self_info_by_index
+  // (:424-464): rows (n, i).
+  py::array_t<uint64_t> three_body_by_index(py::array_t<uint64_t, py::array::forcecast> idx) {
+    return by_index(idx, three_body(), 3);
+  }
+  py::array_t<uint64_t> three_body_self_by_index(py::array_t<uint64_t, py::array::forcecast> idx) {
+    return by_index(idx, three_body_self(), 2);
+  }
+
+ private:
+  py::array_t<uint64_t> fetch(int rows, int64_t ne, int (*fn)(hh_neighbors *, int64_t *)) {
+    std::vector<int64_t> buf((size_t)rows * (size_t)ne);
+    if (ne > 0) check(fn(nb_, buf.data()));
+    py::array_t<uint64_t> out({(py::ssize_t)rows, (py::ssize_t)ne});
+    uint64_t *o = out.mutable_data();
+    for (size_t k = 0; k < buf.size(); k++) o[k] = (uint64_t)buf[k];
+    return out;
+  }
+  // edges of a target are contiguous (row 0 ascending); rows are reversed
+  py::array_t<uint64_t> by_index(py::array_t<uint64_t, py::array::forcecast> idx,
+                                 py::array_t<uint64_t> all, int rows) {
+    auto ix = idx.unchecked<1>();
+    const int64_t ne = all.shape(1);
+    const uint64_t *a = all.data();
+    std::vector<std::pair<int64_t, int64_t>> span;
+    int64_t tot = 0;
+    for (py::ssize_t i = 0; i < idx.shape(0); i++) {
+      const int64_t lo = std::lower_bound(a, a + ne, ix(i)) - a;
+      const int64_t hi = std::upper_bound(a, a + ne, ix(i)) - a;
+      span.emplace_back(lo, hi);
+      tot += hi - lo;
+    }
+    py::array_t<uint64_t> out({(py::ssize_t)rows, (py::ssize_t)tot});
+    uint64_t *o = out.mutable_data();
+    int64_t c = 0;
+    for (auto &sp : span)
+      for (int64_t e = sp.first; e < sp.second; e++, c++)
+        for (int r = 0; r < rows; r++) o[r * tot + c] = a[(rows - 1 - r) * ne + e];
+    return out;
+  }
+
+  hh_neighbors *nb_ = nullptr;
+  int64_t n_ = 0, e2_ = 0, e3_ = 0, es_ = 0;
+};
+
+// Three-body and self corrections (velocity_update_without_2body,
+// python/gravity_field.py:34-59) on the device.
+class ThreeBody {
+ public:
+  ThreeBody() { check(hh_three_body_create(&tb_)); }
+  ~ThreeBody() { hh_three_body_destroy(tb_); }
+  hh_three_body *handle() const { return tb_; }
+  // paths of the torch-saved nets (HIGNN_nn_3body.pkl, HIGNN_nn_self.pkl),
+  // read without unpickling, .npy vectors or "xavier:<seed>"
+  void load_model(const std::string &three_body, const std::string &self_net) {
+    py::object W = py::module_::import("hignn_amd.weights");
+    py::array_t<float, py::array::forcecast | py::array::c_style> a =
+        W.attr("load_net")(three_body, 6);
+    py::array_t<float, py::array::forcecast | py::array::c_style> b =
+        W.attr("load_net")(self_net, 3);
+    check(hh_three_body_load(tb_, a.data(), a.size(), b.data(), b.size()));
+  }
+  void load_weights(py::array_t<float, py::array::forcecast | py::array::c_style> w3,
+                    py::array_t<float, py::array::forcecast | py::array::c_style> ws) {
+    check(hh_three_body_load(tb_, w3.data(), w3.size(), ws.data(), ws.size()));
+  }
+  void set_three_body_epsilon(float e) { check(hh_three_body_set_epsilon(tb_, e)); }
+  void set_precision(const std::string &p) {
+    int v = p == "fp32" ? HH_PREC_FP32 : (p == "fp16" ? HH_PREC_FP16 : -1);
+    if (v < 0) throw std::invalid_argument("precision must be 'fp32' or 'fp16'");
+    check(hh_three_body_set_precision(tb_, v));
+  }
+  py::array_t<float> velocity(py::array_t<float, py::array::forcecast> coord,
+                              py::array_t<float, py::array::forcecast> force) {
+    auto x = rows3(coord, "coord");
+    auto f = rows3(force, "force");
+    const py::ssize_t n = coord.shape(0);
+    if (force.shape(0) != n) throw std::invalid_argument("coord and force sizes differ");
+    py::array_t<float> out({n, (py::ssize_t)3});
+    check(hh_three_body_velocity(tb_, x.data(), f.data(), (int64_t)n, out.mutable_data()));
+    return out;
+  }
+  py::dict last_profile() {
+    int64_t e[2];
+    double ms[2];
+    check(hh_three_body_last_profile(tb_, e, ms));
+    py::dict d;
+    d["three_body_edges"] = e[0];
+    d["self_edges"] = e[1];
+    d["three_body_ms"] = ms[0];
+    d["self_ms"] = ms[1];
+    return d;
+  }
+
+ private:
+  hh_three_body *tb_ = nullptr;
+};
 
 // ---------------------------------------------------------------------------
 // Time integrators (include/TimeIntegrator.hpp).  Positions are float
@@ -605,6 +760,31 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_periodic", &Forcing::set_periodic, py::arg("is_periodic") = false)
       .def("set_domain", &Forcing::set_domain)
       .def("eval", &Forcing::eval);
+
+  py::class_<Neighbors>(m, "NeighborLists")
+      .def(py::init())
+      .def("set_periodic", &Neighbors::set_periodic, py::arg("is_periodic") = false)
+      .def("set_domain", &Neighbors::set_domain)
+      .def("update_coord", &Neighbors::update_coord)
+      .def("set_two_body_epsilon", &Neighbors::set_two_body_epsilon)
+      .def("set_three_body_epsilon", &Neighbors::set_three_body_epsilon)
+      .def("get_two_body_edge_info", &Neighbors::two_body)
+      .def("get_two_body_edge_info_by_index", &Neighbors::two_body_by_index)
+      .def("get_three_body_edge_info", &Neighbors::three_body)
+      .def("get_three_body_edge_info_by_index", &Neighbors::three_body_by_index)
+      .def("get_three_body_edge_self_info", &Neighbors::three_body_self)
+      .def("get_three_body_edge_self_info_by_index", &Neighbors::three_body_self_by_index)
+      .def("build_two_body_info", &Neighbors::build_two_body_info)
+      .def("build_three_body_info", &Neighbors::build_three_body_info);
+
+  py::class_<ThreeBody>(m, "ThreeBodyModel")
+      .def(py::init())
+      .def("load_model", &ThreeBody::load_model)
+      .def("load_weights", &ThreeBody::load_weights)
+      .def("set_three_body_epsilon", &ThreeBody::set_three_body_epsilon)
+      .def("set_precision", &ThreeBody::set_precision)
+      .def("velocity", &ThreeBody::velocity)
+      .def("last_profile", &ThreeBody::last_profile);
 
   py::class_<PotentialForceCls, Forcing>(m, "PotentialForce")
       .def(py::init())

@@ -30,10 +30,19 @@ SHAPES = [("W1", (H1, 3)), ("b1", (H1,)), ("W2", (H2, H1)), ("b2", (H2,)),
 TOTAL = sum(int(np.prod(s)) for _, s in SHAPES)  # 71,177
 
 
-def pack(parts: dict) -> np.ndarray:
-    """Concatenate W1,b1,W2,b2,W3,b3 into the flat float32 layout."""
+def net_shapes(din: int):
+    """Layer shapes of Three_body_net (din 6) and Two_body_self_net (din 3):
+    Linear(din,256) Tanh Linear(256,1024) Tanh Linear(1024,512) Tanh
+    Linear(512,9) (python/HIGNN/model_structure.py:41-68; the nets of
+    python/Saved_Model/<run>/HIGNN_nn_3body.pkl and HIGNN_nn_self.pkl)."""
+    return [("W1", (256, din)), ("b1", (256,)), ("W2", (1024, 256)), ("b2", (1024,)),
+            ("W3", (512, 1024)), ("b3", (512,)), ("W4", (9, 512)), ("b4", (9,))]
+
+
+def pack(parts: dict, shapes=None) -> np.ndarray:
+    """Concatenate the weights (W1,b1,W2,b2,... in order) into the flat float32 layout."""
     out = []
-    for name, shape in SHAPES:
+    for name, shape in shapes or SHAPES:
         a = np.asarray(parts[name], dtype=np.float32)
         if a.shape != shape:
             raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
@@ -41,19 +50,21 @@ def pack(parts: dict) -> np.ndarray:
     return np.ascontiguousarray(np.concatenate(out))
 
 
-def unpack(flat: np.ndarray) -> dict:
+def unpack(flat: np.ndarray, shapes=None) -> dict:
+    shapes = shapes or SHAPES
+    total = sum(int(np.prod(s)) for _, s in shapes)
     flat = np.asarray(flat, dtype=np.float32).reshape(-1)
-    if flat.size != TOTAL:
-        raise ValueError(f"expected {TOTAL} floats, got {flat.size}")
+    if flat.size != total:
+        raise ValueError(f"expected {total} floats, got {flat.size}")
     parts, off = {}, 0
-    for name, shape in SHAPES:
+    for name, shape in shapes:
         n = int(np.prod(shape))
         parts[name] = flat[off:off + n].reshape(shape)
         off += n
     return parts
 
 
-def xavier(seed: int = 0) -> np.ndarray:
+def xavier(seed: int = 0, shapes=None) -> np.ndarray:
     """Seeded initialisation of the same layer shapes.
 
     Weights: Xavier/Glorot uniform, U(-a, a) with a = sqrt(6/(fan_in+fan_out)).
@@ -61,14 +72,15 @@ def xavier(seed: int = 0) -> np.ndarray:
     torch Linear layer).  Generator: numpy PCG64 seeded with ``seed``; the
     draws go W1, b1, W2, b2, W3, b3.
     """
+    shapes = shapes or SHAPES
     rng = np.random.Generator(np.random.PCG64(seed))
     parts = {}
-    for (wn, (fo, fi)), (bn, _) in zip(SHAPES[0::2], SHAPES[1::2]):
+    for (wn, (fo, fi)), (bn, _) in zip(shapes[0::2], shapes[1::2]):
         a = np.sqrt(6.0 / (fi + fo))
         parts[wn] = rng.uniform(-a, a, size=(fo, fi)).astype(np.float32)
         bb = 1.0 / np.sqrt(fi)
         parts[bn] = rng.uniform(-bb, bb, size=(fo,)).astype(np.float32)
-    return pack(parts)
+    return pack(parts, shapes)
 
 
 class _Ref:
@@ -221,10 +233,12 @@ def _tensors(node, path="", out=None, seen=None):
     return out
 
 
-def load_torch_zip(path: str) -> np.ndarray:
-    """Raw weights from a torch-saved Sequential (.pkl) or a TorchScript Net
-    (.pt) archive, without unpickling: the six tensors of shapes W1, b1, W2,
-    b2, W3, b3, taken in the stream's order."""
+def load_torch_zip(path: str, shapes=None) -> np.ndarray:
+    """Raw weights from a torch-saved module (.pkl) or a TorchScript Net
+    (.pt) archive, without unpickling: the tensors of the given layer shapes
+    (default: the pair function's W1, b1, W2, b2, W3, b3), taken in the
+    stream's order."""
+    shapes = shapes or SHAPES
     with zipfile.ZipFile(path) as z:
         names = z.namelist()
         pk = [n for n in names if n.endswith("/data.pkl")]
@@ -232,7 +246,7 @@ def load_torch_zip(path: str) -> np.ndarray:
             raise ValueError(f"{path}: expected one data.pkl, found {pk}")
         root = pk[0][: -len("data.pkl")]
         refs = _tensors(_walk_pickle(z.read(pk[0])))
-        want = [shape for _, shape in SHAPES]
+        want = [shape for _, shape in shapes]
         parts, k = {}, 0
         for p_, key, dtype, offset, size, stride in refs:
             if k < len(want) and size == want[k]:
@@ -243,11 +257,12 @@ def load_torch_zip(path: str) -> np.ndarray:
                 if stride != expect:
                     raise ValueError(f"{path}: tensor {p_} is not contiguous")
                 raw = np.frombuffer(z.read(f"{root}data/{key}"), dtype="<f4")
-                parts[SHAPES[k][0]] = raw[offset:offset + n].reshape(size)
+                parts[shapes[k][0]] = raw[offset:offset + n].reshape(size)
                 k += 1
         if k != len(want):
-            raise ValueError(f"{path}: found {k} of the 6 MLP tensors ({[r[4] for r in refs]})")
-    return pack(parts)
+            raise ValueError(f"{path}: found {k} of the {len(want)} MLP tensors "
+                             f"({[r[4] for r in refs]})")
+    return pack(parts, shapes)
 
 
 def load_pkl(path: str) -> np.ndarray:
@@ -275,3 +290,14 @@ def load(base: str) -> np.ndarray:
     if path.endswith(".npy"):
         return pack(unpack(np.load(path, allow_pickle=False)))
     return load_torch_zip(path)
+
+
+def load_net(path: str, din: int) -> np.ndarray:
+    """Weights of a three-body (din 6) or self (din 3) net from a torch-saved
+    module archive (.pkl), a .npy flat vector, or "xavier:<seed>"."""
+    shapes = net_shapes(din)
+    if path.startswith("xavier:"):
+        return xavier(int(path.split(":", 1)[1]), shapes)
+    if path.endswith(".npy"):
+        return pack(unpack(np.load(path, allow_pickle=False), shapes), shapes)
+    return load_torch_zip(path, shapes)

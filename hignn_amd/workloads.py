@@ -87,6 +87,25 @@ def filaments(nfil: int, nbead: int, bond: float, phi: float, seed: int) -> np.n
     return np.ascontiguousarray(pts.reshape(n, 3).astype(np.float32))
 
 
+def filament_cloud(nfil: int = 1024, nchain: int = 21, rest: float = 3.0, radius: float = 240.0,
+                   seed: int = 0) -> np.ndarray:
+    """Straight filaments of ``nchain`` beads at spacing ``rest`` in random
+    directions, centres uniform in the ball of the given radius (less half a
+    filament length) -- the shape of the reference's filament_cloud
+    configuration (python/config/filament_cloud.json: 1024 x 21, rest length
+    3, R2 = 240).  Bead k of filament f is row f*nchain+k."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    half = 0.5 * (nchain - 1) * rest
+    r = (radius - half) * rng.uniform(0.0, 1.0, size=nfil) ** (1.0 / 3.0)
+    c = rng.normal(size=(nfil, 3))
+    c *= (r / np.linalg.norm(c, axis=1))[:, None]
+    d = rng.normal(size=(nfil, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    s = (np.arange(nchain) - 0.5 * (nchain - 1)) * rest
+    pts = c[:, None, :] + d[:, None, :] * s[None, :, None]
+    return np.ascontiguousarray(pts.reshape(-1, 3).astype(np.float32))
+
+
 def gravity(n: int, g=(0.0, 0.0, -1.0)) -> np.ndarray:
     f = np.empty((n, 3), dtype=np.float32)
     f[:] = np.asarray(g, dtype=np.float32)
