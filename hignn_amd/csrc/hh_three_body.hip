@@ -18,6 +18,7 @@
 // atomic accumulation of the velocities.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "hh_three_body.h"
@@ -272,43 +273,59 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_act_kernel(const T *A, const
 }
 
 // ---------------------------------------------------------------------------
-// Layer 4 (512 -> 9), the 3x3 block over the radii and its product with the
-// force; one wave per edge, 8 inputs per lane, wave sums, FP64 atomics.
+// Layer 4 (512 -> 9) on MFMA, then the 3x3 block over the radii times the
+// force: each wave takes 16 edges, y (16 x 16, outputs 9..15 zero) =
+// h3 (16 x 512) W4^T with W4^T in LDS; y goes through LDS so that 48 lanes
+// form (edge, row a) of M f and add it to the target's FP64 velocity.
+// Blocks stride over the batch so W4 is staged once per block.  Rows are read
+// up to ceil(count / 64) * 64 <= the padded batch.
 // ---------------------------------------------------------------------------
-constexpr int kL4Edges = 64;  // edges per block (16 per wave)
+constexpr int kL4Blocks = 2048;
 
 template <class T>
 __global__ __launch_bounds__(kThreads) void layer4_kernel(EdgeBatch B, const T *h3,
                                                           const float *w4, const float *b4,
                                                           double *u) {
-  __shared__ float sw[9 * 512];
-  for (int k = threadIdx.x; k < 9 * 512; k += kThreads) sw[k] = w4[k];
+  typedef Gemm<T> G;
+  constexpr int RS = 512 + G::kPad;
+  constexpr int KM = G::kKS / G::kSub;  // K per MFMA
+  __shared__ __attribute__((aligned(16))) T sw[16 * RS];
+  __shared__ float ys[4][16][17];
+  for (int k = threadIdx.x; k < 16 * 512; k += kThreads) {
+    const int o = k / 512, c = k % 512;
+    sw[o * RS + c] = to_t<T>(o < 9 ? w4[o * 512 + c] : 0.0f);
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int q = 0; q < kL4Edges / 4; q++) {
-    const int64_t e = (int64_t)blockIdx.x * kL4Edges + wave * (kL4Edges / 4) + q;
-    if (e >= B.count) break;  // e is uniform across the wave
-    float hv[8];
-    for (int j = 0; j < 8; j++) hv[j] = (float)h3[e * 512 + 8 * lane + j];
-    float y[9];
-    for (int c = 0; c < 9; c++) {
-      float s = 0.0f;
-      for (int j = 0; j < 8; j++) s = __builtin_fmaf(sw[c * 512 + 8 * lane + j], hv[j], s);
-      for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-      y[c] = s;
-    }
-    if (lane < 3) {
-      const float4 rr = B.rec[e];
-      const float4 fr = B.frc[e];
-      const int32_t tgt = __float_as_int(fr.w);
-      float v = 0.0f;
-      const float f[3] = {fr.x, fr.y, fr.z};
-      for (int c = 0; c < 3; c++) {
-        const float m = ((y[3 * lane + c] + b4[3 * lane + c]) / rr.x) / rr.y;
-        v = __builtin_fmaf(m, f[c], v);
+  const T *wrow = sw + (lane & 15) * RS;
+  // block-uniform trip count (the loop holds barriers); a wave whose 16 edges
+  // start past the batch multiplies padded rows (ceil(count/128)*128 exist)
+  // and skips the epilogue
+  for (int64_t b0 = (int64_t)blockIdx.x * 64; b0 < B.count; b0 += (int64_t)gridDim.x * 64) {
+    const int64_t e0 = b0 + 16 * wave;
+    const T *arow = h3 + (e0 + (lane & 15)) * 512;
+    f32x4 acc = f32x4{0, 0, 0, 0};
+#pragma unroll 4
+    for (int s = 0; s < 512 / KM; s++) acc = G::mfma(G::frag(arow, s, lane), G::frag(wrow, s, lane), acc);
+    // lane (p, g) holds y[edge 4g + r][output p]
+    for (int r = 0; r < 4; r++) ys[wave][4 * (lane >> 4) + r][lane & 15] = acc[r];
+    __syncthreads();  // uniform: every wave runs the same number of iterations per block
+    if (lane < 48) {
+      const int q = lane / 3, a = lane % 3;
+      const int64_t e = e0 + q;
+      if (e < B.count) {
+        const float4 rr = B.rec[e];
+        const float4 fr = B.frc[e];
+        const float f[3] = {fr.x, fr.y, fr.z};
+        float v = 0.0f;
+        for (int c = 0; c < 3; c++) {
+          const float m = ((ys[wave][q][3 * a + c] + b4[3 * a + c]) / rr.x) / rr.y;
+          v = __builtin_fmaf(m, f[c], v);
+        }
+        atomicAdd(&u[3 * (int64_t)__float_as_int(fr.w) + a], (double)v);
       }
-      atomicAdd(&u[3 * (int64_t)tgt + lane], (double)v);
     }
+    __syncthreads();
   }
 }
 
@@ -343,7 +360,7 @@ hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int h
   const int64_t epad = (B.count + kTile - 1) / kTile * kTile;
   const dim3 g1((unsigned)((B.count + kL1Edges - 1) / kL1Edges));
   const dim3 g2(1024 / kTile, (unsigned)(epad / kTile)), g3(512 / kTile, (unsigned)(epad / kTile));
-  const dim3 g4((unsigned)((B.count + kL4Edges - 1) / kL4Edges));
+  const dim3 g4((unsigned)std::min<int64_t>((B.count + 63) / 64, kL4Blocks));
   if (half) {
     typedef _Float16 H;
     if (din == 6)
