@@ -190,7 +190,6 @@ struct hh_three_body {
 
   void run_edges(const Net &net, const int32_t *edges, int64_t count, const float *x,
                  const float *f, hipStream_t st) {
-    const size_t es = prec == HH_PREC_FP16 ? 2 : 4;
     for (int64_t first = 0; first < count; first += kBatch) {
       EdgeBatch B{};
       B.x = x;
@@ -200,7 +199,6 @@ struct hh_three_body {
       B.count = std::min<int64_t>(kBatch, count - first);
       B.rec = rec.as<float4>();
       B.frc = frc.as<float4>();
-      (void)es;
       HIPCHK(launch_edge_mlp(B, net.dev(), net.din, prec == HH_PREC_FP16 ? 1 : 0, h1.p, h2.p, h3.p,
                              u64.as<double>(), st));
     }
@@ -216,7 +214,7 @@ struct hh_three_body {
     lists.triplets(st);
     const size_t es = prec == HH_PREC_FP16 ? 2 : 4;
     const int64_t bmax = std::min<int64_t>(kBatch, std::max(lists.ntrip, lists.nnb));
-    const int64_t bpad = (std::max<int64_t>(bmax, 1) + 127) / 128 * 128;
+    const int64_t bpad = (std::max<int64_t>(bmax, 1) + 255) / 256 * 256;  // GEMM tile rows
     h1.reserve(es * 256 * (size_t)bpad);
     h2.reserve(es * 1024 * (size_t)bpad);
     h3.reserve(es * 512 * (size_t)bpad);

@@ -32,7 +32,7 @@ hipError_t launch_triplet_count(const uint32_t *off, const int32_t *idx, int64_t
 hipError_t launch_triplet_fill(const uint32_t *off, const int32_t *idx, int64_t n,
                                const uint32_t *toff, int32_t *trip, hipStream_t st);
 // u (FP64, n x 3) += the batch's velocity contributions; h1/h2/h3 hold
-// ceil(count / 128) * 128 rows of 256 / 1024 / 512 activations (FP16 if
+// ceil(count / 256) * 256 rows of 256 / 1024 / 512 activations (FP16 if
 // half, else FP32).
 hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int half, void *h1,
                            void *h2, void *h3, double *u, hipStream_t st);

@@ -140,17 +140,15 @@ __global__ __launch_bounds__(kThreads) void layer1_kernel(EdgeBatch B, const flo
 
 // ---------------------------------------------------------------------------
 // C[e][o] = tanh(sum_k A[e][k] W[o][k] + b[o]) for a batch of edges:
-// 128 x 128 tiles, 256 threads in 2 x 2 waves of 64 x 64.  K is staged
+// large tiles (kTM x kTN below) in waves of WM x WN.  K is staged
 // through two LDS buffers: while the waves multiply one K step, the next is
 // in flight from global memory into registers and then stored to the other
 // buffer, one barrier per step.  MFMA 16x16x32 f16 (FP16 path, K steps of 64)
 // or 16x16x4 f32 (FP32 path, K steps of 32); lane l holds
 // D[4 (l/16) + r][l % 16] of a 16 x 16 tile.
-// Requires E a multiple of 128 (padded batch), O a multiple of 128 and K a
+// Requires E a multiple of TM (padded batch), O a multiple of TN and K a
 // multiple of the K step.
 // ---------------------------------------------------------------------------
-constexpr int kTile = 128;
-
 template <class T>
 struct Gemm;
 
@@ -180,50 +178,65 @@ struct Gemm<float> {
   }
 };
 
-template <class T>
-__global__ __launch_bounds__(kThreads, 2) void gemm_act_kernel(const T *A, const T *W,
-                                                               const float *bias, T *C, int K,
-                                                               int O) {
+// Workgroup tile TM x TN (edges x outputs) in waves of WM x WN.
+template <class T, int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * (TM / WM) * (TN / WN), 256 / ((TM / WM) * (TN / WN) * 64) + 1)
+void gemm_act_kernel(
+    const T *A, const T *W, const float *bias, T *C, int K, int O, int ntiles) {
   typedef Gemm<T> G;
+  constexpr int NTH = 64 * (TM / WM) * (TN / WN);   // threads
+  constexpr int MT = WM / 16, NT = WN / 16;         // 16 x 16 tiles per wave
   constexpr int KS = G::kKS;
   constexpr int RS = KS + G::kPad;                  // LDS row stride (elements)
   constexpr int CPR = KS * (int)sizeof(T) / 16;     // 16-byte chunks per row and stage
-  constexpr int PER = kTile * CPR / kThreads;       // chunks per thread per operand
+  constexpr int PA = TM * CPR / NTH, PW = TN * CPR / NTH;  // chunks per thread
   constexpr int EPC = 16 / (int)sizeof(T);          // elements per chunk
-  constexpr int STAGE = kTile * RS;                 // elements of one operand stage
-  __shared__ __attribute__((aligned(16))) T lds[2 * 2 * STAGE];  // [stage][operand]
+  constexpr int SA = TM * RS, SW = TN * RS;         // elements of one stage
+  static_assert(PA * NTH == TM * CPR && PW * NTH == TN * CPR, "tile / thread mismatch");
+  __shared__ __attribute__((aligned(16))) T lds[2 * (SA + SW)];  // [stage][A | W]
 
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
+  // launch index L runs on XCD L % 8; XCD k takes the k-th contiguous run of
+  // row-major tiles, and the column tiles of a row block share its L2.
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (tile >= ntiles) return;
+  const int nx = O / TN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int64_t e0 = (int64_t)blockIdx.y * kTile;
-  const int o0 = blockIdx.x * kTile;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int64_t e0 = (int64_t)(tile / nx) * TM;
+  const int o0 = (tile % nx) * TN;
+  const int wr = (wave / (TN / WN)) * WM, wc = (wave % (TN / WN)) * WN;
   const T *ga = A + e0 * K, *gw = W + (int64_t)o0 * K;
 
-  i32x4 ra[PER], rw[PER];
-  int goff[PER], soff[PER];
+  i32x4 ra[PA], rw[PW];
+  int gao[PA], sao[PA], gwo[PW], swo[PW];
 #pragma unroll
-  for (int j = 0; j < PER; j++) {
-    const int c = t + kThreads * j, row = c / CPR, q = c % CPR;
-    goff[j] = row * K + q * EPC;
-    soff[j] = row * RS + q * EPC;
+  for (int j = 0; j < PA; j++) {
+    const int c = t + NTH * j, row = c / CPR, q = c % CPR;
+    gao[j] = row * K + q * EPC;
+    sao[j] = row * RS + q * EPC;
+  }
+#pragma unroll
+  for (int j = 0; j < PW; j++) {
+    const int c = t + NTH * j, row = c / CPR, q = c % CPR;
+    gwo[j] = row * K + q * EPC;
+    swo[j] = row * RS + q * EPC;
   }
 
-  f32x4 acc[4][4];
+  f32x4 acc[MT][NT];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < MT; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    for (int b = 0; b < NT; b++) acc[a][b] = f32x4{0, 0, 0, 0};
 
 #pragma unroll
-  for (int j = 0; j < PER; j++) {
-    ra[j] = *(const i32x4 *)(ga + goff[j]);
-    rw[j] = *(const i32x4 *)(gw + goff[j]);
-  }
+  for (int j = 0; j < PA; j++) ra[j] = *(const i32x4 *)(ga + gao[j]);
 #pragma unroll
-  for (int j = 0; j < PER; j++) {
-    *(i32x4 *)(lds + soff[j]) = ra[j];
-    *(i32x4 *)(lds + STAGE + soff[j]) = rw[j];
-  }
+  for (int j = 0; j < PW; j++) rw[j] = *(const i32x4 *)(gw + gwo[j]);
+#pragma unroll
+  for (int j = 0; j < PA; j++) *(i32x4 *)(lds + sao[j]) = ra[j];
+#pragma unroll
+  for (int j = 0; j < PW; j++) *(i32x4 *)(lds + SA + swo[j]) = rw[j];
   __syncthreads();
 
   const int nsteps = K / KS;
@@ -232,45 +245,57 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_act_kernel(const T *A, const
     // next stage (the last step reloads itself into the idle buffer)
     const int k1 = min(step + 1, nsteps - 1) * KS;
 #pragma unroll
-    for (int j = 0; j < PER; j++) {
-      ra[j] = *(const i32x4 *)(ga + goff[j] + k1);
-      rw[j] = *(const i32x4 *)(gw + goff[j] + k1);
-    }
-    const T *sa = lds + cur * 2 * STAGE, *sw = sa + STAGE;
+    for (int j = 0; j < PA; j++) ra[j] = *(const i32x4 *)(ga + gao[j] + k1);
+#pragma unroll
+    for (int j = 0; j < PW; j++) rw[j] = *(const i32x4 *)(gw + gwo[j] + k1);
+    const T *sa = lds + cur * (SA + SW), *sw = sa + SA;
 #pragma unroll
     for (int s = 0; s < G::kSub; s++) {
-      typename G::Frag fa[4], fb[4];
+      typename G::Frag fa[MT], fb[NT];
 #pragma unroll
-      for (int m = 0; m < 4; m++) fa[m] = G::frag(sa + (wr + 16 * m + (lane & 15)) * RS, s, lane);
+      for (int m = 0; m < MT; m++) fa[m] = G::frag(sa + (wr + 16 * m + (lane & 15)) * RS, s, lane);
 #pragma unroll
-      for (int m = 0; m < 4; m++) fb[m] = G::frag(sw + (wc + 16 * m + (lane & 15)) * RS, s, lane);
+      for (int m = 0; m < NT; m++) fb[m] = G::frag(sw + (wc + 16 * m + (lane & 15)) * RS, s, lane);
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < MT; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = G::mfma(fa[a], fb[b], acc[a][b]);
+        for (int b = 0; b < NT; b++) acc[a][b] = G::mfma(fa[a], fb[b], acc[a][b]);
     }
-    T *da = lds + (cur ^ 1) * 2 * STAGE;
+    T *da = lds + (cur ^ 1) * (SA + SW);
 #pragma unroll
-    for (int j = 0; j < PER; j++) {
-      *(i32x4 *)(da + soff[j]) = ra[j];
-      *(i32x4 *)(da + STAGE + soff[j]) = rw[j];
-    }
+    for (int j = 0; j < PA; j++) *(i32x4 *)(da + sao[j]) = ra[j];
+#pragma unroll
+    for (int j = 0; j < PW; j++) *(i32x4 *)(da + SA + swo[j]) = rw[j];
     __syncthreads();
   }
 
+  // epilogue: bias + tanh into the (now idle) LDS as a row-major tile, then
+  // 16-byte coalesced stores of whole rows
+  constexpr int TS = TN + EPC;  // tile row stride (elements)
+  static_assert(TM * TS <= 2 * (SA + SW), "output tile does not fit the stage buffers");
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int o = o0 + wc + 16 * b + (lane & 15);
-    const float bo = bias[o];
+  for (int b = 0; b < NT; b++) {
+    const int col = wc + 16 * b + (lane & 15);
+    const float bo = bias[o0 + col];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < MT; a++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t e = e0 + wr + 16 * a + 4 * (lane >> 4) + r;
-        C[e * O + o] = to_t<T>(act(acc[a][b][r] + bo));
-      }
+      for (int r = 0; r < 4; r++)
+        lds[(wr + 16 * a + 4 * (lane >> 4) + r) * TS + col] = to_t<T>(act(acc[a][b][r] + bo));
+  }
+  __syncthreads();
+  constexpr int OPR = TN / EPC;  // 16-byte chunks per output row
+#pragma unroll 4
+  for (int c = t; c < TM * OPR; c += NTH) {
+    const int row = c / OPR, q = c % OPR;
+    *(i32x4 *)(C + (e0 + row) * O + o0 + q * EPC) = *(const i32x4 *)(lds + row * TS + q * EPC);
   }
 }
+
+// Tiles of the two layer GEMMs: 128 x 128 in 4 waves of 64 x 64 (two
+// workgroups per CU).
+constexpr int kTM = 128, kTN = 128, kWM = 64, kWN = 64;
+constexpr int kGemmThreads = 64 * (kTM / kWM) * (kTN / kWN);
 
 // ---------------------------------------------------------------------------
 // Layer 4 (512 -> 9) on MFMA, then the 3x3 block over the radii times the
@@ -299,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void layer4_kernel(EdgeBatch B, const T *
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const T *wrow = sw + (lane & 15) * RS;
   // block-uniform trip count (the loop holds barriers); a wave whose 16 edges
-  // start past the batch multiplies padded rows (ceil(count/128)*128 exist)
+  // start past the batch multiplies padded rows (ceil(count/256)*256 exist)
   // and skips the epilogue
   for (int64_t b0 = (int64_t)blockIdx.x * 64; b0 < B.count; b0 += (int64_t)gridDim.x * 64) {
     const int64_t e0 = b0 + 16 * wave;
@@ -357,9 +382,10 @@ hipError_t launch_triplet_fill(const uint32_t *off, const int32_t *idx, int64_t 
 hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int half, void *h1,
                            void *h2, void *h3, double *u, hipStream_t st) {
   if (B.count <= 0) return hipSuccess;
-  const int64_t epad = (B.count + kTile - 1) / kTile * kTile;
+  const int64_t epad = (B.count + kTM - 1) / kTM * kTM;
   const dim3 g1((unsigned)((B.count + kL1Edges - 1) / kL1Edges));
-  const dim3 g2(1024 / kTile, (unsigned)(epad / kTile)), g3(512 / kTile, (unsigned)(epad / kTile));
+  const int t2 = (int)(1024 / kTN * (epad / kTM)), t3 = (int)(512 / kTN * (epad / kTM));
+  const dim3 g2((unsigned)((t2 + 7) / 8 * 8)), g3((unsigned)((t3 + 7) / 8 * 8));
   const dim3 g4((unsigned)std::min<int64_t>((B.count + 63) / 64, kL4Blocks));
   if (half) {
     typedef _Float16 H;
@@ -369,10 +395,10 @@ hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int h
     else
       hipLaunchKernelGGL((layer1_kernel<3, H>), g1, dim3(kThreads), 0, st, B, net.w1s, net.b1s,
                          (H *)h1);
-    hipLaunchKernelGGL(gemm_act_kernel<H>, g2, dim3(kThreads), 0, st, (const H *)h1,
-                       (const H *)net.w2h, net.b2s, (H *)h2, 256, 1024);
-    hipLaunchKernelGGL(gemm_act_kernel<H>, g3, dim3(kThreads), 0, st, (const H *)h2,
-                       (const H *)net.w3h, net.b3s, (H *)h3, 1024, 512);
+    hipLaunchKernelGGL((gemm_act_kernel<H, kTM, kTN, kWM, kWN>), g2, dim3(kGemmThreads), 0, st,
+                       (const H *)h1, (const H *)net.w2h, net.b2s, (H *)h2, 256, 1024, t2);
+    hipLaunchKernelGGL((gemm_act_kernel<H, kTM, kTN, kWM, kWN>), g3, dim3(kGemmThreads), 0, st,
+                       (const H *)h2, (const H *)net.w3h, net.b3s, (H *)h3, 1024, 512, t3);
     hipLaunchKernelGGL(layer4_kernel<H>, g4, dim3(kThreads), 0, st, B, (const H *)h3, net.w4,
                        net.b4, u);
   } else {
@@ -382,10 +408,10 @@ hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int h
     else
       hipLaunchKernelGGL((layer1_kernel<3, float>), g1, dim3(kThreads), 0, st, B, net.w1s,
                          net.b1s, (float *)h1);
-    hipLaunchKernelGGL(gemm_act_kernel<float>, g2, dim3(kThreads), 0, st, (const float *)h1,
-                       net.w2s, net.b2s, (float *)h2, 256, 1024);
-    hipLaunchKernelGGL(gemm_act_kernel<float>, g3, dim3(kThreads), 0, st, (const float *)h2,
-                       net.w3s, net.b3s, (float *)h3, 1024, 512);
+    hipLaunchKernelGGL((gemm_act_kernel<float, kTM, kTN, kWM, kWN>), g2, dim3(kGemmThreads), 0,
+                       st, (const float *)h1, net.w2s, net.b2s, (float *)h2, 256, 1024, t2);
+    hipLaunchKernelGGL((gemm_act_kernel<float, kTM, kTN, kWM, kWN>), g3, dim3(kGemmThreads), 0,
+                       st, (const float *)h2, net.w3s, net.b3s, (float *)h3, 1024, 512, t3);
     hipLaunchKernelGGL(layer4_kernel<float>, g4, dim3(kThreads), 0, st, B, (const float *)h3,
                        net.w4, net.b4, u);
   }
