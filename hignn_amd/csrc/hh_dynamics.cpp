@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "../../include/hh_dynamics.h"
+#include "../../include/hh_three_body.h"
 #include "hh_cells.h"
 #include "hh_forcing.h"
 #include "hh_host.h"
@@ -67,6 +68,7 @@ struct hh_forcing {
 struct hh_stepper {
   hh_model *m = nullptr;
   hh_forcing *fc = nullptr;
+  hh_three_body *tb = nullptr;  // optional three-body / self corrections
   int method = HH_STEP_EULER;
   int64_t n = 0;
   double dt = 1.0, tf = 10.0, thr = 1e-3;
@@ -170,6 +172,9 @@ struct hh_stepper {
     if (hh_model_update_coord_device(m, XIN.as<float>(), n, st) != HH_OK ||
         hh_model_dot_device(m, out.as<float>(), F.as<float>(), n, st) != HH_OK)
       throw HHError(HH_ERR_STATE, std::string("model: ") + hh_last_error());
+    if (tb && hh_three_body_velocity_device(tb, XIN.as<float>(), F.as<float>(), n, out.as<float>(), 1,
+                                            st) != HH_OK)
+      throw HHError(HH_ERR_STATE, std::string("three-body: ") + hh_last_error());
     evals++;
   }
 
@@ -405,6 +410,10 @@ int hh_stepper_set_lim(hh_stepper *s, int axis, float lo, float hi) {
     s->periodic = true;
     s->drop_graphs();
   });
+}
+
+int hh_stepper_set_three_body(hh_stepper *s, hh_three_body *tb) {
+  return guarded([&]() { need(s)->tb = tb; });
 }
 
 int hh_stepper_set_graphs(hh_stepper *s, int flag) {

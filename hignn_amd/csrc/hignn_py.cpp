@@ -438,11 +438,12 @@ class Integrator {
   // Device-resident velocity: every stage evaluates `forcing` on the device,
   // updates the model's coordinates and applies M(x) on the device; no
   // Python callback and no host copy of the particle arrays per stage.
-  void set_device_velocity(Model &model, Forcing &forcing) {
+  void set_device_velocity(Model &model, Forcing &forcing, ThreeBody *three_body) {
     if ((size_t)model.size() != n_ && n_ != 0)
       throw std::runtime_error("model and integrator sizes differ");
     dm_ = &model;
     df_ = &forcing;
+    dt3_ = three_body;
     if (stepper_) {
       hh_stepper_destroy(stepper_);
       stepper_ = nullptr;
@@ -525,6 +526,7 @@ class Integrator {
   // ---- device-resident mode (set_device_velocity) ----
   Model *dm_ = nullptr;
   Forcing *df_ = nullptr;
+  ThreeBody *dt3_ = nullptr;
   hh_stepper *stepper_ = nullptr;
   bool graphs_ = true;
   int64_t stats_[3] = {0, 0, 0};
@@ -544,6 +546,7 @@ class Integrator {
   void run_device(int method, double threshold) {
     if ((size_t)dm_->size() != n_) throw std::runtime_error("model and integrator sizes differ");
     if (!stepper_) check(hh_stepper_create(dm_->handle(), df_->handle(), method, &stepper_));
+    check(hh_stepper_set_three_body(stepper_, dt3_ ? dt3_->handle() : nullptr));
     check(hh_stepper_set(stepper_, dt_, tf_, threshold));
     if (periodic_)
       for (int d = 0; d < 3; d++) check(hh_stepper_set_lim(stepper_, d, lim_[0][d], lim_[1][d]));
@@ -724,8 +727,9 @@ PYBIND11_MODULE(hignn, m) {
       .def("initialize", &Euler::initialize)
       .def("run", &Euler::run)
       .def("func_count", &Euler::func_count)
-      .def("set_device_velocity", &Euler::set_device_velocity, py::keep_alive<1, 2>(),
-           py::keep_alive<1, 3>())
+      .def("set_device_velocity", &Euler::set_device_velocity, py::arg("model"),
+           py::arg("forcing"), py::arg("three_body") = nullptr, py::keep_alive<1, 2>(),
+           py::keep_alive<1, 3>(), py::keep_alive<1, 4>())
       .def("set_graphs", &Euler::set_graphs)
       .def("positions", &Euler::positions)
       .def("device_stats", &Euler::device_stats);
@@ -745,8 +749,9 @@ PYBIND11_MODULE(hignn, m) {
       .def("initialize", &Rk4::initialize)
       .def("run", &Rk4::run)
       .def("func_count", &Rk4::func_count)
-      .def("set_device_velocity", &Rk4::set_device_velocity, py::keep_alive<1, 2>(),
-           py::keep_alive<1, 3>())
+      .def("set_device_velocity", &Rk4::set_device_velocity, py::arg("model"),
+           py::arg("forcing"), py::arg("three_body") = nullptr, py::keep_alive<1, 2>(),
+           py::keep_alive<1, 3>(), py::keep_alive<1, 4>())
       .def("set_graphs", &Rk4::set_graphs)
       .def("positions", &Rk4::positions)
       .def("device_stats", &Rk4::device_stats);

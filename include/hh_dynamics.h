@@ -82,6 +82,13 @@ int hh_stepper_set(hh_stepper *s, double time_step, double final_time, double th
  * wrapping of positions along axis 0..2. */
 int hh_stepper_set_lim(hh_stepper *s, int axis, float lo, float hi);
 
+/* Adds the three-body and self corrections of tb (include/hh_three_body.h)
+ * to every stage velocity, as python/gravity_field.py:58-75 adds
+ * velocity_update_without_2body to the matvec; NULL removes them.  tb must
+ * outlive the stepper.  No reference counterpart as a native call. */
+struct hh_three_body;
+int hh_stepper_set_three_body(hh_stepper *s, struct hh_three_body *tb);
+
 /* Per-stage HIP graphs for the stage combination and forcing (default on). */
 int hh_stepper_set_graphs(hh_stepper *s, int flag);
 

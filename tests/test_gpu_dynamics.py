@@ -188,3 +188,44 @@ def test_device_dopri_matches_host(xavier_w):
     assert ch == cd and len(oh) == len(od)
     print("dopri steps:", ch, "max |x_dev - x_host|:", np.abs(xd - xh).max())
     assert close(xd, xh)
+
+
+def test_device_euler_with_three_body_matches_host(xavier_w):
+    # the velocity of python/gravity_field.py:58-75: matvec + three-body and self corrections
+    from hignn_amd import weights as W
+    h = hignn()
+    x = suspension(7, n_fil=4, n_chain=20, n_free=200)
+    tb_w = (W.xavier(1, W.net_shapes(6)), W.xavier(2, W.net_shapes(3)))
+
+    def run(device):
+        m = model(h, x, xavier_w)
+        fc = h.DeviceForcing()
+        fc.set_body_force([0.0, 0.0, -1.0])
+        tb = h.ThreeBodyModel()
+        tb.load_weights(*tb_w)
+        tb.set_three_body_epsilon(5.0)
+        it = h.ExplicitEuler()
+        it.set_time_step(0.01)
+        it.set_final_time(0.03)
+        it.set_num_rigid_body(x.shape[0])
+        if device:
+            it.set_device_velocity(m, fc, tb)
+        else:
+            def vel(t, pos):
+                xx = np.ascontiguousarray(pos[:, :3])
+                m.update_coord(xx)
+                f = fc.eval(xx)
+                u = np.zeros_like(f)
+                m.dot(u, f)
+                return u + tb.velocity(xx, f)
+            it.set_velocity_func(vel)
+        state = np.zeros((x.shape[0], 6), f32)
+        state[:, :3] = x
+        it.initialize(state)
+        it.run()
+        return it.positions()
+
+    xh, xd = run(False), run(True)
+    print("euler + three-body max |x_dev - x_host|:", np.abs(xd - xh).max())
+    assert np.abs(xh - x).max() > 1e-4
+    assert close(xd, xh)
