@@ -83,6 +83,8 @@ struct AcaStepLaunch {
   int nactive;
   int64_t nchunks_c, nchunks_r;
   int pool_f64;
+  int pool_c16;  // BF16 C pool (16-bit modes)
+  int pool_q16;  // BF16 Q pool (16-bit modes)
   void *C;
   void *Q;
 };
@@ -96,6 +98,8 @@ struct AcaApplyLaunch {
   int use_symmetry;
   int nlist;
   int pool_f64;
+  int pool_c16;
+  int pool_q16;
   const void *C;
   const void *Q;
   const int32_t *first_i, *first_j;  // apply slot -> first chunk over I / J (nlist + 1)

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstdint>
 
 #include "hh_internal.h"
@@ -615,10 +617,10 @@ struct AcaStepArgs {
   int chunk;                // items per chunk, 1..kAcaChunk
 };
 
-template <class P>
-__device__ __forceinline__ void load9(const P *src, P (&d)[9]) {
+template <class S, class P>
+__device__ __forceinline__ void load9(const S *src, P (&d)[9]) {
 #pragma unroll
-  for (int e = 0; e < 9; e++) d[e] = src[e];
+  for (int e = 0; e < 9; e++) d[e] = (P)src[e];
 }
 
 template <class P>
@@ -752,9 +754,9 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 constexpr int kCrossLC = 1;
 constexpr int kCrossN = 9 * kCrossLC;  // cross sums per wave reduction
 
-template <class P, int SIDE>
+template <class P, class S, int SIDE>
 __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
-                                               const P *pool, int64_t stride, int k,
+                                               const S *pool, int64_t stride, int k,
                                                const P (*piv)[9], P *sh, double *X) {
   const int ncross = k - 2;  // cross terms of iteration k-1: l < k-2
   P a[kAcaIpt][9];
@@ -763,7 +765,7 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
 #pragma unroll
     for (int e = 0; e < 9; e++) {
       sum[s][e] = (P)0;
-      a[s][e] = (valid[s] && ncross > 0) ? pool[9 * ((int64_t)(k - 1) * stride + 64 * s) + e] : (P)0;
+      a[s][e] = (valid[s] && ncross > 0) ? (P)pool[9 * ((int64_t)(k - 1) * stride + 64 * s) + e] : (P)0;
     }
   for (int l0 = 0; l0 < k; l0 += kCrossLC) {
     const int nl = min(kCrossLC, k - l0);
@@ -820,8 +822,8 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
 // Column pass residual R_k, the column cross-term sums of iteration k-1 and
 // the row pivot candidates of one chunk (src/FarDot.cpp:474-580).  One wave
 // per chunk.
-template <class P>
-__global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp, const P *Qp) {
+template <class P, class SC, class SQ>
+__global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp, const SQ *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[128];
   __shared__ P sh[kCrossN * 65];
@@ -831,7 +833,7 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp,
   const int k = A.it[b];
   const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
   for (int t = threadIdx.x; t < 9 * k; t += 64)
-    piv[t / 9][t % 9] = Qp[9 * (fb.qoff + (int64_t)(t / 9) * fb.n + jk) + t % 9];
+    piv[t / 9][t % 9] = (P)Qp[9 * (fb.qoff + (int64_t)(t / 9) * fb.n + jk) + t % 9];
   for (int l = threadIdx.x; l < k; l += 64) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
   __syncthreads();
   const int iend = min(ch.off + A.chunk, fb.m);
@@ -839,8 +841,8 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp,
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < iend;
   P c[kAcaIpt][9];
-  residual_cross<P, 0>(c, valid, Cp + 9 * (fb.coff + ch.off + threadIdx.x), fb.m, k, piv, sh,
-                       A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
+  residual_cross<P, SC, 0>(c, valid, Cp + 9 * (fb.coff + ch.off + threadIdx.x), fb.m, k, piv, sh,
+                           A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     if (!valid[s]) continue;
@@ -849,14 +851,14 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp,
 #pragma unroll
     for (int e = 0; e < 9; e++) c[s][e] = raw[e] - c[s][e];
   }
-  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
   ArgMax best{kNegHuge, -1};
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     const int i = ch.off + threadIdx.x + 64 * s;
     if (!valid[s]) continue;
 #pragma unroll
-    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = c[s][e];
+    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = (SC)c[s][e];
     bool u = false;
     for (int l = 0; l < k; l++) u |= (used[l] == i);
     const double dt = (double)det3(c[s]);
@@ -871,8 +873,8 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, P *Cp,
 
 // Row pivot of each active block and the inverse of its pivot block
 // (src/FarDot.cpp:537-656).  One wave per block.
-template <class P>
-__global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const P *Cp) {
+template <class SC>
+__global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const SC *Cp) {
   const int slot = blockIdx.x;
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
@@ -884,7 +886,7 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
   if (threadIdx.x == 0) {
     const int ik = best.idx < 0 ? 0 : best.idx;
     A.ipiv[(size_t)b * A.max_iter + k] = ik;
-    const P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
+    const SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
     double m[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) m[e] = (double)Ck[e];
@@ -906,8 +908,8 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
 
 // Column normalisation C_k = R_k inv (src/FarDot.cpp:630-656) and the Gram
 // sums of C_k for one chunk.  One wave per chunk.
-template <class P>
-__global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) {
+template <class P, class SC>
+__global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp) {
   const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) 
   P inv[9];
 #pragma unroll
   for (int e = 0; e < 9; e++) inv[e] = (P)A.inv[9 * (size_t)ch.slot + e];
-  P *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
   const int iend = min(ch.off + A.chunk, fb.m);
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -932,8 +934,9 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) 
     double a[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      Ck[9 * (int64_t)i + e] = o[e];
-      a[e] = (double)o[e];
+      const SC st = (SC)o[e];  // the norms see the stored factor
+      Ck[9 * (int64_t)i + e] = st;
+      a[e] = (double)st;
     }
     gram_col(a, g);
   }
@@ -947,8 +950,8 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, P *Cp) 
 // Row pass residual Q_k, the row cross-term sums of iteration k-1, column
 // pivot candidates and the Gram sums of Q_k for one chunk
 // (src/FarDot.cpp:797-1078).  One wave per chunk.
-template <class P>
-__global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const P *Cp, P *Qp) {
+template <class P, class SC, class SQ>
+__global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const SC *Cp, SQ *Qp) {
   __shared__ P piv[128][9];
   __shared__ int32_t used[129];
   __shared__ P sh[kCrossN * 65];
@@ -958,7 +961,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
   const int k = A.it[b];
   const int ik = A.ipiv[(size_t)b * A.max_iter + k];
   for (int t = threadIdx.x; t < 9 * k; t += 64)
-    piv[t / 9][t % 9] = Cp[9 * (fb.coff + (int64_t)(t / 9) * fb.m + ik) + t % 9];
+    piv[t / 9][t % 9] = (P)Cp[9 * (fb.coff + (int64_t)(t / 9) * fb.m + ik) + t % 9];
   for (int l = threadIdx.x; l <= k; l += 64) used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
   __syncthreads();
   const int jend = min(ch.off + A.chunk, fb.n);
@@ -966,8 +969,8 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < jend;
   P q[kAcaIpt][9];
-  residual_cross<P, 1>(q, valid, Qp + 9 * (fb.qoff + ch.off + threadIdx.x), fb.n, k, piv, sh,
-                       A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
+  residual_cross<P, SQ, 1>(q, valid, Qp + 9 * (fb.qoff + ch.off + threadIdx.x), fb.n, k, piv, sh,
+                           A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     if (!valid[s]) continue;
@@ -976,7 +979,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 #pragma unroll
     for (int e = 0; e < 9; e++) q[s][e] = raw[e] - q[s][e];
   }
-  P *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
+  SQ *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
   ArgMax best{kNegHuge, -1};
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -986,8 +989,10 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
     double a[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      Qk[9 * (int64_t)j + e] = q[s][e];
-      a[e] = (double)q[s][e];
+      const SQ st = (SQ)q[s][e];  // pivot and norms see the stored factor
+      Qk[9 * (int64_t)j + e] = st;
+      q[s][e] = (P)st;
+      a[e] = (double)st;
     }
     bool u = false;
     for (int l = 0; l <= k; l++) u |= (used[l] == j);
@@ -1081,9 +1086,11 @@ struct AcaApplyArgs {
   int chunk;                 // items per chunk, 1..kAcaChunk
 };
 
-template <class P, int SIDE>
-__global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const P *Cp,
-                                                              const P *Qp) {
+template <class SC, class SQ, int SIDE>
+__global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const SC *Cp,
+                                                              const SQ *Qp) {
+  // SIDE 0 reduces over the Q pool, SIDE 1 over the C pool
+  typedef typename std::conditional<SIDE == 0, SQ, SC>::type S;
   __shared__ double sh[32 * 65];
   const AcaTile ch = SIDE == 0 ? locate(A.first_j, A.nlist, blockIdx.x, A.chunk)
                                 : locate(A.first_i, A.nlist, blockIdx.x, A.chunk);
@@ -1092,7 +1099,11 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
   const int K = min(A.it[b], 128);
   const int ns = SIDE == 0 ? fb.n : fb.m;
   const int s0 = SIDE == 0 ? fb.js : fb.is;
-  const P *Src = SIDE == 0 ? Qp + 9 * fb.qoff : Cp + 9 * fb.coff;
+  const S *Src;
+  if constexpr (SIDE == 0)
+    Src = Qp + 9 * fb.qoff;
+  else
+    Src = Cp + 9 * fb.coff;
   const int send = min(ch.off + A.chunk, ns);
   double *W = (SIDE == 0 ? A.w_j : A.w_i) + (size_t)blockIdx.x * A.max_iter * 3;
   float4 f[kAcaIpt];
@@ -1113,7 +1124,7 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
       for (int s = 0; s < kAcaIpt; s++) {
         const int it = ch.off + threadIdx.x + 64 * s;
         if (c >= nl || it >= send) continue;
-        const P *q = Src + 9 * ((int64_t)(l0 + c) * ns + it);
+        const S *q = Src + 9 * ((int64_t)(l0 + c) * ns + it);
         if (SIDE == 0) {
           x[3 * c] += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
           x[3 * c + 1] += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
@@ -1129,9 +1140,11 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
   }
 }
 
-template <class P, int SIDE>
-__global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, const P *Cp,
-                                                               const P *Qp) {
+template <class SC, class SQ, int SIDE>
+__global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, const SC *Cp,
+                                                               const SQ *Qp) {
+  // SIDE 0 updates from the C pool, SIDE 1 from the Q pool
+  typedef typename std::conditional<SIDE == 0, SC, SQ>::type S;
   __shared__ double wsh[128 * 3];
   const AcaTile ch = SIDE == 0 ? locate(A.first_i, A.nlist, blockIdx.x, A.chunk)
                                 : locate(A.first_j, A.nlist, blockIdx.x, A.chunk);
@@ -1149,12 +1162,16 @@ __global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, co
   __syncthreads();
   const int nd = SIDE == 0 ? fb.m : fb.n;
   const int d0 = SIDE == 0 ? fb.is : fb.js;
-  const P *Dst = SIDE == 0 ? Cp + 9 * fb.coff : Qp + 9 * fb.qoff;
+  const S *Dst;
+  if constexpr (SIDE == 0)
+    Dst = Cp + 9 * fb.coff;
+  else
+    Dst = Qp + 9 * fb.qoff;
   const int dend = min(ch.off + A.chunk, nd);
   for (int d = ch.off + threadIdx.x; d < dend; d += blockDim.x) {
     double o0 = 0, o1 = 0, o2 = 0;
     for (int l = 0; l < K; l++) {
-      const P *c = Dst + 9 * ((int64_t)l * nd + d);
+      const S *c = Dst + 9 * ((int64_t)l * nd + d);
       const double x0 = wsh[3 * l], x1 = wsh[3 * l + 1], x2 = wsh[3 * l + 2];
       if (SIDE == 0) {
         o0 += (double)c[0] * x0 + (double)c[1] * x1 + (double)c[2] * x2;
@@ -1708,55 +1725,73 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Pool storage: SC for the C factors, SQ for the Q factors; P is the
+// arithmetic type of the steps.
+template <class P, class SC, class SQ>
+static void step_launch(const AcaStepArgs &A, const AcaStepLaunch &L, int which, dim3 gc, dim3 gr,
+                        dim3 ga, hipStream_t st) {
+  if (which == 0) {
+    hipLaunchKernelGGL((aca_col_resid_kernel<P, SC, SQ>), gc, dim3(64), 0, st, A, (SC *)L.C,
+                       (const SQ *)L.Q);
+    hipLaunchKernelGGL(aca_col_pivot_kernel<SC>, ga, dim3(64), 0, st, A, (const SC *)L.C);
+    hipLaunchKernelGGL((aca_col_norm_kernel<P, SC>), gc, dim3(64), 0, st, A, (SC *)L.C);
+  } else {
+    hipLaunchKernelGGL((aca_row_resid_kernel<P, SC, SQ>), gr, dim3(64), 0, st, A,
+                       (const SC *)L.C, (SQ *)L.Q);
+  }
+}
+
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
   if (L.nactive <= 0) return hipSuccess;
   AcaStepArgs A{L.blk, L.active, L.ybase_c, L.ybase_r, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2,
                 L.stop, L.max_iter, L.eps2, L.nactive, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
                 (AcaPart *)L.part_r, L.x_c, L.x_r, L.inv, L.chunk};
   const dim3 gc((unsigned)L.nchunks_c), gr((unsigned)L.nchunks_r), ga((unsigned)L.nactive);
-  if (which == 0) {
-    if (L.pool_f64) {
-      hipLaunchKernelGGL(aca_col_resid_kernel<double>, gc, dim3(64), 0, st, A, (double *)L.C,
-                         (const double *)L.Q);
-      hipLaunchKernelGGL(aca_col_pivot_kernel<double>, ga, dim3(64), 0, st, A,
-                         (const double *)L.C);
-      hipLaunchKernelGGL(aca_col_norm_kernel<double>, gc, dim3(64), 0, st, A, (double *)L.C);
-    } else {
-      hipLaunchKernelGGL(aca_col_resid_kernel<float>, gc, dim3(64), 0, st, A, (float *)L.C,
-                         (const float *)L.Q);
-      hipLaunchKernelGGL(aca_col_pivot_kernel<float>, ga, dim3(64), 0, st, A, (const float *)L.C);
-      hipLaunchKernelGGL(aca_col_norm_kernel<float>, gc, dim3(64), 0, st, A, (float *)L.C);
-    }
-  } else {
-    if (L.pool_f64)
-      hipLaunchKernelGGL(aca_row_resid_kernel<double>, gr, dim3(64), 0, st, A,
-                         (const double *)L.C, (double *)L.Q);
-    else
-      hipLaunchKernelGGL(aca_row_resid_kernel<float>, gr, dim3(64), 0, st, A, (const float *)L.C,
-                         (float *)L.Q);
+  typedef __bf16 B;
+  if (L.pool_f64)
+    step_launch<double, double, double>(A, L, which, gc, gr, ga, st);
+  else if (L.pool_c16 && L.pool_q16)
+    step_launch<float, B, B>(A, L, which, gc, gr, ga, st);
+  else if (L.pool_c16)
+    step_launch<float, B, float>(A, L, which, gc, gr, ga, st);
+  else if (L.pool_q16)
+    step_launch<float, float, B>(A, L, which, gc, gr, ga, st);
+  else
+    step_launch<float, float, float>(A, L, which, gc, gr, ga, st);
+  if (which == 1) {
     hipLaunchKernelGGL(aca_finalize_kernel, ga, dim3(256), 0, st, A);
   }
   return hipGetLastError();
 }
 
-template <class P>
+template <class SC, class SQ>
 static void apply_launch(const AcaApplyArgs &A, const AcaApplyLaunch &L, hipStream_t st) {
-  const P *C = (const P *)L.C, *Q = (const P *)L.Q;
+  const SC *C = (const SC *)L.C;
+  const SQ *Q = (const SQ *)L.Q;
   const dim3 gi((unsigned)L.nchunks_i), gj((unsigned)L.nchunks_j);
-  hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 0>), gj, dim3(64), 0, st, A, C, Q);
-  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_reduce_kernel<P, 1>), gi, dim3(64), 0, st, A, C, Q);
-  hipLaunchKernelGGL((aca_apply_update_kernel<P, 0>), gi, dim3(64), 0, st, A, C, Q);
-  if (L.use_symmetry) hipLaunchKernelGGL((aca_apply_update_kernel<P, 1>), gj, dim3(64), 0, st, A, C, Q);
+  hipLaunchKernelGGL((aca_apply_reduce_kernel<SC, SQ, 0>), gj, dim3(64), 0, st, A, C, Q);
+  if (L.use_symmetry)
+    hipLaunchKernelGGL((aca_apply_reduce_kernel<SC, SQ, 1>), gi, dim3(64), 0, st, A, C, Q);
+  hipLaunchKernelGGL((aca_apply_update_kernel<SC, SQ, 0>), gi, dim3(64), 0, st, A, C, Q);
+  if (L.use_symmetry)
+    hipLaunchKernelGGL((aca_apply_update_kernel<SC, SQ, 1>), gj, dim3(64), 0, st, A, C, Q);
 }
 
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st) {
   if (L.nlist <= 0) return hipSuccess;
   AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, L.nlist, L.first_i, L.first_j, L.w_j, L.w_i,
                  L.max_iter, L.chunk};
+  typedef __bf16 B;
   if (L.pool_f64)
-    apply_launch<double>(A, L, st);
+    apply_launch<double, double>(A, L, st);
+  else if (L.pool_c16 && L.pool_q16)
+    apply_launch<B, B>(A, L, st);
+  else if (L.pool_c16)
+    apply_launch<B, float>(A, L, st);
+  else if (L.pool_q16)
+    apply_launch<float, B>(A, L, st);
   else
-    apply_launch<float>(A, L, st);
+    apply_launch<float, float>(A, L, st);
   return hipGetLastError();
 }
 

@@ -108,6 +108,10 @@ struct hh_model {
   PBuf h_tfirst_c, h_tfirst_r, h_cfirst_c, h_cfirst_r, h_afirst_i, h_afirst_j;
   // persistent far-field workspace
   int far_mode = 1;  // 1: batched iterations (default), 0: persistent kernel
+  // 16-bit modes: ACA factor pools stored in BF16 (arithmetic in FP32,
+  // norms and sums in FP64); HH_FAR_C16=0 / HH_FAR_Q16=0 keep the C / Q pool
+  // in FP32
+  bool far_c16 = true, far_q16 = true;
   bool verbose = false;  // HH_VERBOSE=1: far-field progress lines on stderr
   int aca_chunk = kAcaChunk;  // HH_ACA_CHUNK: items per ACA step chunk (1..kAcaChunk)
   DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
@@ -432,6 +436,8 @@ struct hh_model {
     al.use_symmetry = use_symmetry;
     al.nlist = nl;
     al.pool_f64 = f64;
+    al.pool_c16 = !f64 && far_c16;
+    al.pool_q16 = !f64 && far_q16;
     al.C = Cpool.p;
     al.Q = Qpool.p;
     al.first_i = a_first_i.as<int32_t>();
@@ -463,12 +469,13 @@ struct hh_model {
     const int64_t nfar = (int64_t)far.size();
     if (nfar == 0) return;
     const bool f64 = prec == PREC_FP32;
-    const size_t esz = (f64 ? 8 : 4) * 9;
+    const bool c16 = !f64 && far_c16, q16 = !f64 && far_q16;
+    const size_t escc = f64 ? 72 : (c16 ? 18 : 36), escq = f64 ? 72 : (q16 ? 18 : 36);
     const int MI = max_iter;
-    // batch budget: C and Q pools, kmax (m + n) entries of esz bytes
+    // batch budget: C and Q pools, kmax m and kmax n entries
     auto block_bytes = [&](const FarBlock &f) {
       const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
-      return kmax * (int64_t)(f.m + f.n) * (int64_t)esz;
+      return kmax * ((int64_t)f.m * (int64_t)escc + (int64_t)f.n * (int64_t)escq);
     };
     int64_t b0 = 0;
     while (b0 < nfar) {
@@ -493,8 +500,8 @@ struct hh_model {
         co += kmax * hb[(size_t)b].m;
         qo += kmax * hb[(size_t)b].n;
       }
-      Cpool.reserve((size_t)std::max<int64_t>(co, 1) * esz);
-      Qpool.reserve((size_t)std::max<int64_t>(qo, 1) * esz);
+      Cpool.reserve((size_t)std::max<int64_t>(co, 1) * escc);
+      Qpool.reserve((size_t)std::max<int64_t>(qo, 1) * escq);
       Y.reserve((size_t)std::max(ysum_c, ysum_r) * 9 * 4);
       upload(blk, hb.data(), hb.size() * sizeof(FarBlock));
       std::vector<int32_t> zeros_it((size_t)nb, 0), jp((size_t)nb * (MI + 1), 0);
@@ -619,6 +626,8 @@ struct hh_model {
         sl.nchunks_c = ncc;
         sl.nchunks_r = ncr;
         sl.pool_f64 = f64;
+        sl.pool_c16 = c16;
+        sl.pool_q16 = q16;
         sl.C = Cpool.p;
         sl.Q = Qpool.p;
         // column pass, residual, row pivot, normalisation
@@ -884,6 +893,8 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
     }
     if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
+    if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
+    if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
     m->aca_chunk = chunk;
     if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
