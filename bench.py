@@ -3,7 +3,7 @@
 Workload (default): 1,000,000 spheres of radius 1 in 200 Gaussian blobs
 (sigma = 15), blob centres uniform in the cube sized for a 10% mean volume
 fraction, minimum centre distance 2.05, seed 2; gravity forcing (0,0,-1);
-leaf 256, ACA epsilon 1e-4, at most 44 ACA iterations, far blocks of at most
+leaf 256, ACA epsilon 1e-4, at most 40 ACA iterations, far blocks of at most
 2048 particles (max_relative_coord); pair-function weights: seeded Xavier
 initialisation of the 3-128-512-9 MLP.
 
@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--check-targets", type=int, default=2000)
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--max-iter", type=int, default=44)
+    ap.add_argument("--max-iter", type=int, default=40)
     ap.add_argument("--max-relative-coord", type=int, default=2048)
     ap.add_argument("--epsilon", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")

@@ -852,13 +852,22 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
     for (int e = 0; e < 9; e++) c[s][e] = raw[e] - c[s][e];
   }
   SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  // 16-bit C pools: the un-normalised residual column stays in FP32, in place
+  // of the pass's raw entries, so the pivot inverse and the normalisation see
+  // it unrounded; only the normalised factor is stored in 16 bit
+  float *Rk = const_cast<float *>(A.Y) + 9 * A.ybase_c[ch.slot];
   ArgMax best{kNegHuge, -1};
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     const int i = ch.off + threadIdx.x + 64 * s;
     if (!valid[s]) continue;
 #pragma unroll
-    for (int e = 0; e < 9; e++) Ck[9 * (int64_t)i + e] = (SC)c[s][e];
+    for (int e = 0; e < 9; e++) {
+      if constexpr (sizeof(SC) == 2)
+        Rk[9 * (int64_t)i + e] = (float)c[s][e];
+      else
+        Ck[9 * (int64_t)i + e] = c[s][e];
+    }
     bool u = false;
     for (int l = 0; l < k; l++) u |= (used[l] == i);
     const double dt = (double)det3(c[s]);
@@ -886,23 +895,38 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
   if (threadIdx.x == 0) {
     const int ik = best.idx < 0 ? 0 : best.idx;
     A.ipiv[(size_t)b * A.max_iter + k] = ik;
-    const SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
     double m[9];
+    if constexpr (sizeof(SC) == 2) {  // the FP32 residual column (see the column pass)
+      const float *Rk = A.Y + 9 * (A.ybase_c[slot] + (int64_t)ik);
 #pragma unroll
-    for (int e = 0; e < 9; e++) m[e] = (double)Ck[e];
+      for (int e = 0; e < 9; e++) m[e] = (double)Rk[e];
+    } else {
+      const SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
+#pragma unroll
+      for (int e = 0; e < 9; e++) m[e] = (double)Ck[e];
+    }
     const double a = m[0], bb = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6],
                  h = m[7], ii = m[8];
     const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
+    double iv[9];
+    iv[0] = (e * ii - f * h) / det;
+    iv[1] = -(bb * ii - c * h) / det;
+    iv[2] = (bb * f - c * e) / det;
+    iv[3] = -(d * ii - f * g) / det;
+    iv[4] = (a * ii - c * g) / det;
+    iv[5] = -(a * f - c * d) / det;
+    iv[6] = (d * h - e * g) / det;
+    iv[7] = -(a * h - bb * g) / det;
+    iv[8] = (a * e - bb * d) / det;
+    // An exhausted residual can leave an exactly singular pivot block (more
+    // likely with 16-bit factor storage); its term is then set to zero, the
+    // norm test stops the block, and no inf/NaN enters the pools.
+    bool finite = det != 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) finite = finite && isfinite(iv[q]);
     double *inv = A.inv + 9 * (size_t)slot;
-    inv[0] = (e * ii - f * h) / det;
-    inv[1] = -(bb * ii - c * h) / det;
-    inv[2] = (bb * f - c * e) / det;
-    inv[3] = -(d * ii - f * g) / det;
-    inv[4] = (a * ii - c * g) / det;
-    inv[5] = -(a * f - c * d) / det;
-    inv[6] = (d * h - e * g) / det;
-    inv[7] = -(a * h - bb * g) / det;
-    inv[8] = (a * e - bb * d) / det;
+#pragma unroll
+    for (int q = 0; q < 9; q++) inv[q] = finite ? iv[q] : 0.0;
   }
 }
 
@@ -925,7 +949,10 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp)
     const int i = ch.off + threadIdx.x + 64 * s;
     if (i >= iend) continue;
     P c[9], o[9];
-    load9(Ck + 9 * (int64_t)i, c);
+    if constexpr (sizeof(SC) == 2)
+      load9(A.Y + 9 * (A.ybase_c[ch.slot] + (int64_t)i), c);
+    else
+      load9(Ck + 9 * (int64_t)i, c);
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
