@@ -229,3 +229,23 @@ def test_device_euler_with_three_body_matches_host(xavier_w):
     print("euler + three-body max |x_dev - x_host|:", np.abs(xd - xh).max())
     assert np.abs(xh - x).max() > 1e-4
     assert close(xd, xh)
+
+
+def test_forcing_edge_cases(xavier_w):
+    h = hignn()
+    x = suspension(8)
+    fc = h.DeviceForcing()
+    assert not fc.eval(x).any()  # no term set: zero force
+    fc.set_filaments(100, 40, 2.0, 1.0, 1.0)  # 4000 beads > N
+    with pytest.raises(RuntimeError, match="filaments"):
+        fc.eval(x)
+    fc.set_filaments(0, 0, 0.0, 0.0, 0.0)
+    fc.set_two_body_epsilon(5.0)
+    fc.set_domain(np.array([[0, 0, 0], [8, 100, 100]], f32))  # 8 < 2 x 5
+    with pytest.raises(RuntimeError, match="twice the radius"):
+        fc.eval(x)
+    # integrator and model of different sizes
+    it = h.ExplicitEuler()
+    it.set_num_rigid_body(x.shape[0] + 1)
+    with pytest.raises(RuntimeError, match="sizes differ"):
+        it.set_device_velocity(model(h, x, xavier_w), h.DeviceForcing())

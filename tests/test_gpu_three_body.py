@@ -131,3 +131,49 @@ def test_three_body_batches_and_repeat(case):
     assert rel_l2(a[sub], want[sub]) <= 5e-3
     c = model(h, "fp32").velocity(x, f)
     assert rel_l2(c[sub], want[sub]) <= 1e-5
+
+
+def test_three_body_edge_cases():
+    h = hignn()
+    x = lattice(4)
+    f = np.ones_like(x)
+    tb = h.ThreeBodyModel()
+    with pytest.raises(RuntimeError, match="weights"):
+        tb.velocity(x, f)
+    tb.load_weights(W.xavier(1, W.net_shapes(6)), W.xavier(2, W.net_shapes(3)))
+    with pytest.raises(RuntimeError, match="radius"):
+        tb.velocity(x, f)
+    with pytest.raises(RuntimeError, match="floats"):
+        tb.load_weights(W.xavier(1, W.net_shapes(3)), W.xavier(2, W.net_shapes(3)))
+    # a radius below the lattice spacing: no edges, zero velocity
+    tb.set_three_body_epsilon(1.0)
+    v = tb.velocity(x, f)
+    assert tb.last_profile()["three_body_edges"] == 0 and tb.last_profile()["self_edges"] == 0
+    assert not v.any()
+    # a radius that only reaches the face neighbours: self edges but no
+    # triplet through a corner (every neighbour of a neighbour except i)
+    tb.set_three_body_epsilon(3.5)
+    tb.velocity(x, f)
+    lists, _ = T.neighbour_lists(x, 3.5)
+    trip, self_e = T.three_body_edges(lists)
+    prof = tb.last_profile()
+    assert prof["three_body_edges"] == trip.shape[1] and prof["self_edges"] == self_e.shape[1]
+
+
+def test_neighbour_lists_state_errors():
+    h = hignn()
+    nl = h.NeighborLists()
+    with pytest.raises(RuntimeError, match="update_coord"):
+        nl.set_two_body_epsilon(1.0)
+        nl.build_two_body_info()
+    nl.update_coord(lattice(3))
+    # before build_three_body_info: empty edge arrays, as the reference's
+    # default-constructed views give
+    assert nl.get_three_body_edge_info().shape == (3, 0)
+    assert nl.get_three_body_edge_self_info().shape == (2, 0)
+    with pytest.raises(RuntimeError, match="radius"):
+        nl.build_three_body_info()
+    nl.set_periodic(True)
+    nl.set_three_body_epsilon(2.0)
+    with pytest.raises(RuntimeError, match="domain"):
+        nl.build_three_body_info()
