@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
 
 #include "hh_forcing.h"
 #include "hh_host.h"
@@ -51,6 +52,11 @@ struct CellBufs {
     L.sidx = sidx.as<int32_t>();
     L.spos = spos.as<float4>();
     L.scell = scell.as<int4>();
+  }
+
+  std::array<void *, 11> pointers() const {
+    return {src.p, src_tgt.p, src_cell.p, cnt.p, off.p, bcnt.p, bstart.p, bsum.p, sidx.p, spos.p,
+            scell.p};
   }
 
   void release() {

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 
@@ -54,7 +55,9 @@ struct hh_forcing {
     L.eps = eps;
     L.margin = eps;
     if (!L.morse || n <= 0) return L;
+    const std::array<void *, 11> before = cells.pointers();
     cells.setup(L, n);
+    if (cells.pointers() != before) version++;  // moved buffers invalidate captured graphs
     return L;
   }
 

@@ -249,3 +249,32 @@ def test_forcing_edge_cases(xavier_w):
     it.set_num_rigid_body(x.shape[0] + 1)
     with pytest.raises(RuntimeError, match="sizes differ"):
         it.set_device_velocity(model(h, x, xavier_w), h.DeviceForcing())
+
+
+def test_stepper_recaptures_after_forcing_buffers_move(xavier_w):
+    # the stage graphs hold the forcing's cell-list buffers; an evaluation of a
+    # larger set through the same forcing moves them, and the stepper must
+    # capture its graphs again before the next run
+    h = hignn()
+    x = suspension(9, n_fil=2, n_chain=20, n_free=160)
+    m = model(h, x, xavier_w)
+    fc = h.DeviceForcing()
+    fc.set_body_force([0.0, 0.0, -1.0])
+    fc.set_filaments(2, 20, FIL["rest"], FIL["k_t"], FIL["k_b"])
+    fc.set_two_body_epsilon(5.0)
+    it = h.ExplicitEuler()
+    it.set_time_step(0.02)
+    it.set_final_time(0.04)
+    it.set_num_rigid_body(x.shape[0])
+    it.set_device_velocity(m, fc)
+    state = np.zeros((x.shape[0], 6), f32)
+    state[:, :3] = x
+
+    def run():
+        it.initialize(state)
+        it.run()
+        return it.positions()
+
+    first = run()
+    fc.eval(suspension(10, n_fil=2, n_chain=20, n_free=4000))  # grows the cell list
+    assert close(run(), first, 1e-6)
