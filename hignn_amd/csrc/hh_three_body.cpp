@@ -63,6 +63,15 @@ struct Lists {
     tcnt.reserve(4 * (size_t)std::max<int64_t>(n, 1));
     toff.reserve(4 * (size_t)(n + 1));
     HIPCHK(launch_triplet_count(off.as<uint32_t>(), idx.as<int32_t>(), n, tcnt.as<uint32_t>(), st));
+    {  // offsets are 32-bit: refuse lists that would wrap
+      std::vector<uint32_t> c((size_t)n);
+      HIPCHK(hipMemcpyAsync(c.data(), tcnt.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      uint64_t tot = 0;
+      for (uint32_t v : c) tot += v;
+      if (tot > 0xffffffffull)
+        throw HHError(HH_ERR_ARG, "more than 2^32 three-body edges: use a smaller radius or fewer particles");
+    }
     HIPCHK(exclusive_scan(tcnt.as<uint32_t>(), n, toff.as<uint32_t>(), cells.bsum.as<uint32_t>(), st));
     uint32_t total = 0;
     HIPCHK(hipMemcpyAsync(&total, toff.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));
