@@ -50,10 +50,14 @@ struct Lists {
     off.reserve(4 * (size_t)(n + 1));
     HIPCHK(launch_neighbour_count(L, cnt.as<uint32_t>(), off.as<uint32_t>(), cells.bsum.as<uint32_t>(),
                                   st));
-    uint32_t total = 0;
-    HIPCHK(hipMemcpyAsync(&total, off.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> c((size_t)n);
+    HIPCHK(hipMemcpyAsync(c.data(), cnt.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    nnb = total;
+    uint64_t total = 0;
+    for (uint32_t v : c) total += v;
+    if (total > 0x7fffffffull)  // offsets are 32-bit, self edges index with int32
+      throw HHError(HH_ERR_ARG, "more than 2^31 neighbour pairs: use a smaller radius or fewer particles");
+    nnb = (int64_t)total;
     idx.reserve(4 * (size_t)std::max<int64_t>(nnb, 1));
     HIPCHK(launch_neighbour_fill(L, off.as<uint32_t>(), idx.as<int32_t>(), st));
   }
