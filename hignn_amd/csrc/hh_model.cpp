@@ -107,7 +107,10 @@ struct hh_model {
   DBuf a_first_i, a_first_j, w_i, w_j;
   PBuf h_tfirst_c, h_tfirst_r, h_cfirst_c, h_cfirst_r, h_afirst_i, h_afirst_j;
   // persistent far-field workspace
-  int far_mode = 1;  // 1: batched iterations (default), 0: persistent kernel
+  // far field: 1 batched iterations, 0 persistent kernel, -1 (default) the
+  // persistent kernel when there are no more far blocks than compute units
+  // (one workgroup per block, no per-iteration launches), batched otherwise
+  int far_mode = -1;
   // 16-bit modes: ACA factor pools stored in BF16 (arithmetic in FP32,
   // norms and sums in FP64); HH_FAR_C16=0 / HH_FAR_Q16=0 keep the C / Q pool
   // in FP32
@@ -462,7 +465,8 @@ struct hh_model {
   }
 
   void far_field(hipStream_t st) {
-    if (far_mode == 0) {
+    const int mode = far_mode >= 0 ? far_mode : ((int64_t)far.size() <= (int64_t)g_cus ? 0 : 1);
+    if (mode == 0) {
       far_field_persistent(st);
       return;
     }

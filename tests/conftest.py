@@ -24,3 +24,12 @@ def gpu_available():
 def xavier_w():
     from hignn_amd import weights
     return weights.xavier(0)
+
+
+@pytest.fixture(autouse=True, scope="session")
+def _batched_far_field_by_default():
+    """The library picks the persistent far-field kernel for systems with few
+    far blocks; the GPU tests cover the batched path unless a test sets
+    HH_FAR_MODE itself (test_far_mode_auto tests the automatic choice)."""
+    os.environ.setdefault("HH_FAR_MODE", "1")
+    yield

@@ -241,3 +241,26 @@ def test_tree_reuse_keeps_lists_and_tracks_positions(xavier_w):
     assert err <= 1e-3 and moved > 0.0
     m.update_coord(x0)
     np.testing.assert_allclose(m.dot(f), u0, rtol=1e-5, atol=1e-6 * np.abs(u0).max())
+
+
+def test_far_mode_auto(model_small, xavier_w, monkeypatch):
+    # unset: the persistent kernel when far blocks <= compute units (1200
+    # particles: a few hundred blocks at most), same result as both forced modes
+    x, f, om = model_small["x"], model_small["f"], model_small["om"]
+    ref = om.dot(xavier_w, f)
+    out = {}
+    for mode in (None, "0", "1"):
+        if mode is None:
+            monkeypatch.delenv("HH_FAR_MODE", raising=False)
+        else:
+            monkeypatch.setenv("HH_FAR_MODE", mode)
+        m = abi.Model(x, 64)
+        m.load_weights(xavier_w)
+        m.set(epsilon=1e-5, max_iter=50)
+        m.update_coord()
+        out[mode] = m.dot(f)
+        assert O.rel_l2(out[mode], ref) <= 1e-5, mode
+    nfar = len(abi.tree_build(x, 64)["far"])  # 256 compute units on MI355X
+    expect = "0" if nfar <= 256 else "1"
+    print("far blocks:", nfar, "-> mode", expect)
+    assert O.rel_l2(out[None], out[expect]) <= 1e-7
