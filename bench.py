@@ -226,8 +226,10 @@ def stepper_bench(args):
             el = time.perf_counter() - t0
             res.append((it.func_count(), el))
         cnt, el = res[1]
-        log(f"[stepper] {mode}: {cnt} steps in {el:.3f}s ({1e3 * el / cnt:.2f} ms/step)")
-        return cnt, el, it.positions()
+        whole = it.device_stats()["whole_stage_graphs"] if mode != "callback" else 0
+        log(f"[stepper] {mode}: {cnt} steps in {el:.3f}s ({1e3 * el / cnt:.2f} ms/step, "
+            f"{whole} whole-stage graph launches)")
+        return cnt, el, it.positions(), whole
 
     out = {}
     for mode in ("callback", "device_graphs", "device_eager", "device_reuse"):
@@ -256,6 +258,7 @@ def stepper_bench(args):
             "callback_steps_per_s": round(rate["callback"], 3),
             "device_eager_steps_per_s": round(rate["device_eager"], 3),
             "device_tree_reuse_steps_per_s": round(rate["device_reuse"], 3),
+            "device_tree_reuse_whole_stage_graph_launches": out["device_reuse"][3],
             "speedup_vs_callback": round(rate["device_graphs"] / rate["callback"], 3),
             "max_abs_position_diff_vs_callback": diff,
         },

@@ -83,6 +83,12 @@ struct hh_stepper {
   // graphs: [0] Euler input + forcing, [1..6] DOPRI stages + forcing
   hipGraphExec_t g[7] = {};
   int g_version = -1;
+  // whole-stage graphs (stage, forcing, coordinate gather and matvec) while
+  // the model's matvec is capturable; valid for one model signature
+  hipGraphExec_t fg[7] = {};
+  int fg_version = -1;
+  uint64_t fg_sig = 0;
+  int64_t full_launches = 0;
   double t = 0;
   int64_t count = 0, evals = 0, rejected = 0;
   int out_step = 1;
@@ -105,6 +111,16 @@ struct hh_stepper {
         e = nullptr;
       }
     g_version = -1;
+    drop_full();
+  }
+
+  void drop_full() {
+    for (auto &e : fg)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+    fg_version = -1;
   }
 
   // Stage descriptor: row 0 = Euler input (x0 = X), rows 1..6 = DOPRI stages.
@@ -140,41 +156,75 @@ struct hh_stepper {
     HIPCHK(launch_forcing(L, st));
   }
 
+  template <class Enqueue>
+  void capture(hipGraphExec_t &exec, Enqueue &&enqueue) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+  }
+
   void run_prep(int row) {
     if (!use_graphs) {
       enqueue_prep(row);
       return;
     }
+    // reserve every forcing buffer before capture (no allocation inside)
+    (void)fc->prepare(XIN.as<float>(), F.as<float>(), n);
     if (g_version != fc->version) {
       drop_graphs();
       g_version = fc->version;
     }
-    if (!g[row]) {
-      // reserve every forcing buffer before capture (no allocation inside)
-      (void)fc->prepare(XIN.as<float>(), F.as<float>(), n);
-      hipGraph_t graph = nullptr;
-      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      try {
-        enqueue_prep(row);
-      } catch (...) {
-        (void)hipStreamEndCapture(st, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        throw;
-      }
-      HIPCHK(hipStreamEndCapture(st, &graph));
-      hipError_t e = hipGraphInstantiate(&g[row], graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HIPCHK(e);
-    }
+    if (!g[row]) capture(g[row], [&]() { enqueue_prep(row); });
     HIPCHK(hipGraphLaunch(g[row], st));
+  }
+
+  void model_call(int rc) {
+    if (rc != HH_OK) throw HHError(HH_ERR_STATE, std::string("model: ") + hh_last_error());
+  }
+
+  // The whole stage as one graph when the matvec allows it; false otherwise.
+  bool run_full(int row, DBuf &out) {
+    if (!use_graphs || tb) return false;  // the three-body term waits on the host
+    int cap = 0;
+    uint64_t sig = 0;
+    model_call(hh_model_dot_capturable(m, &cap, &sig));
+    if (!cap) return false;
+    (void)fc->prepare(XIN.as<float>(), F.as<float>(), n);
+    if (fg_version != fc->version || fg_sig != sig) {
+      drop_full();
+      fg_version = fc->version;
+      fg_sig = sig;
+    }
+    if (!fg[row])
+      capture(fg[row], [&]() {
+        enqueue_prep(row);
+        model_call(hh_model_update_coord_device(m, XIN.as<float>(), n, st));
+        model_call(hh_model_dot_enqueue(m, out.as<float>(), F.as<float>(), n, st));
+      });
+    HIPCHK(hipGraphLaunch(fg[row], st));
+    full_launches++;
+    return true;
   }
 
   // Velocity of a stage: forcing at the stage input, update_coord, u = M f.
   void velocity(int row, DBuf &out) {
+    if (run_full(row, out)) {
+      evals++;
+      return;
+    }
     run_prep(row);
-    if (hh_model_update_coord_device(m, XIN.as<float>(), n, st) != HH_OK ||
-        hh_model_dot_device(m, out.as<float>(), F.as<float>(), n, st) != HH_OK)
-      throw HHError(HH_ERR_STATE, std::string("model: ") + hh_last_error());
+    model_call(hh_model_update_coord_device(m, XIN.as<float>(), n, st));
+    model_call(hh_model_dot_device(m, out.as<float>(), F.as<float>(), n, st));
     if (tb && hh_three_body_velocity_device(tb, XIN.as<float>(), F.as<float>(), n, out.as<float>(), 1,
                                             st) != HH_OK)
       throw HHError(HH_ERR_STATE, std::string("three-body: ") + hh_last_error());
@@ -194,9 +244,7 @@ struct hh_stepper {
     while (t < tf - 1e-5 && (max_steps <= 0 || taken < max_steps)) {
       const double h = std::min(h0, tf - t);
       velocity(0, V);
-      float *fdt = hcoef.as<float>() + 7 * 8;
-      fdt[0] = (float)h;
-      HIPCHK(hipMemcpyAsync(coef.as<float>() + 7 * 8, fdt, 4, hipMemcpyHostToDevice, st));
+      HIPCHK(launch_set_f32(coef.as<float>() + 7 * 8, (float)h, st));
       HIPCHK(launch_euler_update(X.as<float>(), V.as<float>(), OFF.as<float>(), n,
                                  coef.as<float>() + 7 * 8, periodic ? 1 : 0, lim, st));
       t += h;
@@ -448,7 +496,7 @@ int hh_stepper_initialize(hh_stepper *s, const float *x, int64_t n) {
     for (DBuf &k : s->K) HIPCHK(hipMemsetAsync(k.p, 0, b, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     s->t = 0;
-    s->count = s->evals = s->rejected = 0;
+    s->count = s->evals = s->rejected = s->full_launches = 0;
     s->started = false;
     s->initialized = true;
   });
@@ -473,7 +521,7 @@ int hh_stepper_run(hh_stepper *s, int64_t max_steps) {
   });
 }
 
-int hh_stepper_state(hh_stepper *s, float *x, float *v, double *t, int64_t counts[3]) {
+int hh_stepper_state(hh_stepper *s, float *x, float *v, double *t, int64_t counts[4]) {
   return guarded([&]() {
     if (!need(s)->initialized) throw HHError(HH_ERR_STATE, "initialize has not run");
     HIPCHK(hipStreamSynchronize(s->st));
@@ -485,6 +533,7 @@ int hh_stepper_state(hh_stepper *s, float *x, float *v, double *t, int64_t count
       counts[0] = s->evals;
       counts[1] = s->count;
       counts[2] = s->rejected;
+      counts[3] = s->full_launches;
     }
   });
 }

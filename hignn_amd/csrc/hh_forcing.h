@@ -82,6 +82,8 @@ hipError_t launch_stage(const StageLaunch &S, hipStream_t st);
 hipError_t launch_euler_update(float *x, const float *v, float *off, int64_t n, const float *fdt,
                                int periodic, Lim lim, hipStream_t st);
 hipError_t launch_add(float *x, const float *off, int64_t n3, hipStream_t st);
+// *p = v in stream order (no pinned staging buffer left in flight).
+hipError_t launch_set_f32(float *p, float v, hipStream_t st);
 int64_t err_partials_needed(int64_t n);
 hipError_t launch_dopri_err(const ErrLaunch &E, double *out, hipStream_t st);
 

@@ -575,6 +575,15 @@ hipError_t launch_add(float *x, const float *off, int64_t n3, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ void set_f32_kernel(float *p, float v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+hipError_t launch_set_f32(float *p, float v, hipStream_t st) {
+  hipLaunchKernelGGL(set_f32_kernel, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
+}
+
 int64_t err_partials_needed(int64_t n) { return (n + kThreads - 1) / kThreads; }
 
 hipError_t launch_dopri_err(const ErrLaunch &E, double *out, hipStream_t st) {

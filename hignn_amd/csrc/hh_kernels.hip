@@ -1877,6 +1877,17 @@ hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ void zero_f64_kernel(double *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_forces(const float *f, int64_t n, const int32_t *perm, float4 *fp,
                                 hipStream_t st) {
   if (n <= 0) return hipSuccess;

@@ -78,6 +78,7 @@ struct hh_model {
   int64_t max_rel_coord = 500000;
   bool post_check = false, use_symmetry = true;
   bool tree_reuse = false;  // update_coord keeps the last tree and lists (opt-in)
+  uint64_t builds = 0, loads = 0;  // tree builds and weight loads (graph signatures)
   int prec = PREC_FP32;
   // byte budget of one far-field batch (factor pools + Gram histories); 0:
   // half the device memory free at the first far field, at most 192 GiB
@@ -258,6 +259,7 @@ struct hh_model {
     prep_h(wh_bf16, to_bf16);
     prep_h(wh_f16, to_f16);
     has_weights = true;
+    loads++;
   }
 
   DevWeightsF devw_f(bool f64) const {
@@ -285,6 +287,7 @@ struct hh_model {
       return;
     }
     build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
+    builds++;
     // positions in tree order
     std::vector<float> xph(4 * (size_t)n);
     std::vector<int32_t> ph((size_t)n);
@@ -465,8 +468,7 @@ struct hh_model {
   }
 
   void far_field(hipStream_t st) {
-    const int mode = far_mode >= 0 ? far_mode : ((int64_t)far.size() <= (int64_t)g_cus ? 0 : 1);
-    if (mode == 0) {
+    if (resolved_far_mode() == 0) {
       far_field_persistent(st);
       return;
     }
@@ -706,6 +708,51 @@ struct hh_model {
       for (auto &e : ev) HIPCHK(hipEventCreate(&e));
       for (auto &e : evm) HIPCHK(hipEventCreate(&e));
     }
+  }
+
+  int resolved_far_mode() const {
+    return far_mode >= 0 ? far_mode : ((int64_t)far.size() <= (int64_t)g_cus ? 0 : 1);
+  }
+
+  // The matvec can be enqueued inside a stream capture: a kept tree (no host
+  // build), the far field on the persistent kernel (no host reads per ACA
+  // iteration) and its buffers set up by an earlier dot.
+  bool capturable() const {
+    if (!built || !tree_reuse || !has_weights || resolved_far_mode() != 0 || !ev[0]) return false;
+    if (fp.bytes < 16 * (size_t)n || u.bytes < 24 * (size_t)n) return false;
+    return far.empty() || (p_ready && p_maxiter == max_iter && p_f64 == (int)(prec == PREC_FP32));
+  }
+
+  // Changes whenever something a captured matvec reads or launches with does.
+  uint64_t graph_signature() const {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+      h ^= v;
+      h *= 1099511628211ull;
+    };
+    uint64_t e;
+    std::memcpy(&e, &epsilon, 8);
+    for (uint64_t v : {(uint64_t)prec, (uint64_t)max_iter, e, (uint64_t)use_symmetry,
+                       (uint64_t)n_near_tasks, (uint64_t)p_nslots, (uint64_t)p_slotC,
+                       (uint64_t)p_slotQ, (uint64_t)p_slotY, (uint64_t)far.size(), builds, loads})
+      mix(v);
+    for (const DBuf *b : {&xp, &perm, &fp, &u, &near_blocks, &near_tasks, &p_blk, &p_order, &p_rank,
+                          &p_poolC, &p_poolQ, &p_Y})
+      mix((uint64_t)(uintptr_t)b->p);
+    for (const DBuf *arr : {wf32, wf64}) for (int k = 0; k < 5; k++) mix((uint64_t)(uintptr_t)arr[k].p);
+    for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)wh_f[k].p);
+    for (int k = 0; k < 2; k++) mix((uint64_t)(uintptr_t)wh_bf16[k].p), mix((uint64_t)(uintptr_t)wh_f16[k].p);
+    return h;
+  }
+
+  // u = M(x) f without host synchronisation or timing (capturable state only).
+  void enqueue_dot(float *u_dev, const float *f_dev, hipStream_t st) {
+    if (!capturable()) throw HHError(HH_ERR_STATE, "matvec not capturable in the current state");
+    HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
+    HIPCHK(launch_zero_f64(u.as<double>(), 3 * n, st));
+    near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+    far_field_persistent(st);
+    HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
   }
 
   void dot_device(float *u_dev, const float *f_dev, hipStream_t st) {
@@ -997,6 +1044,22 @@ int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n,
     HIPCHK(hipMemcpyAsync(m->x.data(), coord_dev, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     m->update();
+  });
+}
+
+int hh_model_dot_capturable(hh_model *m, int *capturable, uint64_t *signature) {
+  return guarded([&]() {
+    if (!m || !capturable || !signature) throw HHError(HH_ERR_ARG, "NULL argument");
+    *capturable = m->capturable() ? 1 : 0;
+    *signature = m->graph_signature();
+  });
+}
+
+int hh_model_dot_enqueue(hh_model *m, float *u_dev, const float *f_dev, int64_t n, void *stream) {
+  return guarded([&]() {
+    if (!m || !u_dev || !f_dev) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    m->enqueue_dot(u_dev, f_dev, stream ? (hipStream_t)stream : m->stream);
   });
 }
 

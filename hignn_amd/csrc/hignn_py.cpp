@@ -462,6 +462,7 @@ class Integrator {
     d["velocity_evaluations"] = stats_[0];
     d["accepted"] = stats_[1];
     d["rejected"] = stats_[2];
+    d["whole_stage_graphs"] = stats_[3];
     return d;
   }
 
@@ -529,7 +530,7 @@ class Integrator {
   ThreeBody *dt3_ = nullptr;
   hh_stepper *stepper_ = nullptr;
   bool graphs_ = true;
-  int64_t stats_[3] = {0, 0, 0};
+  int64_t stats_[4] = {0, 0, 0, 0};
 
   bool device_mode() const { return dm_ != nullptr; }
 

@@ -65,6 +65,21 @@ int hh_forcing_eval(hh_forcing *fc, const float *x, float *f, int64_t n);
  * device; otherwise the coordinates are copied to the host for the tree build. */
 int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n, void *stream);
 
+/* Whether hh_model_dot_enqueue may run now (*capturable = 1): tree reuse on,
+ * a built tree, the far field on the persistent kernel and its buffers set up
+ * by an earlier dot at the current precision and iteration cap.  *signature
+ * changes whenever anything such a matvec launches with changes (buffers,
+ * tree, weights, settings); a graph captured with one signature is valid
+ * while it holds.  Same role as Dot (src/HignnModel.cpp:765-836) split into a
+ * state query and a host-wait-free enqueue; no reference counterpart of its
+ * own. */
+int hh_model_dot_capturable(hh_model *m, int *capturable, uint64_t *signature);
+
+/* u = M(x) f for device arrays (n,3), enqueued on stream with no host wait,
+ * timing or allocation, so that it can be captured into a HIP graph.  Fails
+ * with HH_ERR_STATE unless hh_model_dot_capturable reports 1. */
+int hh_model_dot_enqueue(hh_model *m, float *u_dev, const float *f_dev, int64_t n, void *stream);
+
 typedef struct hh_stepper hh_stepper;
 
 #define HH_STEP_EULER 0 /* ExplicitEuler::Run, include/TimeIntegrator.hpp:431-488 */
@@ -109,8 +124,9 @@ int hh_stepper_set_output(hh_stepper *s, int output_step, hh_output_fn fn, void 
 int hh_stepper_run(hh_stepper *s, int64_t max_steps);
 
 /* Host copies of the positions and of the last velocity (either may be NULL);
- * t: current time; counts: [velocity evaluations, accepted, rejected steps]. */
-int hh_stepper_state(hh_stepper *s, float *x, float *v, double *t, int64_t counts[3]);
+ * t: current time; counts: [velocity evaluations, accepted, rejected steps,
+ * evaluations run as one whole-stage graph (forcing and matvec together)]. */
+int hh_stepper_state(hh_stepper *s, float *x, float *v, double *t, int64_t counts[4]);
 
 #ifdef __cplusplus
 }

@@ -107,7 +107,7 @@ def model(h, x, w, reuse=False):
 
 
 def integrate(h, cls, x, w, device, dt=0.02, tf=0.1, thr=None, lims=None, reuse=False,
-              graphs=True, morse=5.0):
+              graphs=True, morse=5.0, stats=None):
     m = model(h, x, w, reuse)
     fc = forcing(h, morse=morse)
     it = cls()
@@ -138,6 +138,8 @@ def integrate(h, cls, x, w, device, dt=0.02, tf=0.1, thr=None, lims=None, reuse=
     state[:, :3] = x
     it.initialize(state)
     it.run()
+    if stats is not None:
+        stats.append(it.device_stats())
     return it.positions(), it.func_count(), outs
 
 
@@ -278,3 +280,68 @@ def test_stepper_recaptures_after_forcing_buffers_move(xavier_w):
     first = run()
     fc.eval(suspension(10, n_fil=2, n_chain=20, n_free=4000))  # grows the cell list
     assert close(run(), first, 1e-6)
+
+
+@pytest.mark.parametrize("method", ["euler", "dopri"])
+def test_whole_stage_graphs_match_eager(xavier_w, monkeypatch, method):
+    # tree reuse and the persistent far-field kernel: after the first (eager)
+    # evaluation every stage (stage input, forcing, coordinate gather, matvec)
+    # replays as one graph.  Every step's positions are checked against the
+    # eager stepper; the output callback synchronises with the host between
+    # replays.
+    monkeypatch.setenv("HH_FAR_MODE", "0")
+    h = hignn()
+    x = suspension(11)
+    cls, kw = ((h.ExplicitEuler, dict(dt=0.02, tf=0.12)) if method == "euler" else
+               (h.ExplicitRk4, dict(dt=0.002, tf=0.006, thr=1e-2)))
+    sg, se = [], []
+    xg, cg, og = integrate(h, cls, x, xavier_w, device=True, reuse=True, graphs=True, stats=sg, **kw)
+    xe, ce, oe = integrate(h, cls, x, xavier_w, device=True, reuse=True, graphs=False, stats=se,
+                           **kw)
+    print(method, "stats graphs:", sg[0], "eager:", se[0])
+    assert cg == ce and len(og) == len(oe) >= 3
+    for a, b in zip(og, oe):
+        assert close(a, b, 1e-6)
+    assert close(xg, xe, 1e-6)
+    assert np.abs(xg - x).max() > 1e-3
+    assert sg[0]["whole_stage_graphs"] == sg[0]["velocity_evaluations"] - 1
+    assert se[0]["whole_stage_graphs"] == 0
+
+
+def test_whole_stage_graphs_recapture_after_model_settings(xavier_w, monkeypatch):
+    # a new ACA tolerance and iteration cap change what the captured matvec
+    # launches with: the stepper captures again and still matches the eager one
+    monkeypatch.setenv("HH_FAR_MODE", "0")
+    h = hignn()
+    x = suspension(12)
+    state = np.zeros((x.shape[0], 6), f32)
+    state[:, :3] = x
+
+    def make(graphs):
+        m = model(h, x, xavier_w, reuse=True)
+        it = h.ExplicitEuler()
+        it.set_time_step(0.02)
+        it.set_final_time(0.08)
+        it.set_num_rigid_body(x.shape[0])
+        it.set_device_velocity(m, forcing(h))
+        it.set_graphs(graphs)
+        return m, it
+
+    def run(it):
+        it.initialize(state)
+        it.run()
+        return it.positions(), it.device_stats()
+
+    (mg, itg), (me, ite) = make(True), make(False)
+    a, sa = run(itg)
+    b, _ = run(ite)
+    assert close(a, b, 1e-6) and sa["whole_stage_graphs"] == sa["velocity_evaluations"] - 1
+    for m in (mg, me):
+        m.set_epsilon(1e-2)
+        m.set_max_iter(8)
+    c, sc = run(itg)
+    d, _ = run(ite)
+    print("before:", sa, "after:", sc)
+    assert close(c, d, 1e-6)
+    # one eager evaluation sets the persistent kernel up for the new cap
+    assert sc["whole_stage_graphs"] == sc["velocity_evaluations"] - 1
