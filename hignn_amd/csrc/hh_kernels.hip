@@ -197,43 +197,80 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) y[t] = f32x4{0, 0, 0, 0};
-  for (int q = 0; q < 16; q++) {
-    const float *bb0 = W.b2 + 32 * q + 4 * g;
-    const float *bb1 = bb0 + 16;
-    const f32x4 b0v = f32x4{bb0[0], bb0[1], bb0[2], bb0[3]};
-    const f32x4 b1v = f32x4{bb1[0], bb1[1], bb1[2], bb1[3]};
-    f32x4 acc0[NT], acc1[NT];
+  // Layer 2 on MFMA (bias as C), tanh on the VALU and layer 3 on MFMA, per
+  // block q of 32 hidden units.  Software pipelined: the layer-2 MFMAs of
+  // tile t+1 (after the last tile: of tile 0 of block q+1) are issued beside
+  // the activation of tile t, so that the matrix cores and the VALU work at
+  // the same time.  A tile's accumulators are refilled only after they have
+  // been consumed, so the pipeline costs no registers; every output keeps
+  // the same operation order as an unpipelined loop.
+  H fa0[4], fa1[4];
+  f32x4 bq0, bq1;
+  f32x4 acc0[NT], acc1[NT];
+  auto load_block = [&](int q) {
     const H *a0p = lw2 + (size_t)(2 * q) * 4 * 64 + lane;
-    const H *a1p = lw2 + (size_t)(2 * q + 1) * 4 * 64 + lane;
-    {
-      const H a0 = a0p[0];
-      const H a1 = a1p[0];
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        acc0[t] = mma(a0, h1[t][0], b0v);  // bias enters as C
-        acc1[t] = mma(a1, h1[t][0], b1v);
-      }
+    for (int s = 0; s < 4; s++) {
+      fa0[s] = a0p[s * 64];
+      fa1[s] = a0p[(4 + s) * 64];
     }
+    const float *bb0 = W.b2 + 32 * q + 4 * g;
+    bq0 = f32x4{bb0[0], bb0[1], bb0[2], bb0[3]};
+    bq1 = f32x4{bb0[16], bb0[17], bb0[18], bb0[19]};
+  };
+  auto layer2 = [&](int t) {
+    f32x4 c0 = mma(fa0[0], h1[t][0], bq0);
+    f32x4 c1 = mma(fa1[0], h1[t][0], bq1);
 #pragma unroll
     for (int s = 1; s < 4; s++) {
-      const H a0 = a0p[s * 64];
-      const H a1 = a1p[s * 64];
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        acc0[t] = mma(a0, h1[t][s], acc0[t]);
-        acc1[t] = mma(a1, h1[t][s], acc1[t]);
-      }
+      c0 = mma(fa0[s], h1[t][s], c0);
+      c1 = mma(fa1[s], h1[t][s], c1);
     }
+    acc0[t] = c0;
+    acc1[t] = c1;
+  };
+  auto layer3 = [&](int t, const H &a3) {
+    H b;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      b[r] = (E)act(acc0[t][r]);
+      b[4 + r] = (E)act(acc1[t][r]);
+    }
+    y[t] = mma(a3, b, y[t]);
+  };
+  load_block(0);
+  layer2(0);
+  for (int q = 0; q < 15; q++) {
     const H a3 = lw3[q * 64 + lane];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      H b;
+    for (int t = 0; t + 1 < NT; t++) {
+      layer2(t + 1);
+      layer3(t, a3);
+    }
+    load_block(q + 1);
+    layer2(0);
+    layer3(NT - 1, a3);
+    // interleave: one MFMA, two transcendental, two other VALU instructions
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        b[r] = (E)act(acc0[t][r]);
-        b[4 + r] = (E)act(acc1[t][r]);
-      }
-      y[t] = mma(a3, b, y[t]);
+    for (int i = 0; i < 9 * NT; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x400, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+    }
+  }
+  {
+    const H a3 = lw3[15 * 64 + lane];
+#pragma unroll
+    for (int t = 0; t + 1 < NT; t++) {
+      layer2(t + 1);
+      layer3(t, a3);
+    }
+    layer3(NT - 1, a3);
+#pragma unroll
+    for (int i = 0; i < 9 * NT; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x400, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
     }
   }
   const float *b3 = W.b3 + 4 * g;
