@@ -796,27 +796,41 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
                                                const S *pool, int64_t stride, int k,
                                                const P (*piv)[9], P *sh, double *X) {
   const int ncross = k - 2;  // cross terms of iteration k-1: l < k-2
+  // Loads are unconditional, so that all of a lane's items are in flight at
+  // once: an item past the chunk's end reads the chunk's first item (always
+  // valid) and is zeroed after the load.
+  int64_t off[kAcaIpt];
+#pragma unroll
+  for (int s = 0; s < kAcaIpt; s++) off[s] = valid[s] ? 64 * s : -(int64_t)threadIdx.x;
   P a[kAcaIpt][9];
 #pragma unroll
-  for (int s = 0; s < kAcaIpt; s++)
+  for (int s = 0; s < kAcaIpt; s++) {
+    if (ncross > 0)  // uniform
+      load9(pool + 9 * ((int64_t)(k - 1) * stride + off[s]), a[s]);
 #pragma unroll
     for (int e = 0; e < 9; e++) {
       sum[s][e] = (P)0;
-      a[s][e] = (valid[s] && ncross > 0) ? (P)pool[9 * ((int64_t)(k - 1) * stride + 64 * s) + e] : (P)0;
+      a[s][e] = (valid[s] && ncross > 0) ? a[s][e] : (P)0;
     }
+  }
   for (int l0 = 0; l0 < k; l0 += kCrossLC) {
     const int nl = min(kCrossLC, k - l0);
     const bool cross = l0 < ncross;  // uniform
     P x[kCrossN];
 #pragma unroll
     for (int e = 0; e < kCrossN; e++) x[e] = (P)0;
+    P vv[kAcaIpt][kCrossLC][9];
+#pragma unroll
+    for (int s = 0; s < kAcaIpt; s++)
+#pragma unroll
+      for (int c = 0; c < kCrossLC; c++)
+        if (c < nl) load9(pool + 9 * ((int64_t)(l0 + c) * stride + off[s]), vv[s][c]);
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++) {
 #pragma unroll
       for (int c = 0; c < kCrossLC; c++) {
         if (!valid[s] || c >= nl) continue;
-        P v[9];
-        load9(pool + 9 * ((int64_t)(l0 + c) * stride + 64 * s), v);
+        const P(&v)[9] = vv[s][c];
         const P *cl = SIDE == 0 ? v : piv[l0 + c];
         const P *ql = SIDE == 0 ? piv[l0 + c] : v;
 #pragma unroll
