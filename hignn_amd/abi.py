@@ -47,6 +47,9 @@ def lib():
             "hh_model_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dense_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dot_device": [c_vp, c_vp, c_vp, c_i64, c_vp],
+            "hh_model_update_coord_device": [c_vp, c_vp, c_i64, c_vp],
+            "hh_model_dot_capturable": [c_vp, c_vp, c_vp],
+            "hh_model_dot_enqueue": [c_vp, c_vp, c_vp, c_i64, c_vp],
             "hh_model_tree_counts": [c_vp, c_vp],
             "hh_model_tree_export": [c_vp] * 7,
             "hh_model_far_ranks": [c_vp, c_vp],
@@ -163,6 +166,21 @@ class Model:
     def dot_device(self, u_ptr, f_ptr, stream=None):
         _chk(lib().hh_model_dot_device(self.h, c_vp(u_ptr), c_vp(f_ptr), self.n,
                                        c_vp(stream) if stream else None))
+
+    def update_coord_device(self, x_ptr, stream=None):
+        _chk(lib().hh_model_update_coord_device(self.h, c_vp(x_ptr), self.n,
+                                                c_vp(stream) if stream else None))
+
+    def dot_capturable(self):
+        """(capturable, signature) of hh_model_dot_capturable."""
+        cap = ctypes.c_int(0)
+        sig = ctypes.c_uint64(0)
+        _chk(lib().hh_model_dot_capturable(self.h, ctypes.byref(cap), ctypes.byref(sig)))
+        return bool(cap.value), int(sig.value)
+
+    def dot_enqueue(self, u_ptr, f_ptr, stream=None):
+        _chk(lib().hh_model_dot_enqueue(self.h, c_vp(u_ptr), c_vp(f_ptr), self.n,
+                                        c_vp(stream) if stream else None))
 
     def export(self):
         return _export(lib().hh_model_tree_counts, lib().hh_model_tree_export, self.h, self.n)

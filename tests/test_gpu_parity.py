@@ -110,6 +110,68 @@ def test_dot_fp32_persistent_far_field_parity(model_small, xavier_w, monkeypatch
     np.testing.assert_array_equal(m.far_ranks(), om.far_ranks())
 
 
+def test_dot_enqueue_states_and_graph_replays(model_small, xavier_w, monkeypatch):
+    """hh_model_dot_capturable / hh_model_dot_enqueue: refused without tree
+    reuse, in batched mode and before the persistent kernel is set up for
+    the current iteration cap; otherwise the enqueued matvec, directly and
+    as a HIP graph replayed with eager work in between, equals the eager
+    one.  The signature follows the ACA settings."""
+    import torch
+    x, f, om = model_small["x"], model_small["f"], model_small["om"]
+    dev = torch.device("cuda:0")
+    f_dev = torch.from_numpy(f).to(dev)
+    u_dev = torch.zeros_like(f_dev)
+
+    def make():
+        m = abi.Model(x, 64)
+        m.load_weights(xavier_w)
+        m.set(epsilon=1e-5, max_iter=50)
+        m.update_coord()
+        return m
+
+    monkeypatch.setenv("HH_FAR_MODE", "1")  # batched: stop flags read back per iteration
+    mb = make()
+    mb.set(tree_reuse=True)
+    mb.dot(f)
+    assert not mb.dot_capturable()[0]
+
+    monkeypatch.setenv("HH_FAR_MODE", "0")
+    m = make()
+    m.dot(f)
+    assert not m.dot_capturable()[0]  # tree reuse off: update_coord rebuilds on the host
+    with pytest.raises(RuntimeError, match="capturable"):
+        m.dot_enqueue(u_dev.data_ptr(), f_dev.data_ptr())
+    m.set(tree_reuse=True)
+    want = m.dot(f)
+    assert O.rel_l2(want, om.dot(xavier_w, f)) <= 1e-5
+    cap, sig = m.dot_capturable()
+    assert cap
+    m.dot_enqueue(u_dev.data_ptr(), f_dev.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert O.rel_l2(u_dev.cpu().numpy(), want) <= 1e-6  # FP64 atomics: order only
+
+    g = torch.cuda.CUDAGraph()
+    u_g = torch.zeros_like(f_dev)
+    with torch.cuda.graph(g):
+        m.dot_enqueue(u_g.data_ptr(), f_dev.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        u_g.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        err = O.rel_l2(u_g.cpu().numpy(), want)
+        print(f"graph replay {rep}: rel diff {err:.2e}")
+        assert err <= 1e-6
+        assert O.rel_l2(m.dot(f), want) <= 1e-6  # eager matvec between replays
+
+    m.set(epsilon=1e-4)
+    cap2, sig2 = m.dot_capturable()
+    assert cap2 and sig2 != sig  # the tolerance is a launch argument
+    m.set(max_iter=30)
+    assert not m.dot_capturable()[0]  # per-block pools sized for 50 iterations
+    m.dot(f)
+    assert m.dot_capturable()[0]
+
+
 def test_dense_dot_parity(model_small, xavier_w):
     m, om, f = model_small["m"], model_small["om"], model_small["f"]
     err = O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f))
