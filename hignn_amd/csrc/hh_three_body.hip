@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void layer1_kernel(EdgeBatch B, const flo
 
 // ---------------------------------------------------------------------------
 // C[e][o] = tanh(sum_k A[e][k] W[o][k] + b[o]) for a batch of edges:
-// large tiles (kTM x kTN below) in waves of WM x WN.  K is staged
+// large tiles (TM x TN, Tiles below) in waves of WM x WN.  K is staged
 // through two LDS buffers: while the waves multiply one K step, the next is
 // in flight from global memory into registers and then stored to the other
 // buffer, one barrier per step.  MFMA 16x16x32 f16 (FP16 path, K steps of 64)
@@ -292,10 +292,20 @@ void gemm_act_kernel(
   }
 }
 
-// Tiles of the two layer GEMMs: 128 x 128 in 4 waves of 64 x 64 (two
-// workgroups per CU).
-constexpr int kTM = 128, kTN = 128, kWM = 64, kWN = 64;
-constexpr int kGemmThreads = 64 * (kTM / kWM) * (kTN / kWN);
+// Tiles of the two layer GEMMs.  A wave tile of WM x WN does WM WN / (WM + WN)
+// FLOP per LDS byte; 128 x 64 (43) keeps the MFMA units fed within the CU's
+// LDS bandwidth, where 64 x 64 (32) sits right at it.  The batch rows are
+// padded to a multiple of 256.
+template <class T>
+struct Tiles {  // FP16: 256 x 256 in 8 waves of 128 x 64, one workgroup per CU
+  static constexpr int TM = 256, TN = 256, WM = 128, WN = 64;
+  static constexpr int threads = 64 * (TM / WM) * (TN / WN);
+};
+template <>
+struct Tiles<float> {  // FP32: 128 x 128 in 4 waves of 64 x 64
+  static constexpr int TM = 128, TN = 128, WM = 64, WN = 64;
+  static constexpr int threads = 64 * (TM / WM) * (TN / WN);
+};
 
 // ---------------------------------------------------------------------------
 // Layer 4 (512 -> 9) on MFMA, then the 3x3 block over the radii times the
@@ -382,35 +392,46 @@ hipError_t launch_triplet_fill(const uint32_t *off, const int32_t *idx, int64_t 
 hipError_t launch_edge_mlp(const EdgeBatch &B, const NetDev &net, int din, int half, void *h1,
                            void *h2, void *h3, double *u, hipStream_t st) {
   if (B.count <= 0) return hipSuccess;
-  const int64_t epad = (B.count + kTM - 1) / kTM * kTM;
-  const dim3 g1((unsigned)((B.count + kL1Edges - 1) / kL1Edges));
-  const int t2 = (int)(1024 / kTN * (epad / kTM)), t3 = (int)(512 / kTN * (epad / kTM));
-  const dim3 g2((unsigned)((t2 + 7) / 8 * 8)), g3((unsigned)((t3 + 7) / 8 * 8));
   const dim3 g4((unsigned)std::min<int64_t>((B.count + 63) / 64, kL4Blocks));
+  auto grids = [&](auto tiles, int &t2, int &t3, dim3 &g2, dim3 &g3) {
+    typedef decltype(tiles) TL;
+    const int64_t epad = (B.count + TL::TM - 1) / TL::TM * TL::TM;
+    t2 = (int)(1024 / TL::TN * (epad / TL::TM));
+    t3 = (int)(512 / TL::TN * (epad / TL::TM));
+    g2 = dim3((unsigned)((t2 + 7) / 8 * 8));
+    g3 = dim3((unsigned)((t3 + 7) / 8 * 8));
+  };
+  const dim3 g1((unsigned)((B.count + kL1Edges - 1) / kL1Edges));
+  int t2, t3;
+  dim3 g2, g3;
   if (half) {
     typedef _Float16 H;
+    typedef Tiles<H> TL;
+    grids(TL{}, t2, t3, g2, g3);
     if (din == 6)
       hipLaunchKernelGGL((layer1_kernel<6, H>), g1, dim3(kThreads), 0, st, B, net.w1s, net.b1s,
                          (H *)h1);
     else
       hipLaunchKernelGGL((layer1_kernel<3, H>), g1, dim3(kThreads), 0, st, B, net.w1s, net.b1s,
                          (H *)h1);
-    hipLaunchKernelGGL((gemm_act_kernel<H, kTM, kTN, kWM, kWN>), g2, dim3(kGemmThreads), 0, st,
+    hipLaunchKernelGGL((gemm_act_kernel<H, TL::TM, TL::TN, TL::WM, TL::WN>), g2, dim3(TL::threads), 0, st,
                        (const H *)h1, (const H *)net.w2h, net.b2s, (H *)h2, 256, 1024, t2);
-    hipLaunchKernelGGL((gemm_act_kernel<H, kTM, kTN, kWM, kWN>), g3, dim3(kGemmThreads), 0, st,
+    hipLaunchKernelGGL((gemm_act_kernel<H, TL::TM, TL::TN, TL::WM, TL::WN>), g3, dim3(TL::threads), 0, st,
                        (const H *)h2, (const H *)net.w3h, net.b3s, (H *)h3, 1024, 512, t3);
     hipLaunchKernelGGL(layer4_kernel<H>, g4, dim3(kThreads), 0, st, B, (const H *)h3, net.w4,
                        net.b4, u);
   } else {
+    typedef Tiles<float> TL;
+    grids(TL{}, t2, t3, g2, g3);
     if (din == 6)
       hipLaunchKernelGGL((layer1_kernel<6, float>), g1, dim3(kThreads), 0, st, B, net.w1s,
                          net.b1s, (float *)h1);
     else
       hipLaunchKernelGGL((layer1_kernel<3, float>), g1, dim3(kThreads), 0, st, B, net.w1s,
                          net.b1s, (float *)h1);
-    hipLaunchKernelGGL((gemm_act_kernel<float, kTM, kTN, kWM, kWN>), g2, dim3(kGemmThreads), 0,
+    hipLaunchKernelGGL((gemm_act_kernel<float, TL::TM, TL::TN, TL::WM, TL::WN>), g2, dim3(TL::threads), 0,
                        st, (const float *)h1, net.w2s, net.b2s, (float *)h2, 256, 1024, t2);
-    hipLaunchKernelGGL((gemm_act_kernel<float, kTM, kTN, kWM, kWN>), g3, dim3(kGemmThreads), 0,
+    hipLaunchKernelGGL((gemm_act_kernel<float, TL::TM, TL::TN, TL::WM, TL::WN>), g3, dim3(TL::threads), 0,
                        st, (const float *)h2, net.w3s, net.b3s, (float *)h3, 1024, 512, t3);
     hipLaunchKernelGGL(layer4_kernel<float>, g4, dim3(kThreads), 0, st, B, (const float *)h3,
                        net.w4, net.b4, u);
