@@ -660,6 +660,53 @@ __device__ __forceinline__ void load9(const S *src, P (&d)[9]) {
   for (int e = 0; e < 9; e++) d[e] = (P)src[e];
 }
 
+// Batched ACA factor pools, addressed by item (one 3x3 entry block each).
+// FP32 / FP64 pools interleave: entry e of item i at 9 i + e.  BF16 pools
+// hold items in groups of 64: the group's entries 0-7 as 16-byte vectors
+// (64 x 16 B), then its entries 8 (64 x 2 B), 1,152 bytes per group, so a
+// wave's loads of consecutive items are aligned 16-byte loads plus one
+// coalesced 2-byte load (interleaved 18-byte items load at ~3/4 the rate).
+template <class S>
+struct PoolAt {
+  __device__ static S get(const S *p, int64_t i, int e) { return p[9 * i + e]; }
+  template <class P>
+  __device__ static void load(const S *p, int64_t i, P (&d)[9]) {
+    load9(p + 9 * i, d);
+  }
+  __device__ static void store(S *p, int64_t i, const S (&v)[9]) {
+#pragma unroll
+    for (int e = 0; e < 9; e++) p[9 * i + e] = v[e];
+  }
+};
+template <>
+struct PoolAt<__bf16> {
+  __device__ static int64_t at8(int64_t i) { return (i >> 6) * 576 + (i & 63) * 8; }
+  __device__ static int64_t at1(int64_t i) { return (i >> 6) * 576 + 512 + (i & 63); }
+  __device__ static __bf16 get(const __bf16 *p, int64_t i, int e) {
+    return e < 8 ? p[at8(i) + e] : p[at1(i)];
+  }
+  template <class P>
+  __device__ static void load(const __bf16 *p, int64_t i, P (&d)[9]) {
+    const uint4 w = *(const uint4 *)(p + at8(i));
+    const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {  // bf16 -> FP32 is exact: the bits move up
+      d[2 * q] = (P)__uint_as_float(ww[q] << 16);
+      d[2 * q + 1] = (P)__uint_as_float(ww[q] & 0xffff0000u);
+    }
+    d[8] = (P)p[at1(i)];
+  }
+  __device__ static void store(__bf16 *p, int64_t i, const __bf16 (&v)[9]) {
+    uint32_t ww[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      ww[q] = (uint32_t)__builtin_bit_cast(uint16_t, v[2 * q]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, v[2 * q + 1]) << 16);
+    *(uint4 *)(p + at8(i)) = make_uint4(ww[0], ww[1], ww[2], ww[3]);
+    p[at1(i)] = v[8];
+  }
+};
+
 template <class P>
 __device__ __forceinline__ P det3(const P (&m)[9]) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
@@ -728,6 +775,24 @@ __device__ double block_sum(double v, double *sh) {
 // N rows of 65 values of type T, the odd stride spreads the rows over the
 // banks); lanes e and e + 32 each add half of row e and one exchange
 // completes the sum.
+// wave_sum_lds: the same once the lanes have written their values to
+// sh[e * 65 + lane] themselves.
+template <class T, int N>
+__device__ __forceinline__ void wave_sum_lds(T *sh, double *dst, int nvalid) {
+  static_assert(N <= 32, "at most 32 values");
+  const int lane = threadIdx.x & 63;
+  __syncthreads();
+  const int e = lane & 31, h = lane >> 5;
+  double t = 0.0;
+  if (e < N) {
+#pragma unroll
+    for (int q = 0; q < 32; q++) t += (double)sh[e * 65 + 32 * h + q];
+  }
+  t += __shfl_xor(t, 32, 64);
+  if (lane < nvalid) dst[lane] = t;
+  __syncthreads();
+}
+
 template <class T, int N>
 __device__ __forceinline__ void wave_sum_store(const T (&x)[N], T *sh, double *dst, int nvalid) {
   static_assert(N <= 32, "at most 32 values");
@@ -786,27 +851,28 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 //                x[9l + 3d1 + d2] = sum_i sum_row C_{k-1}(i)[row][d1] C_l(i)[row][d2]
 //   row side:    L_l = C_l(ik) from LDS, R_l = Q_l(j) from the pool,
 //                x[9l + 3d1 + d2] = sum_j sum_col Q_{k-1}(j)[d1][col] Q_l(j)[d2][col]
-// kCrossLC iterations per wave reduction.  pool points at the lane's first
-// item of C_0 (Q_0); the lane's items are 64 apart.
+// kCrossLC iterations per wave reduction.  item0 is the lane's first item of
+// C_0 (Q_0) in the pool, factor l starts stride items later; the lane's items
+// are 64 apart.
 constexpr int kCrossLC = 1;
 constexpr int kCrossN = 9 * kCrossLC;  // cross sums per wave reduction
 
 template <class P, class S, int SIDE>
 __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
-                                               const S *pool, int64_t stride, int k,
-                                               const P (*piv)[9], P *sh, double *X) {
+                                               const S *pool, int64_t item0, int64_t stride,
+                                               int k, const P (*piv)[9], P *sh, double *X) {
   const int ncross = k - 2;  // cross terms of iteration k-1: l < k-2
   // Loads are unconditional, so that all of a lane's items are in flight at
   // once: an item past the chunk's end reads the chunk's first item (always
   // valid) and is zeroed after the load.
   int64_t off[kAcaIpt];
 #pragma unroll
-  for (int s = 0; s < kAcaIpt; s++) off[s] = valid[s] ? 64 * s : -(int64_t)threadIdx.x;
+  for (int s = 0; s < kAcaIpt; s++) off[s] = item0 + (valid[s] ? 64 * s : -(int64_t)threadIdx.x);
   P a[kAcaIpt][9];
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     if (ncross > 0)  // uniform
-      load9(pool + 9 * ((int64_t)(k - 1) * stride + off[s]), a[s]);
+      PoolAt<S>::load(pool, (int64_t)(k - 1) * stride + off[s], a[s]);
 #pragma unroll
     for (int e = 0; e < 9; e++) {
       sum[s][e] = (P)0;
@@ -824,7 +890,7 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
     for (int s = 0; s < kAcaIpt; s++)
 #pragma unroll
       for (int c = 0; c < kCrossLC; c++)
-        if (c < nl) load9(pool + 9 * ((int64_t)(l0 + c) * stride + off[s]), vv[s][c]);
+        if (c < nl) PoolAt<S>::load(pool, (int64_t)(l0 + c) * stride + off[s], vv[s][c]);
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++) {
 #pragma unroll
@@ -884,7 +950,7 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
   const int k = A.it[b];
   const int jk = A.jpiv[(size_t)b * (A.max_iter + 1) + k];
   for (int t = threadIdx.x; t < 9 * k; t += 64)
-    piv[t / 9][t % 9] = (P)Qp[9 * (fb.qoff + (int64_t)(t / 9) * fb.n + jk) + t % 9];
+    piv[t / 9][t % 9] = (P)PoolAt<SQ>::get(Qp, fb.qoff + (int64_t)(t / 9) * fb.n + jk, t % 9);
   for (int l = threadIdx.x; l < k; l += 64) used[l] = A.ipiv[(size_t)b * A.max_iter + l];
   __syncthreads();
   const int iend = min(ch.off + A.chunk, fb.m);
@@ -892,7 +958,7 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < iend;
   P c[kAcaIpt][9];
-  residual_cross<P, SC, 0>(c, valid, Cp + 9 * (fb.coff + ch.off + threadIdx.x), fb.m, k, piv, sh,
+  residual_cross<P, SC, 0>(c, valid, Cp, fb.coff + ch.off + threadIdx.x, fb.m, k, piv, sh,
                            A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
@@ -902,7 +968,7 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
 #pragma unroll
     for (int e = 0; e < 9; e++) c[s][e] = raw[e] - c[s][e];
   }
-  SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  const int64_t ck0 = fb.coff + (int64_t)k * fb.m;  // first item of C_k
   // 16-bit C pools: the un-normalised residual column stays in FP32, in place
   // of the pass's raw entries, so the pivot inverse and the normalisation see
   // it unrounded; only the normalised factor is stored in 16 bit
@@ -912,12 +978,14 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
   for (int s = 0; s < kAcaIpt; s++) {
     const int i = ch.off + threadIdx.x + 64 * s;
     if (!valid[s]) continue;
+    if constexpr (sizeof(SC) == 2) {
 #pragma unroll
-    for (int e = 0; e < 9; e++) {
-      if constexpr (sizeof(SC) == 2)
-        Rk[9 * (int64_t)i + e] = (float)c[s][e];
-      else
-        Ck[9 * (int64_t)i + e] = c[s][e];
+      for (int e = 0; e < 9; e++) Rk[9 * (int64_t)i + e] = (float)c[s][e];
+    } else {
+      SC v[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) v[e] = (SC)c[s][e];
+      PoolAt<SC>::store(Cp, ck0 + i, v);
     }
     bool u = false;
     for (int l = 0; l < k; l++) u |= (used[l] == i);
@@ -952,9 +1020,7 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
 #pragma unroll
       for (int e = 0; e < 9; e++) m[e] = (double)Rk[e];
     } else {
-      const SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m + ik);
-#pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = (double)Ck[e];
+      PoolAt<SC>::load(Cp, fb.coff + (int64_t)k * fb.m + ik, m);
     }
     const double a = m[0], bb = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6],
                  h = m[7], ii = m[8];
@@ -992,7 +1058,7 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp)
   P inv[9];
 #pragma unroll
   for (int e = 0; e < 9; e++) inv[e] = (P)A.inv[9 * (size_t)ch.slot + e];
-  SC *Ck = Cp + 9 * (fb.coff + (int64_t)k * fb.m);
+  const int64_t ck0 = fb.coff + (int64_t)k * fb.m;  // first item of C_k
   const int iend = min(ch.off + A.chunk, fb.m);
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -1003,19 +1069,20 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp)
     if constexpr (sizeof(SC) == 2)
       load9(A.Y + 9 * (A.ybase_c[ch.slot] + (int64_t)i), c);
     else
-      load9(Ck + 9 * (int64_t)i, c);
+      PoolAt<SC>::load(Cp, ck0 + i, c);
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
       for (int cc = 0; cc < 3; cc++)
         o[3 * r + cc] = c[3 * r] * inv[cc] + c[3 * r + 1] * inv[3 + cc] + c[3 * r + 2] * inv[6 + cc];
     double a[9];
+    SC st[9];  // the norms see the stored factor
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      const SC st = (SC)o[e];  // the norms see the stored factor
-      Ck[9 * (int64_t)i + e] = st;
-      a[e] = (double)st;
+      st[e] = (SC)o[e];
+      a[e] = (double)st[e];
     }
+    PoolAt<SC>::store(Cp, ck0 + i, st);
     gram_col(a, g);
   }
 #pragma unroll
@@ -1039,7 +1106,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
   const int k = A.it[b];
   const int ik = A.ipiv[(size_t)b * A.max_iter + k];
   for (int t = threadIdx.x; t < 9 * k; t += 64)
-    piv[t / 9][t % 9] = (P)Cp[9 * (fb.coff + (int64_t)(t / 9) * fb.m + ik) + t % 9];
+    piv[t / 9][t % 9] = (P)PoolAt<SC>::get(Cp, fb.coff + (int64_t)(t / 9) * fb.m + ik, t % 9);
   for (int l = threadIdx.x; l <= k; l += 64) used[l] = A.jpiv[(size_t)b * (A.max_iter + 1) + l];
   __syncthreads();
   const int jend = min(ch.off + A.chunk, fb.n);
@@ -1047,7 +1114,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < jend;
   P q[kAcaIpt][9];
-  residual_cross<P, SQ, 1>(q, valid, Qp + 9 * (fb.qoff + ch.off + threadIdx.x), fb.n, k, piv, sh,
+  residual_cross<P, SQ, 1>(q, valid, Qp, fb.qoff + ch.off + threadIdx.x, fb.n, k, piv, sh,
                            A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
@@ -1057,7 +1124,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 #pragma unroll
     for (int e = 0; e < 9; e++) q[s][e] = raw[e] - q[s][e];
   }
-  SQ *Qk = Qp + 9 * (fb.qoff + (int64_t)k * fb.n);
+  const int64_t qk0 = fb.qoff + (int64_t)k * fb.n;  // first item of Q_k
   ArgMax best{kNegHuge, -1};
   double g[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -1065,13 +1132,14 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
     const int j = ch.off + threadIdx.x + 64 * s;
     if (!valid[s]) continue;
     double a[9];
+    SQ st[9];  // pivot and norms see the stored factor
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      const SQ st = (SQ)q[s][e];  // pivot and norms see the stored factor
-      Qk[9 * (int64_t)j + e] = st;
-      q[s][e] = (P)st;
-      a[e] = (double)st;
+      st[e] = (SQ)q[s][e];
+      q[s][e] = (P)st[e];
+      a[e] = (double)st[e];
     }
+    PoolAt<SQ>::store(Qp, qk0 + j, st);
     bool u = false;
     for (int l = 0; l <= k; l++) u |= (used[l] == j);
     const double dt = (double)det3(q[s]);
@@ -1178,10 +1246,11 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
   const int ns = SIDE == 0 ? fb.n : fb.m;
   const int s0 = SIDE == 0 ? fb.js : fb.is;
   const S *Src;
+  int64_t item0;  // first item of the side's factor 0
   if constexpr (SIDE == 0)
-    Src = Qp + 9 * fb.qoff;
+    Src = Qp, item0 = fb.qoff;
   else
-    Src = Cp + 9 * fb.coff;
+    Src = Cp, item0 = fb.coff;
   const int send = min(ch.off + A.chunk, ns);
   double *W = (SIDE == 0 ? A.w_j : A.w_i) + (size_t)blockIdx.x * A.max_iter * 3;
   float4 f[kAcaIpt];
@@ -1190,31 +1259,39 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
     const int it = ch.off + threadIdx.x + 64 * s;
     f[s] = it < send ? A.fp[s0 + it] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  // ten rank indices (30 sums, padded to 32) per wave reduction
+  // ten rank indices (30 sums, padded to 32) per wave reduction; each rank
+  // index's three partial sums go to LDS as they are formed
+  const int lane = threadIdx.x & 63;
   for (int l0 = 0; l0 < K; l0 += 10) {
     const int nl = min(10, K - l0);
-    double x[32];
-#pragma unroll
-    for (int e = 0; e < 32; e++) x[e] = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int c = 0; c < 10; c++) {
+      double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+      if (c < nl) {  // uniform
 #pragma unroll
-      for (int s = 0; s < kAcaIpt; s++) {
-        const int it = ch.off + threadIdx.x + 64 * s;
-        if (c >= nl || it >= send) continue;
-        const S *q = Src + 9 * ((int64_t)(l0 + c) * ns + it);
-        if (SIDE == 0) {
-          x[3 * c] += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
-          x[3 * c + 1] += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
-          x[3 * c + 2] += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
-        } else {
-          x[3 * c] += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
-          x[3 * c + 1] += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
-          x[3 * c + 2] += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+        for (int s = 0; s < kAcaIpt; s++) {
+          const int it = ch.off + threadIdx.x + 64 * s;
+          if (it >= send) continue;
+          typename std::conditional<sizeof(S) == 8, double, float>::type q[9];  // exact
+          PoolAt<S>::load(Src, item0 + (int64_t)(l0 + c) * ns + it, q);
+          if (SIDE == 0) {
+            x0 += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
+            x1 += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
+            x2 += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
+          } else {
+            x0 += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
+            x1 += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
+            x2 += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+          }
         }
       }
+      sh[(3 * c) * 65 + lane] = x0;
+      sh[(3 * c + 1) * 65 + lane] = x1;
+      sh[(3 * c + 2) * 65 + lane] = x2;
     }
-    wave_sum_store<double, 32>(x, sh, W + 3 * l0, 3 * nl);
+    sh[30 * 65 + lane] = 0.0;
+    sh[31 * 65 + lane] = 0.0;
+    wave_sum_lds<double, 32>(sh, W + 3 * l0, 3 * nl);
   }
 }
 
@@ -1241,15 +1318,17 @@ __global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, co
   const int nd = SIDE == 0 ? fb.m : fb.n;
   const int d0 = SIDE == 0 ? fb.is : fb.js;
   const S *Dst;
+  int64_t item0;  // first item of the side's factor 0
   if constexpr (SIDE == 0)
-    Dst = Cp + 9 * fb.coff;
+    Dst = Cp, item0 = fb.coff;
   else
-    Dst = Qp + 9 * fb.qoff;
+    Dst = Qp, item0 = fb.qoff;
   const int dend = min(ch.off + A.chunk, nd);
   for (int d = ch.off + threadIdx.x; d < dend; d += blockDim.x) {
     double o0 = 0, o1 = 0, o2 = 0;
     for (int l = 0; l < K; l++) {
-      const S *c = Dst + 9 * ((int64_t)l * nd + d);
+      typename std::conditional<sizeof(S) == 8, double, float>::type c[9];  // exact
+      PoolAt<S>::load(Dst, item0 + (int64_t)l * nd + d, c);
       const double x0 = wsh[3 * l], x1 = wsh[3 * l + 1], x2 = wsh[3 * l + 2];
       if (SIDE == 0) {
         o0 += (double)c[0] * x0 + (double)c[1] * x1 + (double)c[2] * x2;

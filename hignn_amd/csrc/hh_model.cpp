@@ -506,8 +506,14 @@ struct hh_model {
         co += kmax * hb[(size_t)b].m;
         qo += kmax * hb[(size_t)b].n;
       }
-      Cpool.reserve((size_t)std::max<int64_t>(co, 1) * escc);
-      Qpool.reserve((size_t)std::max<int64_t>(qo, 1) * escq);
+      // BF16 pools keep items in groups of 64 (1,152 bytes, the kernels'
+      // aligned layout); FP32 / FP64 pools interleave 9 entries per item
+      auto pool_bytes = [](int64_t items, size_t esc, bool grouped) {
+        items = std::max<int64_t>(items, 1);
+        return grouped ? (size_t)((items + 63) / 64) * 1152 : (size_t)items * esc;
+      };
+      Cpool.reserve(pool_bytes(co, escc, c16));
+      Qpool.reserve(pool_bytes(qo, escq, q16));
       Y.reserve((size_t)std::max(ysum_c, ysum_r) * 9 * 4);
       upload(blk, hb.data(), hb.size() * sizeof(FarBlock));
       std::vector<int32_t> zeros_it((size_t)nb, 0), jp((size_t)nb * (MI + 1), 0);
