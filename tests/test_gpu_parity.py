@@ -206,6 +206,32 @@ def test_chosen_form_accuracy(model_small, prec):
 
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_chosen_form_persistent_far_field(model_small, xavier_w, monkeypatch, prec):
+    """The 16-bit persistent far-field kernel (one workgroup per block, FP32
+    factor slots) against the FP64 evaluation and against the batched
+    iterations (BF16 factor pools): both within the 1e-3 bar, and close to
+    each other."""
+    x, f = model_small["x"], model_small["f"]
+    tg = np.arange(0, x.shape[0], 5)
+    ref = model_small["m"].check_fp64(f, tg, 0)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("HH_FAR_MODE", mode)
+        m = abi.Model(x, 64)
+        m.load_weights(xavier_w)
+        m.set(epsilon=1e-5, max_iter=50, precision=prec)
+        m.update_coord()
+        out[mode] = m.dot(f)
+        assert np.isfinite(out[mode]).all()
+        e = O.rel_l2(out[mode][tg], ref)
+        print(f"{prec} far mode {mode}: error vs fp64 blocks {e:.2e}")
+        assert e <= 1e-3
+    d = O.rel_l2(out["0"], out["1"])
+    print(f"{prec}: persistent vs batched {d:.2e}")
+    assert d <= 1e-3
+
+
 @pytest.mark.parametrize("chunk", [16, 33])
 def test_dot_fp32_parity_multichunk_blocks(model_small, xavier_w, monkeypatch, chunk):
     """ACA steps and apply cut far blocks into chunks of HH_ACA_CHUNK items;
