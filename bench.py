@@ -167,11 +167,16 @@ def stepper_bench(args):
     3, k_t 20, k_b 50, body force (0,0,-1), Explicit Euler dt 0.005; model
     settings of python/simulate.py:41-51).  Each step is one full velocity
     evaluation (forcing, update_coord with a tree rebuild, matvec) plus the
-    position update.  Timed four ways on the same model settings: the Python
+    position update.  Each model first times its far-field launch choices on
+    the forces at the initial positions and keeps the fastest (untimed;
+    --no-tune-far keeps the automatic choice).  Timed four ways on the same
+    model settings: the Python
     velocity callback over host arrays (as the reference's scripts drive the
     integrator), the device-resident integrator with per-stage HIP graphs,
     the same without graphs, and the device-resident integrator with the tree
     kept across steps (opt-in)."""
+    import torch
+
     from hignn_amd import hignn, weights, workloads
 
     hignn.Init()
@@ -180,6 +185,7 @@ def stepper_bench(args):
     n = x.shape[0]
     w = weights.xavier(0)
     prec = args.prec
+    tuning = [None]  # rows (mode, chunk, far ms) of the last model's far-field tuning
 
     def setup(reuse):
         m = hignn.HignnModel(x, 100 if n > 100 else n // 2)
@@ -196,6 +202,10 @@ def stepper_bench(args):
         fc = hignn.DeviceForcing()
         fc.set_body_force([0.0, 0.0, -1.0])
         fc.set_filaments(args.filaments, nchain, rest, k_t, k_b)
+        if not args.no_tune_far:  # far-field launch choice, timed on the first forces
+            m.update_coord(x)
+            f0 = torch.from_numpy(fc.eval(x)).to(torch.device("cuda:0"))
+            tuning[0] = m.tune_far_field(f0.data_ptr(), torch.cuda.current_stream().cuda_stream)
         return m, fc
 
     def timed(mode, steps):
@@ -260,6 +270,9 @@ def stepper_bench(args):
             "device_tree_reuse_steps_per_s": round(rate["device_reuse"], 3),
             "device_tree_reuse_whole_stage_graph_launches": out["device_reuse"][3],
             "speedup_vs_callback": round(rate["device_graphs"] / rate["callback"], 3),
+            "far_field_tuning": ("off" if args.no_tune_far else
+                                 [{"mode": ("persistent", "batched")[r[0]], "chunk": r[1],
+                                   "far_ms": round(r[2], 3)} for r in tuning[0]]),
             "max_abs_position_diff_vs_callback": diff,
         },
     }
@@ -285,6 +298,9 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size before the timed region "
                          "(known-size reference for HBM byte counters)")
+    ap.add_argument("--no-tune-far", action="store_true",
+                    help="stepper: keep the automatic far-field choice instead of timing the "
+                         "candidates on the first forces")
     ap.add_argument("--stepper", action="store_true",
                     help="time Explicit Euler steps of the filament_cloud suspension instead "
                          "(Python velocity callback vs the device-resident integrator)")

@@ -50,6 +50,7 @@ def lib():
             "hh_model_update_coord_device": [c_vp, c_vp, c_i64, c_vp],
             "hh_model_dot_capturable": [c_vp, c_vp, c_vp],
             "hh_model_dot_enqueue": [c_vp, c_vp, c_vp, c_i64, c_vp],
+            "hh_model_tune_far_field": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
             "hh_model_tree_counts": [c_vp, c_vp],
             "hh_model_tree_export": [c_vp] * 7,
             "hh_model_far_ranks": [c_vp, c_vp],
@@ -170,6 +171,16 @@ class Model:
     def update_coord_device(self, x_ptr, stream=None):
         _chk(lib().hh_model_update_coord_device(self.h, c_vp(x_ptr), self.n,
                                                 c_vp(stream) if stream else None))
+
+    def tune_far_field(self, f_ptr, stream=None):
+        """Rows (mode, chunk, far-field ms) of hh_model_tune_far_field; the
+        model keeps the fastest."""
+        t = np.zeros((8, 3), np.float64)
+        rows = ctypes.c_int(0)
+        _chk(lib().hh_model_tune_far_field(self.h, c_vp(f_ptr), self.n,
+                                           c_vp(stream) if stream else None, _p(t), 8,
+                                           ctypes.byref(rows)))
+        return [(int(a), int(b), float(c)) for a, b, c in t[:rows.value]]
 
     def dot_capturable(self):
         """(capturable, signature) of hh_model_dot_capturable."""

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -751,6 +752,32 @@ struct hh_model {
     return h;
   }
 
+  // Far-field launch tuning on the caller's forces: each candidate (the
+  // persistent kernel when there are at most 4 far blocks per compute unit;
+  // the batched iterations with step chunks of 64, 128 and 256 items) runs
+  // one matvec untimed and one timed (HIP events around the far field); the
+  // fastest is kept.  Rows of `out`: {mode, chunk, far-field ms}.
+  void tune_far_field(const float *f_dev, hipStream_t st, std::vector<std::array<double, 3>> &out) {
+    check_ready();
+    std::vector<std::array<int, 2>> cand;
+    if ((int64_t)far.size() <= 4 * (int64_t)g_cus) cand.push_back({0, aca_chunk});
+    for (int c : {64, 128, 256})
+      if (c <= kAcaChunk) cand.push_back({1, c});
+    out.clear();
+    double best = 0;
+    std::array<int, 2> pick = cand.front();
+    for (const auto &c : cand) {
+      far_mode = c[0];
+      aca_chunk = c[1];
+      for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
+      HIPCHK(hipStreamSynchronize(st));
+      out.push_back({(double)c[0], (double)c[1], t_far});
+      if (out.size() == 1 || t_far < best) best = t_far, pick = c;
+    }
+    far_mode = pick[0];
+    aca_chunk = pick[1];
+  }
+
   // u = M(x) f without host synchronisation or timing (capturable state only).
   void enqueue_dot(float *u_dev, const float *f_dev, hipStream_t st) {
     if (!capturable()) throw HHError(HH_ERR_STATE, "matvec not capturable in the current state");
@@ -1050,6 +1077,20 @@ int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n,
     HIPCHK(hipMemcpyAsync(m->x.data(), coord_dev, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     m->update();
+  });
+}
+
+int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *stream,
+                            double *table, int max_rows, int *rows) {
+  return guarded([&]() {
+    if (!m || !f_dev || !rows) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    std::vector<std::array<double, 3>> out;
+    m->tune_far_field(f_dev, stream ? (hipStream_t)stream : m->stream, out);
+    const int nr = (int)std::min<size_t>(out.size(), (size_t)std::max(max_rows, 0));
+    for (int r = 0; r < nr && table; r++)
+      for (int c = 0; c < 3; c++) table[3 * r + c] = out[(size_t)r][(size_t)c];
+    *rows = nr;
   });
 }
 

@@ -117,6 +117,21 @@ class Model {
     check(hh_model_dot_device(m_, (float *)u, (const float *)f, n_, (void *)stream));
   }
 
+  // Far-field launch tuning on device forces f (a data pointer); returns the
+  // rows (mode, chunk, far-field ms) and keeps the fastest candidate.
+  py::list tune_far_field(uintptr_t f, uintptr_t stream) {
+    double t[8 * 3];
+    int rows = 0;
+    {
+      py::gil_scoped_release nogil;
+      check(hh_model_tune_far_field(m_, (const float *)f, n_, (void *)stream, t, 8, &rows));
+    }
+    py::list out;
+    for (int r = 0; r < rows; r++)
+      out.append(py::make_tuple((int)t[3 * r], (int)t[3 * r + 1], t[3 * r + 2]));
+    return out;
+  }
+
   py::dict tree_export() {
     int64_t c[4];
     check(hh_model_tree_counts(m_, c));
@@ -707,6 +722,7 @@ PYBIND11_MODULE(hignn, m) {
       .def("dense_dot", &Model::dense_dot)
       .def("dot_device", &Model::dot_device, py::arg("u_ptr"), py::arg("f_ptr"),
            py::arg("stream") = 0)
+      .def("tune_far_field", &Model::tune_far_field, py::arg("f_ptr"), py::arg("stream") = 0)
       .def("tree_export", &Model::tree_export)
       .def("far_ranks", &Model::far_ranks)
       .def("check_fp64", &Model::check_fp64, py::arg("f"), py::arg("targets"),

@@ -132,6 +132,19 @@ int hh_model_last_timings(hh_model *m, double t[3]);
  * src/FarDot.cpp:74-79); used for roofline accounting. */
 int hh_model_last_profile(hh_model *m, double ms[2], int64_t pairs[2], int64_t launches[2]);
 
+/* Far-field launch tuning on the caller's forces f (device array (n,3),
+ * stream as hh_model_dot_device).  Candidates: the persistent far-field
+ * kernel (only with at most 4 far blocks per compute unit) and the batched
+ * ACA iterations with step chunks of 64, 128 and 256 items.  Each runs one
+ * untimed and one timed matvec; the fastest is kept for the model's later
+ * matvecs.  table receives up to
+ * max_rows rows of {mode (0 persistent, 1 batched), chunk, far-field ms},
+ * *rows their number.  Every candidate computes the same matvec up to
+ * rounding.  No reference counterpart (the reference has fixed launch
+ * shapes, src/FarDot.cpp:64-120). */
+int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *stream,
+                            double *table, int max_rows, int *rows);
+
 #ifdef __cplusplus
 }
 #endif

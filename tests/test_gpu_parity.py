@@ -172,6 +172,27 @@ def test_dot_enqueue_states_and_graph_replays(model_small, xavier_w, monkeypatch
     assert m.dot_capturable()[0]
 
 
+def test_tune_far_field(model_small, xavier_w):
+    """hh_model_tune_far_field times every candidate (the persistent kernel
+    only with at most 4 far blocks per compute unit, 256 on MI355X; batched
+    with chunks of 64, 128, 256 items) and keeps the fastest; the tuned
+    model's matvec still meets the FP32 bar against the oracle."""
+    import torch
+    x, f, om = model_small["x"], model_small["f"], model_small["om"]
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50)
+    m.update_coord()
+    f_dev = torch.from_numpy(f).to(torch.device("cuda:0"))
+    rows = m.tune_far_field(f_dev.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    print("tuning rows (mode, chunk, far ms):", rows)
+    nfar = len(abi.tree_build(x, 64)["far"])
+    assert sorted(r[1] for r in rows if r[0] == 1) == [64, 128, 256]
+    assert any(r[0] == 0 for r in rows) == (nfar <= 4 * 256)
+    assert all(r[2] > 0 for r in rows)
+    assert O.rel_l2(m.dot(f), om.dot(xavier_w, f)) <= 1e-5
+
+
 def test_dense_dot_parity(model_small, xavier_w):
     m, om, f = model_small["m"], model_small["om"], model_small["f"]
     err = O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f))
