@@ -166,7 +166,7 @@ struct H16<f16x8> {
 };
 
 struct WH {
-  const float *w1b;  // [128][4]
+  const float *w1b;  // [4 s][2 half][64 lanes]: layer-1 A fragments (below)
   const float *b2;   // [512]
   const float *b3;   // [16]
   const void *w2f;   // global: [32 mt][4 s][64 lanes] x 8 halves
@@ -183,17 +183,28 @@ __device__ __forceinline__ void mlp_tiles_h(const WH &W, const H *lw2, const H *
   typedef typename H16<H>::E E;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
+  // Layer 1 on MFMA (v_mfma_f32_16x16x4f32, the VALU being the busy pipe):
+  // A = 16 rows of [W1 | b1] per (s, half), B = (rx, ry, rz, 1) of the
+  // tile's 16 pairs.  The host orders the rows so that output row 4g + r of
+  // (s, half) is hidden unit 32s + 8g + 4 half + r: lane group g then holds
+  // units 32s + 8g + j (j = 0..7) of its pair, layer 2's B operand.
+  float wa[4][2];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int half = 0; half < 2; half++) wa[s][half] = W.w1b[(2 * s + half) * 64 + lane];
   H h1[NT][4];
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
+  for (int t = 0; t < NT; t++) {
+    const float b = g == 0 ? rx[t] : g == 1 ? ry[t] : g == 2 ? rz[t] : 1.0f;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const float *w = W.w1b + 4 * (32 * s + 8 * g + j);
-      const float w0 = w[0], w1 = w[1], w2 = w[2], b = w[3];
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        h1[t][s][j] = (E)act(__builtin_fmaf(w2, rz[t], __builtin_fmaf(w1, ry[t], __builtin_fmaf(w0, rx[t], b))));
-    }
+      for (int half = 0; half < 2; half++) {
+        const f32x4 d = mma(wa[s][half], b, f32x4{0, 0, 0, 0});
+#pragma unroll
+        for (int r = 0; r < 4; r++) h1[t][s][4 * half + r] = (E)act(d[r]);
+      }
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) y[t] = f32x4{0, 0, 0, 0};

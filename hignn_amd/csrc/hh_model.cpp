@@ -241,14 +241,20 @@ struct hh_model {
     w.assign(wts, wts + 71177);
     prep_f<float>(wf32);
     prep_f<double>(wf64);
-    // 16-bit path: FP32 layer 1 and biases in the FP32 layout
+    // 16-bit path: FP32 layer 1 (MFMA A fragments) and biases.  Fragment
+    // (s, half), lane 16 g + p holds column g of [W1 | b1] (x, y, z, bias)
+    // for hidden unit 32 s + 8 (p / 4) + 4 half + p % 4, so that the layer-1
+    // output rows land in layer 2's operand order.
     const float *W1 = w.data(), *B1 = W1 + 384, *B2 = B1 + 128 + 65536, *B3 = B2 + 512 + 4608;
     const double sc = act_scale(false);
     std::vector<float> w1b(512), b2(512), b3(16, 0.0f);
-    for (int k = 0; k < 128; k++) {
-      for (int d = 0; d < 3; d++) w1b[4 * k + d] = (float)(sc * W1[3 * k + d]);
-      w1b[4 * k + 3] = (float)(sc * B1[k]);
-    }
+    for (int s = 0; s < 4; s++)
+      for (int half = 0; half < 2; half++)
+        for (int lane = 0; lane < 64; lane++) {
+          const int g = lane / 16, p = lane % 16;
+          const int k = 32 * s + 8 * (p / 4) + 4 * half + p % 4;
+          w1b[(size_t)(2 * s + half) * 64 + lane] = (float)(sc * (g < 3 ? W1[3 * k + g] : B1[k]));
+        }
     for (int k = 0; k < 512; k++) b2[k] = (float)(sc * B2[k]);
     for (int d = 0; d < 16; d++) {
       const int e = entry_of_row(d, false);
