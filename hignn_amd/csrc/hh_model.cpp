@@ -772,13 +772,20 @@ struct hh_model {
     out.clear();
     double best = 0;
     std::array<int, 2> pick = cand.front();
-    for (const auto &c : cand) {
-      far_mode = c[0];
-      aca_chunk = c[1];
-      for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
-      HIPCHK(hipStreamSynchronize(st));
-      out.push_back({(double)c[0], (double)c[1], t_far});
-      if (out.size() == 1 || t_far < best) best = t_far, pick = c;
+    const int mode0 = far_mode, chunk0 = aca_chunk;
+    try {
+      for (const auto &c : cand) {
+        far_mode = c[0];
+        aca_chunk = c[1];
+        for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
+        HIPCHK(hipStreamSynchronize(st));
+        out.push_back({(double)c[0], (double)c[1], t_far});
+        if (out.size() == 1 || t_far < best) best = t_far, pick = c;
+      }
+    } catch (...) {  // a failed candidate leaves the settings as they were
+      far_mode = mode0;
+      aca_chunk = chunk0;
+      throw;
     }
     far_mode = pick[0];
     aca_chunk = pick[1];
