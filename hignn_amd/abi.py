@@ -121,10 +121,13 @@ class Model:
         h = c_vp()
         _chk(lib().hh_model_create(_p(self.x), self.n, int(leaf), ctypes.byref(h)))
         self.h = h
+        # held by the object: at interpreter exit the module's globals may be
+        # gone before the last model is collected
+        self._destroy = lib().hh_model_destroy
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().hh_model_destroy(self.h)
+        if getattr(self, "h", None) and getattr(self, "_destroy", None):
+            self._destroy(self.h)
             self.h = None
 
     def load_weights(self, w):
