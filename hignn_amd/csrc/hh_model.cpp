@@ -131,6 +131,11 @@ struct hh_model {
   double prof_ms[2] = {0, 0};
   int64_t prof_pairs[2] = {0, 0}, prof_launches[2] = {0, 0};
   hipEvent_t evm[4] = {nullptr, nullptr, nullptr, nullptr};
+  // persistent far field: the near field runs beside it on a second stream
+  // (forked and joined with events), on the compute units the far blocks
+  // leave idle
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, far0_ev = nullptr;
 
   ~hh_model() {
     for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y}) b->release();
@@ -153,6 +158,9 @@ struct hh_model {
       if (e) (void)hipEventDestroy(e);
     for (auto &e : evm)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {fork_ev, join_ev, far0_ev})
+      if (e) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -720,7 +728,28 @@ struct hh_model {
     if (!ev[0]) {
       for (auto &e : ev) HIPCHK(hipEventCreate(&e));
       for (auto &e : evm) HIPCHK(hipEventCreate(&e));
+      HIPCHK(hipEventCreate(&far0_ev));
+      HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+      HIPCHK(hipStreamCreate(&side));  // default priority
     }
+  }
+
+  // Near field on the side stream and the persistent far field on st, both
+  // after the forces are gathered and u is cleared on st; st continues once
+  // both are done.  Both accumulate into u with FP64 atomics.  With timed,
+  // ev[0..1] bracket the near field and far0_ev / ev[2] the far field.
+  void near_beside_persistent_far(hipStream_t st, bool timed) {
+    HIPCHK(hipEventRecord(fork_ev, st));
+    HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+    if (timed) HIPCHK(hipEventRecord(ev[0], side));
+    near_field(side, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+    if (timed) HIPCHK(hipEventRecord(ev[1], side));
+    HIPCHK(hipEventRecord(join_ev, side));
+    if (timed) HIPCHK(hipEventRecord(far0_ev, st));
+    far_field_persistent(st);
+    if (timed) HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
   }
 
   int resolved_far_mode() const {
@@ -796,8 +825,10 @@ struct hh_model {
     if (!capturable()) throw HHError(HH_ERR_STATE, "matvec not capturable in the current state");
     HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
     HIPCHK(launch_zero_f64(u.as<double>(), 3 * n, st));
-    near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
-    far_field_persistent(st);
+    if (!far.empty())
+      near_beside_persistent_far(st, false);
+    else
+      near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
     HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
   }
 
@@ -807,16 +838,22 @@ struct hh_model {
     HIPCHK(hipMemsetAsync(u.p, 0, 24 * (size_t)n, st));
     prof_ms[1] = 0;
     prof_pairs[1] = prof_launches[1] = 0;
-    HIPCHK(hipEventRecord(ev[0], st));
-    near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
-    HIPCHK(hipEventRecord(ev[1], st));
-    far_field(st);
-    HIPCHK(hipEventRecord(ev[2], st));
+    const bool beside = resolved_far_mode() == 0 && !far.empty();
+    if (beside) {
+      near_beside_persistent_far(st, true);
+    } else {
+      HIPCHK(hipEventRecord(ev[0], st));
+      near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+      HIPCHK(hipEventRecord(ev[1], st));
+      far_field(st);
+      HIPCHK(hipEventRecord(ev[2], st));
+    }
     HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
     HIPCHK(hipEventSynchronize(ev[2]));
+    HIPCHK(hipEventSynchronize(ev[1]));
     float a = 0, b = 0;
     HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
-    HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    HIPCHK(hipEventElapsedTime(&b, beside ? far0_ev : ev[1], ev[2]));
     t_close = a;
     t_far = b;
     t_total = (double)a + (double)b;
