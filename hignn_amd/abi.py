@@ -51,6 +51,12 @@ def lib():
             "hh_model_dot_capturable": [c_vp, c_vp, c_vp],
             "hh_model_dot_enqueue": [c_vp, c_vp, c_vp, c_i64, c_vp],
             "hh_model_tune_far_field": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
+            "hh_surrogate_create": [ctypes.c_float, c_vp],
+            "hh_surrogate_set_params": [c_vp, c_vp],
+            "hh_surrogate_get_params": [c_vp, c_vp],
+            "hh_surrogate_eval_device": [c_vp, c_vp, c_i64, c_vp],
+            "hh_surrogate_gradient_device": [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp],
+            "hh_surrogate_train_device": [c_vp, c_vp, c_vp, c_i64, c_int, ctypes.c_float, c_vp],
             "hh_model_tree_counts": [c_vp, c_vp],
             "hh_model_tree_export": [c_vp] * 7,
             "hh_model_far_ranks": [c_vp, c_vp],
@@ -70,6 +76,8 @@ def lib():
         L.hh_model_destroy.restype = None
         L.hh_tree_free.argtypes = [c_vp]
         L.hh_tree_free.restype = None
+        L.hh_surrogate_destroy.argtypes = [c_vp]
+        L.hh_surrogate_destroy.restype = None
         _lib = L
     return _lib
 
@@ -231,6 +239,52 @@ class Model:
         launches = np.zeros(2, np.int64)
         _chk(lib().hh_model_last_profile(self.h, _p(ms), _p(pairs), _p(launches)))
         return ms, pairs, launches
+
+
+class Surrogate:
+    """The far-field surrogate of include/hh_surrogate.h; r, targets and
+    outputs are device pointers (e.g. torch data_ptr()) of (n,3) and (n,9)
+    float32 arrays."""
+
+    NPARAMS = 1513
+
+    def __init__(self, r0):
+        h = c_vp()
+        _chk(lib().hh_surrogate_create(float(r0), ctypes.byref(h)))
+        self.h = h
+        self._destroy = lib().hh_surrogate_destroy
+
+    def __del__(self):
+        if getattr(self, "h", None) and getattr(self, "_destroy", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+        if p.size != self.NPARAMS:
+            raise ValueError(f"{self.NPARAMS} parameters expected")
+        _chk(lib().hh_surrogate_set_params(self.h, _p(p)))
+
+    def params(self):
+        p = np.empty(self.NPARAMS, np.float32)
+        _chk(lib().hh_surrogate_get_params(self.h, _p(p)))
+        return p
+
+    def eval(self, r_ptr, n, out_ptr):
+        _chk(lib().hh_surrogate_eval_device(self.h, c_vp(r_ptr), int(n), c_vp(out_ptr)))
+
+    def gradient(self, r_ptr, t_ptr, n):
+        g = np.empty(self.NPARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        _chk(lib().hh_surrogate_gradient_device(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n), _p(g),
+                                                _p(loss)))
+        return float(loss[0]), g
+
+    def train(self, r_ptr, t_ptr, n, steps, lr):
+        loss = np.zeros(1, np.float32)
+        _chk(lib().hh_surrogate_train_device(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n),
+                                             int(steps), float(lr), _p(loss)))
+        return float(loss[0])
 
 
 def header_symbols():

@@ -38,8 +38,8 @@ def _run(cmd):
 
 def build_libhh():
     srcs = [os.path.join(CSRC, f) for f in ("hh_kernels.hip", "hh_forcing.hip", "hh_three_body.hip",
-                                             "hh_model.cpp", "hh_tree.cpp", "hh_dynamics.cpp",
-                                             "hh_three_body.cpp")]
+                                             "hh_surrogate.hip", "hh_model.cpp", "hh_tree.cpp",
+                                             "hh_dynamics.cpp", "hh_three_body.cpp")]
     tmp = LIBHH + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", *srcs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIBHH)
