@@ -125,6 +125,7 @@ class OracleModel:
                  max_relative_coord=500000, threads=0):
         self.x = _f32(x, (-1, 3))
         self.n = self.x.shape[0]
+        self._free = lib().orc_model_free  # kept: module globals may be gone at exit
         self.h = lib().orc_model_new(_p(self.x), self.n, int(leaf))
         self.params = (float(epsilon), int(max_iter), int(bool(use_symmetry)),
                        int(max_relative_coord), int(threads))
@@ -132,7 +133,7 @@ class OracleModel:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_model_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def update_coord(self, x=None):

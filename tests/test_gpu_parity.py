@@ -373,3 +373,39 @@ def test_far_mode_auto(model_small, xavier_w, monkeypatch):
     expect = "0" if nfar <= 256 else "1"
     print("far blocks:", nfar, "-> mode", expect)
     assert O.rel_l2(out[None], out[expect]) <= 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("case", ["one", "two_coincident", "three_within_cutoff"])
+def test_dot_tiny_systems_closed_form(xavier_w, prec, case):
+    # Pairs closer than 1e-5 interact through the identity block
+    # (python/convert.py:14-19), the self pair included, so every particle
+    # moves with the sum of the forces on the group.
+    x = {"one": [[0.5, -1.0, 2.0]],
+         "two_coincident": [[1.0, 2.0, 3.0]] * 2,
+         "three_within_cutoff": [[0.0, 0.0, 0.0], [3e-6, 0.0, 0.0], [0.0, -4e-6, 2e-6]]}[case]
+    x = np.asarray(x, np.float32)
+    f = np.random.default_rng(len(x)).normal(size=x.shape).astype(np.float32)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50, precision=prec)
+    m.update_coord()
+    u = m.dot(f)
+    np.testing.assert_allclose(u, np.broadcast_to(f.sum(0), x.shape), rtol=1e-6, atol=1e-7)
+    if prec == "fp32":
+        np.testing.assert_allclose(m.dense_dot(f), u, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("n", [2, 50, 65, 130])
+def test_dot_fp32_parity_below_and_near_one_leaf(xavier_w, n):
+    # a root that stays a leaf (n <= 64), one split (65) and two levels (130)
+    x = workloads.uniform_cube(n, 0.05, n)
+    f = np.random.default_rng(n).normal(size=x.shape).astype(np.float32)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50)
+    m.update_coord()
+    om = O.OracleModel(x, 64, epsilon=1e-5, max_iter=50)
+    om.update_coord()
+    assert O.rel_l2(m.dot(f), om.dot(xavier_w, f)) <= 1e-5
+    assert O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f)) <= 1e-5
