@@ -231,8 +231,12 @@ int hh_surrogate_create(float r0, hh_surrogate **out) {
     s->r0 = r0;
     try {
       HIPCHK(hipStreamCreate(&s->st));
-      for (DBuf *b : {&s->params, &s->grad, &s->m, &s->v}) b->reserve(4 * kNP);
+      for (DBuf *b : {&s->params, &s->grad, &s->m, &s->v}) {
+        b->reserve(4 * kNP);
+        HIPCHK(hipMemset(b->p, 0, 4 * kNP));  // all-zero net and Adam state
+      }
       s->loss.reserve(4);
+      HIPCHK(hipMemset(s->loss.p, 0, 4));
     } catch (...) {
       delete s;
       throw;
@@ -294,6 +298,7 @@ int hh_surrogate_train_device(hh_surrogate *s, const float *r_dev, const float *
     need(s);
     check_n(n);
     if (steps < 0 || !(lr > 0)) throw HHError(HH_ERR_ARG, "steps >= 0 and lr > 0 required");
+    if (steps == 0) s->run_grad(r_dev, t_dev, n);  // the loss at the current parameters
     for (int k = 0; k < steps; k++) {
       s->run_grad(r_dev, t_dev, n);
       s->step++;

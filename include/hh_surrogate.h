@@ -22,7 +22,8 @@ extern "C" {
 
 typedef struct hh_surrogate hh_surrogate;
 
-/* A surrogate with feature scale r0 (> 0); parameters must be set before use. */
+/* A surrogate with feature scale r0 (> 0), all parameters zero (set them
+ * with hh_surrogate_set_params before training). */
 int hh_surrogate_create(float r0, hh_surrogate **out);
 void hh_surrogate_destroy(hh_surrogate *s);
 
@@ -40,7 +41,8 @@ int hh_surrogate_gradient_device(hh_surrogate *s, const float *r_dev, const floa
                                  int64_t n, float *grad, float *loss);
 
 /* steps full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8) at learning rate
- * lr; *final_loss (may be NULL) is the loss at the last step's gradient. */
+ * lr; *final_loss (may be NULL) is the loss at the last step's gradient
+ * (steps == 0: the loss at the current parameters, which stay unchanged). */
 int hh_surrogate_train_device(hh_surrogate *s, const float *r_dev, const float *t_dev, int64_t n,
                               int steps, float lr, float *final_loss);
 

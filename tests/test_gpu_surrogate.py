@@ -34,6 +34,7 @@ def test_gradient_and_forward_match_oracle():
     t = np.random.default_rng(1).normal(size=(3000, 9)).astype(np.float32)
     p = (S.init(2) + np.random.default_rng(3).normal(size=S.NPARAMS) * 0.05).astype(np.float32)
     s = abi.Surrogate(R0)
+    assert not s.params().any()  # a new surrogate starts from zero parameters
     s.set_params(p)
     np.testing.assert_array_equal(s.params(), p)
     rd, td = dev(r), dev(t)
@@ -42,6 +43,9 @@ def test_gradient_and_forward_match_oracle():
     print(f"loss {loss:.6e} vs {loss64:.6e}; gradient rel L2 {rel(g, g64):.2e}")
     assert abs(loss - loss64) <= 1e-5 * loss64
     assert rel(g, g64) <= 1e-5
+    # zero steps: the loss at the current parameters, parameters unchanged
+    assert s.train(rd.data_ptr(), td.data_ptr(), r.shape[0], 0, 1e-3) == loss
+    np.testing.assert_array_equal(s.params(), p)
     out = dev(np.zeros((3000, 9)))
     s.eval(rd.data_ptr(), r.shape[0], out.data_ptr())
     y64, _ = S.forward(p, r, R0)
