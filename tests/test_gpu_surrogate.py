@@ -44,7 +44,8 @@ def test_gradient_and_forward_match_oracle():
     assert abs(loss - loss64) <= 1e-5 * loss64
     assert rel(g, g64) <= 1e-5
     # zero steps: the loss at the current parameters, parameters unchanged
-    assert s.train(rd.data_ptr(), td.data_ptr(), r.shape[0], 0, 1e-3) == loss
+    # (the loss is an atomic sum over workgroups: equal up to summation order)
+    assert abs(s.train(rd.data_ptr(), td.data_ptr(), r.shape[0], 0, 1e-3) - loss) <= 1e-6 * loss
     np.testing.assert_array_equal(s.params(), p)
     out = dev(np.zeros((3000, 9)))
     s.eval(rd.data_ptr(), r.shape[0], out.data_ptr())
