@@ -581,6 +581,7 @@ void close_dot(Model &m, const float *w, const double *fp, double *up) {
 struct AcaOut {
   std::vector<std::vector<double>> C, Q;
   std::vector<int64_t> ipiv, jpiv;
+  std::vector<double> nu2, mu2;  // per iteration, after its stop-test update
 };
 
 int aca_block(const Model &m, const float *w, int64_t I, int64_t J, const double *fp,
@@ -727,6 +728,10 @@ int aca_block(const Model &m, const float *w, int64_t I, int64_t J, const double
                   Qk[9 * j + 3 * d1 + 2] * Q[l][9 * j + 3 * d2 + 2];
           mu2 += 2.0 * std::fabs(cd) * std::fabs(qd);
         }
+    if (dbg) {
+      dbg->nu2.push_back(nu2);
+      dbg->mu2.push_back(mu2);
+    }
     // 6. stop test (:1099-1108)
     it = k + 1;
     if (nu2 < mu2 * eps2 || it >= maxIter || it >= std::min(mI, nJ)) break;
@@ -1080,6 +1085,19 @@ int orc_model_aca_factors(void *h, const float *w, int64_t b, double *C, double 
   }
   if (jpiv)
     for (int l = 0; l <= k; l++) jpiv[l] = o.jpiv[l];
+  return k;
+}
+
+// Stop-test quantities of far block b per iteration (k entries each): nu2
+// (src/FarDot.cpp:948-1037) and mu2 after its update (:1037, :1041-1078).
+int orc_model_aca_norms(void *h, const float *w, int64_t b, double *nu2, double *mu2) {
+  Model *m = (Model *)h;
+  AcaOut o;
+  const int k = aca_block(*m, w, m->lists.farI[b], m->lists.farJ[b], nullptr, nullptr, &o);
+  for (int l = 0; l < k; l++) {
+    if (nu2) nu2[l] = o.nu2[l];
+    if (mu2) mu2[l] = o.mu2[l];
+  }
   return k;
 }
 

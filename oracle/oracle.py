@@ -77,6 +77,8 @@ def lib():
         L.orc_model_sample_time.argtypes = [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]
         L.orc_model_aca_factors.restype = ctypes.c_int
         L.orc_model_aca_factors.argtypes = [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]
+        L.orc_model_aca_norms.restype = ctypes.c_int
+        L.orc_model_aca_norms.argtypes = [c_vp, c_vp, c_i64, c_vp, c_vp]
         _lib = L
     return _lib
 
@@ -205,6 +207,14 @@ class OracleModel:
         k = lib().orc_model_aca_factors(self.h, _p(_f32(w)), int(b), _p(C), _p(Q), _p(ip),
                                         _p(jp))
         return dict(k=k, C=C[:k], Q=Q[:k], ipiv=ip[:k], jpiv=jp[:k + 1], I=int(I), J=int(J))
+
+    def aca_norms(self, w, b, max_iter):
+        """Per-iteration nu2 and mu2 (after the iteration's update) of far
+        block b's ACA."""
+        nu2 = np.zeros(max_iter)
+        mu2 = np.zeros(max_iter)
+        k = lib().orc_model_aca_norms(self.h, _p(_f32(w)), int(b), _p(nu2), _p(mu2))
+        return nu2[:k], mu2[:k]
 
 
 def rel_l2(u, ref):

@@ -333,3 +333,153 @@ def test_dot_vs_fp64_blocks(aca_case, xavier_w):
     # and the near part alone is exact to float rounding
     un = m.dot(xavier_w, f, parts=1)
     assert np.isfinite(un).all()
+
+
+# ------------------------------------------------ ACA decisions of the reference
+def _sym_block(x, tree, perm, I, J, w):
+    """The ACA's entries of far block (I, J): S(Net(x_j - x_i)) with the
+    off-diagonal entries averaged (src/FarDot.cpp:441-456), from the FP32
+    pair function, as float64 (m, n, 3, 3)."""
+    xp = x[perm]
+    xi, xj = xp[tree[I, 2]:tree[I, 3]], xp[tree[J, 2]:tree[J, 3]]
+    r = (xj[None, :, :] - xi[:, None, :]).reshape(-1, 3)
+    M = O.mlp_f32(w, r).reshape(len(xi), len(xj), 3, 3)
+    # float sum, then the double 0.5 (0.5 * (float + float) in the reference)
+    T = (M + M.transpose(0, 1, 3, 2)).astype(np.float64)
+    S = M.astype(np.float64)
+    off = ~np.eye(3, dtype=bool)
+    S[..., off] = 0.5 * T[..., off]
+    return S
+
+
+def _pick(vals, eligible):
+    """First index of the largest value among eligible indices (np.argmax
+    keeps the first maximum), and whether the runner-up is so close that
+    rounding of the test's own sums could reorder them."""
+    v = np.where(eligible, vals, -np.inf)
+    i = int(np.argmax(v))
+    w = v.copy()
+    w[i] = -np.inf
+    close = np.isfinite(w).any() and (v[i] - w.max()) <= 1e-9 * np.abs(vals[eligible]).max()
+    return i, close
+
+
+@pytest.fixture(scope="module")
+def aca_trace_blocks(aca_case, xavier_w):
+    """Far blocks of the ACA test case with at most 160 x 160 particles, with
+    their dense entries and the oracle's factors and stop-test trace."""
+    x, m = aca_case
+    e = m.export()
+    tree, perm = e["tree"].astype(np.int64), e["perm"].astype(np.int64)
+    out = []
+    for b, (I, J) in enumerate(e["far"]):
+        if tree[I, 3] - tree[I, 2] > 160 or tree[J, 3] - tree[J, 2] > 160:
+            continue
+        fa = m.aca_factors(xavier_w, b, 40)
+        nu2, mu2 = m.aca_norms(xavier_w, b, 40)
+        out.append((b, _sym_block(x, tree, perm, I, J, xavier_w), fa, nu2, mu2))
+        if len(out) == 24:
+            break
+    assert len(out) >= 8
+    return out
+
+
+def test_aca_pivots_are_signed_determinant_argmax(aca_trace_blocks):
+    """Every pivot is the first index of the largest SIGNED determinant of the
+    residual 3x3 blocks over the unused indices (src/FarDot.cpp:537-582 for
+    rows, :863-902 for columns), with the first column pivot |J|/2 (:257).
+    Residuals are recomputed densely from the block and the returned
+    factors.  The case is one where the max-|det| rule picks differently, so
+    a |det| pivot fails here."""
+    abs_differs = 0
+    checked = 0
+    for b, S, fa, _, _ in aca_trace_blocks:
+        k, C, Q, ip, jp = fa["k"], fa["C"], fa["Q"], fa["ipiv"], fa["jpiv"]
+        m, n = S.shape[:2]
+        assert jp[0] == n // 2
+        for it in range(k):
+            # column residual at jpiv[it] and the row pivot
+            col = S[:, jp[it]] - np.einsum("lirc,lcd->ird", C[:it], Q[:it, jp[it]])
+            dets = np.linalg.det(col)
+            elig = np.ones(m, dtype=bool)
+            elig[ip[:it]] = False
+            want, close = _pick(dets, elig)
+            if not close:
+                assert ip[it] == want, (b, it)
+                checked += 1
+            if int(np.argmax(np.where(elig, np.abs(dets), -np.inf))) != want:
+                abs_differs += 1
+            # the normalised factor is the residual column times the inverse
+            # of its pivot block (:586-656): C_k times the pivot block gives
+            # the residual back, to the rounding the inversion allows
+            P = col[ip[it]]
+            tol = 1e-12 * np.linalg.cond(P) * np.abs(C[it]).max() * np.abs(P).max()
+            assert np.abs(C[it] @ P - col).max() < tol, (b, it)
+            scale = np.abs(S).max()
+            # row residual at ipiv[it] is the stored Q factor (:690-841)
+            row = S[ip[it]] - np.einsum("lrc,ljcd->jrd", C[:it, ip[it]], Q[:it])
+            assert np.abs(Q[it] - row).max() < 1e-10 * scale, (b, it)
+            dq = np.linalg.det(Q[it])
+            eq = np.ones(n, dtype=bool)
+            eq[jp[:it + 1]] = False
+            if eq.any():
+                want, close = _pick(dq, eq)
+                if not close:
+                    assert jp[it + 1] == want, (b, it)
+                    checked += 1
+                if int(np.argmax(np.where(eq, np.abs(dq), -np.inf))) != want:
+                    abs_differs += 1
+    assert checked > 100
+    assert abs_differs > 0, "case does not discriminate signed from |det| pivots"
+
+
+def test_aca_nu2_is_frobenius_norm_of_new_term(aca_trace_blocks):
+    """nu2 of iteration k is ||C_k Q_k||_F^2 over the whole block
+    (src/FarDot.cpp:948-1037), from a dense product of the factors."""
+    for b, S, fa, nu2, _ in aca_trace_blocks:
+        for it in range(fa["k"]):
+            term = np.einsum("ira,jab->ijrb", fa["C"][it], fa["Q"][it])
+            np.testing.assert_allclose(nu2[it], np.sum(term ** 2), rtol=1e-9)
+
+
+def _mu2_sequence(fa, nu2, last):
+    """mu2 after each iteration when the cross terms of iteration k run over
+    l < k - last (src/FarDot.cpp:1041-1078 reads l < k - 1)."""
+    C, Q = fa["C"], fa["Q"]
+    mu, out = 0.0, []
+    for k in range(fa["k"]):
+        mu += nu2[k]
+        for l in range(k - last):
+            cd = np.einsum("ira,irb->ab", C[k], C[l])
+            qd = np.einsum("jac,jbc->ab", Q[k], Q[l])
+            mu += 2.0 * np.sum(np.abs(cd) * np.abs(qd))
+        out.append(mu)
+    return np.array(out)
+
+
+def test_aca_mu2_cross_terms_skip_previous_iteration(aca_trace_blocks):
+    """The mu2 update adds |<C_k, C_l>| |<Q_k, Q_l>| cross terms for l < k-1
+    only (src/FarDot.cpp:1041 loops to workingNodeIteration - 1 before the
+    counter advances at :1094).  The rule is read from the reference, which
+    cannot be run here; the test discriminates it from the textbook l < k
+    sum, which the blocks below separate by more than rounding."""
+    separated = 0
+    for b, S, fa, nu2, mu2 in aca_trace_blocks:
+        ref = _mu2_sequence(fa, nu2, 1)
+        alt = _mu2_sequence(fa, nu2, 0)
+        np.testing.assert_allclose(mu2, ref, rtol=1e-9)
+        separated += int(np.any(np.abs(alt - ref) > 1e-6 * ref))
+    assert separated > 0
+
+
+def test_aca_stop_is_first_iteration_meeting_the_test(aca_trace_blocks):
+    """The rank is the first iteration count with nu2 < mu2 eps^2, max_iter
+    or min(|I|, |J|) reached (src/FarDot.cpp:1099-1108), eps = 1e-3."""
+    stops = 0
+    for b, S, fa, nu2, mu2 in aca_trace_blocks:
+        m, n = S.shape[:2]
+        k = fa["k"]
+        met = [(nu2[i] < mu2[i] * 1e-6) or (i + 1 >= 40) or (i + 1 >= min(m, n)) for i in range(k)]
+        assert met[-1] and not any(met[:-1]), b
+        stops += int(nu2[k - 1] < mu2[k - 1] * 1e-6)
+    assert stops > 0
