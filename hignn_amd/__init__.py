@@ -1,13 +1,12 @@
 """MI355X implementation of the H-HIGNN mobility matvec and its neighbours.
 
-libhh.so (HIP kernels and the C ABI) and PyTorch each bring a HIP runtime
-with the soname libamdhip64.so.7, and a process keeps whichever copy it
-loads first. PyTorch cannot initialise its devices on another copy, while
-libhh.so runs on PyTorch's. So when PyTorch is installed it is imported
-here, before any submodule loads libhh.so, and both work in one process in
-any import order.
+The package loads libhh.so (HIP kernels and the C ABI) and needs nothing
+else on the GPU side: device buffers for the device-pointer entry points
+come from the library itself (include/hh_device.h, abi.DeviceArray).
+
+libhh.so and PyTorch each bring a HIP runtime with the soname
+libamdhip64.so.7, and a process keeps whichever copy it loads first;
+PyTorch only initialises its devices on its own copy, while libhh.so runs
+on either.  A process that also uses PyTorch on the GPU imports torch before
+this package loads libhh.so.
 """
-try:
-    import torch  # noqa: F401
-except ImportError:  # libhh.so alone: its own runtime
-    pass

@@ -51,12 +51,24 @@ def lib():
             "hh_model_dot_capturable": [c_vp, c_vp, c_vp],
             "hh_model_dot_enqueue": [c_vp, c_vp, c_vp, c_i64, c_vp],
             "hh_model_tune_far_field": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
-            "hh_surrogate_create": [ctypes.c_float, c_vp],
+            "hh_surrogate_create": [ctypes.c_float, ctypes.c_float, c_vp],
+            "hh_surrogate_range": [c_vp, c_vp, c_vp],
             "hh_surrogate_set_params": [c_vp, c_vp],
             "hh_surrogate_get_params": [c_vp, c_vp],
-            "hh_surrogate_eval_device": [c_vp, c_vp, c_i64, c_vp],
-            "hh_surrogate_gradient_device": [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp],
-            "hh_surrogate_train_device": [c_vp, c_vp, c_vp, c_i64, c_int, ctypes.c_float, c_vp],
+            "hh_surrogate_eval_device": [c_vp, c_vp, c_i64, c_vp, c_vp],
+            "hh_surrogate_gradient_device": [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+            "hh_surrogate_train_device": [c_vp, c_vp, c_vp, c_i64, c_int, c_i64, ctypes.c_float,
+                                          ctypes.c_float, c_vp, c_vp],
+            "hh_model_surrogate_samples": [c_vp, c_i64, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_uint64, c_vp, c_vp, c_vp],
+            "hh_model_set_surrogate": [c_vp, c_vp, c_dbl, c_int],
+            "hh_model_surrogate_stats": [c_vp, c_vp],
+            "hh_model_surrogate_errors": [c_vp, c_vp],
+            "hh_device_alloc": [ctypes.c_size_t, c_vp],
+            "hh_device_free": [c_vp],
+            "hh_memcpy_to_device": [c_vp, c_vp, ctypes.c_size_t],
+            "hh_memcpy_to_host": [c_vp, c_vp, ctypes.c_size_t],
+            "hh_device_synchronize": [],
             "hh_model_tree_counts": [c_vp, c_vp],
             "hh_model_tree_export": [c_vp] * 7,
             "hh_model_far_ranks": [c_vp, c_vp],
@@ -207,6 +219,12 @@ class Model:
     def export(self):
         return _export(lib().hh_model_tree_counts, lib().hh_model_tree_export, self.h, self.n)
 
+    def counts(self):
+        """[tree rows, close pairs, far pairs, leaves] of the last build."""
+        c = np.zeros(4, dtype=np.int64)
+        _chk(lib().hh_model_tree_counts(self.h, _p(c)))
+        return [int(v) for v in c]
+
     def far_ranks(self):
         c = np.zeros(4, dtype=np.int64)
         _chk(lib().hh_model_tree_counts(self.h, _p(c)))
@@ -232,6 +250,31 @@ class Model:
         _chk(lib().hh_model_last_timings(self.h, _p(t)))
         return tuple(float(v) for v in t)
 
+    def surrogate_samples(self, n, r_lo, r_hi, seed=0):
+        """(r, targets) DeviceArrays: n displacements, |r| log-uniform in
+        [r_lo, r_hi], labelled |r| S(M(r)) by the FP32 MLP."""
+        r = DeviceArray((n, 3))
+        t = DeviceArray((n, 6))
+        _chk(lib().hh_model_surrogate_samples(self.h, int(n), float(r_lo), float(r_hi),
+                                              int(seed), r.p, t.p, None))
+        return r, t
+
+    def set_surrogate(self, s, tol=2e-3, samples=32):
+        self._sur = s  # kept alive while attached
+        _chk(lib().hh_model_set_surrogate(self.h, s.h if s is not None else None, float(tol),
+                                          int(samples)))
+
+    def surrogate_stats(self):
+        """[judged, on surrogate, fell back, surrogate entries, full-MLP far entries]."""
+        st = np.zeros(5, np.int64)
+        _chk(lib().hh_model_surrogate_stats(self.h, _p(st)))
+        return [int(v) for v in st]
+
+    def surrogate_errors(self):
+        e = np.empty(int(self.counts()[2]), np.float32)
+        _chk(lib().hh_model_surrogate_errors(self.h, _p(e)))
+        return e
+
     def profile(self):
         """(ms, pair evaluations, launches) of [near kernel, ACA passes]."""
         ms = np.zeros(2, np.float64)
@@ -241,17 +284,63 @@ class Model:
         return ms, pairs, launches
 
 
+class DeviceArray:
+    """Device memory of the library's own HIP runtime (include/hh_device.h),
+    holding a float32 array: callers need no other GPU framework."""
+
+    def __init__(self, shape, data=None, dtype=np.float32):
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = c_vp()
+        _chk(lib().hh_device_alloc(max(self.nbytes, 1), ctypes.byref(p)))
+        self.p = p
+        self._free = lib().hh_device_free
+        if data is not None:
+            self.upload(data)
+
+    @classmethod
+    def of(cls, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        return cls(a.shape, a, dtype)
+
+    @property
+    def ptr(self):
+        return int(self.p.value or 0)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"{a.nbytes} bytes for a device array of {self.nbytes}")
+        _chk(lib().hh_memcpy_to_device(self.p, _p(a), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _chk(lib().hh_memcpy_to_host(_p(out), self.p, self.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and getattr(self, "_free", None):
+            self._free(self.p)
+            self.p = None
+
+
+def synchronize():
+    _chk(lib().hh_device_synchronize())
+
+
 class Surrogate:
     """The far-field surrogate of include/hh_surrogate.h; r, targets and
-    outputs are device pointers (e.g. torch data_ptr()) of (n,3) and (n,9)
+    outputs are device pointers (DeviceArray.ptr) of (n,3) and (n,6)
     float32 arrays."""
 
-    NPARAMS = 1513
+    NPARAMS = 1382
 
-    def __init__(self, r0):
+    def __init__(self, r_lo, r_hi):
         h = c_vp()
-        _chk(lib().hh_surrogate_create(float(r0), ctypes.byref(h)))
+        _chk(lib().hh_surrogate_create(float(r_lo), float(r_hi), ctypes.byref(h)))
         self.h = h
+        self.r_lo, self.r_hi = float(r_lo), float(r_hi)
         self._destroy = lib().hh_surrogate_destroy
 
     def __del__(self):
@@ -270,20 +359,23 @@ class Surrogate:
         _chk(lib().hh_surrogate_get_params(self.h, _p(p)))
         return p
 
-    def eval(self, r_ptr, n, out_ptr):
-        _chk(lib().hh_surrogate_eval_device(self.h, c_vp(r_ptr), int(n), c_vp(out_ptr)))
+    def eval(self, r_ptr, n, out_ptr, stream=None):
+        _chk(lib().hh_surrogate_eval_device(self.h, c_vp(r_ptr), int(n), c_vp(out_ptr),
+                                            c_vp(stream)))
 
-    def gradient(self, r_ptr, t_ptr, n):
+    def gradient(self, r_ptr, t_ptr, n, stream=None):
         g = np.empty(self.NPARAMS, np.float32)
         loss = np.zeros(1, np.float32)
         _chk(lib().hh_surrogate_gradient_device(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n), _p(g),
-                                                _p(loss)))
+                                                _p(loss), c_vp(stream)))
         return float(loss[0]), g
 
-    def train(self, r_ptr, t_ptr, n, steps, lr):
+    def train(self, r_ptr, t_ptr, n, steps, lr, batch=0, lr_final=None, stream=None):
         loss = np.zeros(1, np.float32)
         _chk(lib().hh_surrogate_train_device(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n),
-                                             int(steps), float(lr), _p(loss)))
+                                             int(steps), int(batch), float(lr),
+                                             float(lr if lr_final is None else lr_final),
+                                             _p(loss), c_vp(stream)))
         return float(loss[0])
 
 

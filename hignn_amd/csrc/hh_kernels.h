@@ -130,6 +130,45 @@ struct PAcaLaunch {
   DevWeightsH wh;
 };
 
+// Surrogate entries for tiles [tile0, tile0 + ntiles) of an expanded ACA
+// pass tile list (blocks that passed the judge).
+struct AcaSurLaunch {
+  const float4 *xp;
+  const void *tiles;
+  int64_t tile0, ntiles;
+  int row_pass;
+  float *Y;
+  const float *params;
+  float inv_scale;
+};
+
+// Surrogate judge of nblk far blocks: samples pairs per block, the full FP32
+// MLP at them (r: nblk*samples*3 floats, m9: nblk*samples*9 floats of
+// workspace), flag[b] = 1 when the surrogate passes, err[b] its sampled
+// relative error.
+struct JudgeLaunch {
+  const FarBlock *blk;
+  int64_t nblk;
+  const float4 *xp;
+  int samples;
+  float *r, *m9;
+  const float *params;
+  float r_lo, r_hi;
+  double tol;
+  int32_t *flag;
+  float *err;
+  DevWeightsF wf;
+  DevWeightsH wh;
+  int cus;
+};
+
+hipError_t launch_aca_sur(const AcaSurLaunch &L, hipStream_t st);
+hipError_t launch_judge(const JudgeLaunch &L, hipStream_t st);
+// Surrogate training samples and their targets |r| S(M) (n,6) from blocks m9.
+hipError_t launch_sur_samples(int64_t n, float r_lo, float r_hi, uint64_t seed, float *r,
+                              hipStream_t st);
+hipError_t launch_sur_targets(const float *r, const float *m9, int64_t n, float *t,
+                              hipStream_t st);
 hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st);
 hipError_t launch_near(const NearLaunch &L, hipStream_t st);
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);

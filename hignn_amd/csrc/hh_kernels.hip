@@ -31,6 +31,7 @@
 
 #include "hh_internal.h"
 #include "hh_kernels.h"
+#include "hh_surrogate_net.h"
 
 namespace hh {
 
@@ -620,6 +621,150 @@ __global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, 
   stage_fragments<H>(W, lw2, lw3);
   aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
                       f32x4(&y)[4]) { mlp_tiles_h<H, 4>(W, lw2, lw3, rx, ry, rz, y); });
+}
+
+// ---- surrogate-generated ACA entries -------------------------------------
+// The tiles of blocks that passed the surrogate judge: one thread per item,
+// Y receives the symmetric block s(r)/|r| in the same row-major layout as
+// the full MLP's entries (the step kernels' symmetrisation leaves it
+// unchanged).  Same displacement convention as aca_mlp_body: x_j - x_i.
+__global__ __launch_bounds__(256) void aca_sur_kernel(const float4 *xp, const AcaPassTile *tiles,
+                                                      int64_t ntiles, int row_pass, float *Y,
+                                                      const float *params, float inv_scale) {
+  __shared__ float P[sur::kNP];
+  sur::stage(params, P);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = t >> 4;
+  const int p = (int)(t & 15);
+  if (q >= ntiles) return;
+  const AcaPassTile T = tiles[q];
+  if (p >= T.nvalid) return;
+  const float4 xi = xp[T.item0 + p], xv = xp[T.piv];
+  float rx, ry, rz;
+  if (!row_pass) {
+    rx = xv.x - xi.x, ry = xv.y - xi.y, rz = xv.z - xi.z;
+  } else {
+    rx = xi.x - xv.x, ry = xi.y - xv.y, rz = xi.z - xv.z;
+  }
+  float y[sur::kOut];
+  sur::eval(P, rx, ry, rz, inv_scale, y);
+  const float inv = rsqrt_r(rx * rx + ry * ry + rz * rz);
+  float *o = Y + 9 * (T.yrow0 + p);
+  o[0] = y[0] * inv, o[4] = y[1] * inv, o[8] = y[2] * inv;
+  o[1] = o[3] = y[3] * inv;
+  o[5] = o[7] = y[4] * inv;
+  o[2] = o[6] = y[5] * inv;
+}
+
+// ---- surrogate judge --------------------------------------------------------
+// Sample t of far block b: pair (i, j) = (h(bS+t) mod m, h'(bS+t) mod n), a
+// fixed integer hash spreading the samples over the block; r = x_j - x_i.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__global__ void judge_pairs_kernel(const FarBlock *blk, int64_t nblk, const float4 *xp,
+                                   int samples, float *r) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk * samples) return;
+  const FarBlock B = blk[k / samples];
+  const uint32_t h = (uint32_t)k;
+  const int i = (int)(mix32(2u * h) % (uint32_t)B.m);
+  const int j = (int)(mix32(2u * h + 1u) % (uint32_t)B.n);
+  const float4 a = xp[B.is + i], c = xp[B.js + j];
+  r[3 * k] = c.x - a.x;
+  r[3 * k + 1] = c.y - a.y;
+  r[3 * k + 2] = c.z - a.z;
+}
+
+// One wave per block: lane t < samples compares the surrogate with the full
+// MLP's block m9 at its sample (both as 3x3 blocks, the MLP's symmetrised
+// as the ACA uses it); the block passes when the sampled Frobenius error is
+// at most tol times the sampled Frobenius norm and every sampled |r| lies in
+// [r_lo, r_hi].  err[b] receives the relative error (inf when out of range).
+__global__ __launch_bounds__(256) void judge_kernel(const float *r, const float *m9, int64_t nblk,
+                                                    int samples, const float *params, float r_lo,
+                                                    float r_hi, double tol, int32_t *flag,
+                                                    float *err) {
+  __shared__ float P[sur::kNP];
+  sur::stage(params, P);
+  const int lane = threadIdx.x & 63;
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= nblk) return;
+  double e2 = 0.0, n2 = 0.0;
+  int out = 0;
+  for (int t = lane; t < samples; t += 64) {
+    const int64_t k = b * samples + t;
+    const float rx = r[3 * k], ry = r[3 * k + 1], rz = r[3 * k + 2];
+    const float d = sqrtf(rx * rx + ry * ry + rz * rz);
+    if (!(d >= r_lo && d <= r_hi)) out = 1;
+    float y[sur::kOut];
+    sur::eval(P, rx, ry, rz, 1.0f / r_hi, y);
+    const float *M = m9 + 9 * k;
+#pragma unroll
+    for (int o = 0; o < sur::kOut; o++) {
+      const int a = sur::entry_row(o), c = sur::entry_col(o);
+      const double full = 0.5 * (double)(M[3 * a + c] + M[3 * c + a]);
+      const double s = (double)y[o] / (double)d;
+      const double w = sur::entry_weight(o);
+      e2 += w * (s - full) * (s - full);
+      n2 += w * full * full;
+    }
+  }
+  e2 = wave_sum(e2);
+  n2 = wave_sum(n2);
+  for (int m = 32; m >= 1; m >>= 1) out |= __shfl_xor(out, m, 64);
+  if (lane == 0) {
+    const double rel = n2 > 0.0 ? sqrt(e2 / n2) : (e2 > 0.0 ? INFINITY : 0.0);
+    const bool ok = !out && rel <= tol;
+    flag[b] = ok ? 1 : 0;
+    err[b] = out ? INFINITY : (float)rel;
+  }
+}
+
+// ---- surrogate training samples -------------------------------------------
+// Sample k: |r| = r_lo (r_hi / r_lo)^u0 and a uniform direction (cos theta =
+// 2 u1 - 1, phi = 2 pi u2), the u from a splitmix64 counter hash of
+// (seed, k).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double unit(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+
+__global__ void sur_samples_kernel(int64_t n, float r_lo, float r_hi, uint64_t seed, float *r) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t base = splitmix64(seed ^ 0x5851f42d4c957f2dull) + 3ull * (uint64_t)k;
+  const double u0 = unit(splitmix64(base)), u1 = unit(splitmix64(base + 1)),
+               u2 = unit(splitmix64(base + 2));
+  const double d = (double)r_lo * exp(u0 * log((double)r_hi / (double)r_lo));
+  const double ct = 2.0 * u1 - 1.0, st = sqrt(fmax(0.0, 1.0 - ct * ct));
+  const double ph = 6.283185307179586 * u2;
+  r[3 * k] = (float)(d * st * cos(ph));
+  r[3 * k + 1] = (float)(d * st * sin(ph));
+  r[3 * k + 2] = (float)(d * ct);
+}
+
+// t = |r| S(M) as the six surrogate outputs (xx, yy, zz, xy, yz, zx).
+__global__ void sur_targets_kernel(const float *r, const float *m9, int64_t n, float *t) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float rx = r[3 * k], ry = r[3 * k + 1], rz = r[3 * k + 2];
+  const float d = sqrtf(rx * rx + ry * ry + rz * rz);
+  const float *M = m9 + 9 * k;
+#pragma unroll
+  for (int o = 0; o < sur::kOut; o++) {
+    const int a = sur::entry_row(o), c = sur::entry_col(o);
+    t[sur::kOut * k + o] = d * (0.5f * (M[3 * a + c] + M[3 * c + a]));
+  }
 }
 
 // ---- chunked per-block steps ------------------------------------------------
@@ -1868,11 +2013,12 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
 }
 
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
-  if (L.ntiles <= 0) return hipSuccess;
+  if (L.nactive <= 0) return hipSuccess;
   AcaExpandArgs E{L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, L.tfirst, L.nactive,
                   L.max_iter, L.row_pass, (AcaPassTile *)L.tiles};
   hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)((L.nactive + 255) / 256)), dim3(256), 0,
                      st, E);
+  if (L.ntiles <= 0) return hipGetLastError();
   AcaMlpArgs A{L.xp, (const AcaPassTile *)L.tiles, L.ntiles, L.row_pass, L.Y};
   const int64_t waves = (L.ntiles + 3) / 4;
   if (L.prec == PREC_FP32) {
@@ -1890,6 +2036,44 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
     else
       hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A, W);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_sur(const AcaSurLaunch &L, hipStream_t st) {
+  if (L.ntiles <= 0) return hipSuccess;
+  const int64_t threads = 16 * L.ntiles;
+  hipLaunchKernelGGL(aca_sur_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                     L.xp, (const AcaPassTile *)L.tiles + L.tile0, L.ntiles, L.row_pass, L.Y,
+                     L.params, L.inv_scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_judge(const JudgeLaunch &L, hipStream_t st) {
+  if (L.nblk <= 0) return hipSuccess;
+  const int64_t k = L.nblk * L.samples;
+  hipLaunchKernelGGL(judge_pairs_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st,
+                     L.blk, L.nblk, L.xp, L.samples, L.r);
+  hipError_t e = launch_pair(PREC_FP32, L.r, k, L.m9, L.wf, L.wh, L.cus, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(judge_kernel, dim3((unsigned)((L.nblk + 3) / 4)), dim3(256), 0, st, L.r,
+                     (const float *)L.m9, L.nblk, L.samples, L.params, L.r_lo, L.r_hi, L.tol,
+                     L.flag, L.err);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_samples(int64_t n, float r_lo, float r_hi, uint64_t seed, float *r,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sur_samples_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     r_lo, r_hi, seed, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_targets(const float *r, const float *m9, int64_t n, float *t,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sur_targets_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r,
+                     m9, n, t);
   return hipGetLastError();
 }
 

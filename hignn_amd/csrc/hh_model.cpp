@@ -22,9 +22,12 @@
 
 #include "../../include/hh.h"
 #include "../../include/hh_dynamics.h"
+#include "../../include/hh_device.h"
+#include "../../include/hh_surrogate.h"
 #include "hh_host.h"
 #include "hh_internal.h"
 #include "hh_kernels.h"
+#include "hh_surrogate_internal.h"
 
 using namespace hh;
 
@@ -136,9 +139,20 @@ struct hh_model {
   // leave idle
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr, far0_ev = nullptr;
+  // far-field surrogate (borrowed; attached by hh_model_set_surrogate)
+  hh_surrogate *sur = nullptr;
+  double sur_tol = 2e-3;
+  int sur_samples = 32;
+  DBuf jd_blk, jd_r, jd_m9, jd_flag, jd_err;
+  std::vector<int32_t> sur_flag;  // per far block: 1 = surrogate entries
+  std::vector<float> sur_err;     // per far block: sampled relative error
+  // [judged, on surrogate, fell back, surrogate entries, full-MLP far entries]
+  int64_t sur_stats[5] = {0, 0, 0, 0, 0};
 
   ~hh_model() {
-    for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y}) b->release();
+    for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y, &jd_blk, &jd_r, &jd_m9,
+                    &jd_flag, &jd_err})
+      b->release();
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
                     &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
                     &active_r, &ybase_c, &ybase_r, &applylist, &Y, &Cpool,
@@ -482,6 +496,45 @@ struct hh_model {
     return far_pool_bytes;
   }
 
+  // The surrogate judge (hh_model_set_surrogate): per far block, the full
+  // FP32 MLP against the surrogate on sampled entries; sur_flag[b] = 1 for
+  // the blocks whose ACA takes surrogate entries.
+  void judge_surrogate(hipStream_t st) {
+    const int64_t nfar = (int64_t)far.size();
+    sur_flag.assign((size_t)nfar, 0);
+    sur_err.assign((size_t)nfar, 0.0f);
+    for (auto &v : sur_stats) v = 0;
+    if (!sur || nfar == 0) return;
+    const int S = sur_samples;
+    upload(jd_blk, far.data(), far.size() * sizeof(FarBlock));
+    jd_r.reserve((size_t)nfar * S * 12);
+    jd_m9.reserve((size_t)nfar * S * 36);
+    jd_flag.reserve((size_t)nfar * 4);
+    jd_err.reserve((size_t)nfar * 4);
+    JudgeLaunch L{};
+    L.blk = jd_blk.as<FarBlock>();
+    L.nblk = nfar;
+    L.xp = xp.as<float4>();
+    L.samples = S;
+    L.r = jd_r.as<float>();
+    L.m9 = jd_m9.as<float>();
+    L.params = sur->params.as<float>();
+    L.r_lo = sur->r_lo;
+    L.r_hi = sur->r_hi;
+    L.tol = sur_tol;
+    L.flag = jd_flag.as<int32_t>();
+    L.err = jd_err.as<float>();
+    L.wf = devw_f(false);
+    L.wh = devw_h(prec);
+    L.cus = g_cus;
+    HIPCHK(launch_judge(L, st));
+    HIPCHK(hipMemcpyAsync(sur_flag.data(), jd_flag.p, (size_t)nfar * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sur_err.data(), jd_err.p, (size_t)nfar * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    sur_stats[0] = nfar;
+    for (int32_t f : sur_flag) sur_stats[f ? 1 : 2]++;
+  }
+
   void far_field(hipStream_t st) {
     if (resolved_far_mode() == 0) {
       far_field_persistent(st);
@@ -489,6 +542,7 @@ struct hh_model {
     }
     const int64_t nfar = (int64_t)far.size();
     if (nfar == 0) return;
+    judge_surrogate(st);
     const bool f64 = prec == PREC_FP32;
     const bool c16 = !f64 && far_c16, q16 = !f64 && far_q16;
     const size_t escc = f64 ? 72 : (c16 ? 18 : 36), escq = f64 ? 72 : (q16 ? 18 : 36);
@@ -549,8 +603,14 @@ struct hh_model {
       h_ybase_r.reserve((size_t)nb * 8);
       h_stop.reserve((size_t)nb * 4);
       h_list.reserve((size_t)nb * 4);
-      std::vector<int32_t> act((size_t)nb);
-      for (int b = 0; b < nb; b++) act[(size_t)b] = b;
+      // active list: blocks on full-MLP entries first, then the surrogate's
+      // (the order survives the filtering below), so that each pass's tiles
+      // split into one full-MLP range and one surrogate range
+      std::vector<int32_t> act;
+      act.reserve((size_t)nb);
+      for (int pass = 0; pass < 2; pass++)
+        for (int b = 0; b < nb; b++)
+          if ((sur_flag[(size_t)(b0 + b)] != 0) == (pass == 1)) act.push_back(b);
       int iter = 0;
       auto last = std::chrono::steady_clock::now();
       while (!act.empty()) {
@@ -577,8 +637,11 @@ struct hh_model {
         int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
         int32_t *ha = h_active.as<int32_t>();
         int64_t sc = 0, sr = 0, ntc = 0, ntr = 0, ncc = 0, ncr = 0;
+        int nfull = 0;  // active blocks on full-MLP entries (the first nfull)
+        int64_t full_items = 0;
         for (int a = 0; a < na; a++) {
           const FarBlock &f = hb[(size_t)act[(size_t)a]];
+          if (!sur_flag[(size_t)(b0 + act[(size_t)a])]) nfull = a + 1, full_items += f.m + f.n;
           ha[a] = act[(size_t)a];
           yc[a] = sc;
           yr[a] = sr;
@@ -658,15 +721,23 @@ struct hh_model {
         sl.C = Cpool.p;
         sl.Q = Qpool.p;
         // column pass, residual, row pivot, normalisation
+        // tiles of the full-MLP blocks come first in each pass's tile list
+        const int64_t ntc_full = tc[nfull], ntr_full = tr[nfull];
+        AcaSurLaunch su{};
+        su.xp = xp.as<float4>();
+        su.Y = Y.as<float>();
+        if (sur) su.params = sur->params.as<float>(), su.inv_scale = 1.0f / sur->r_hi;
         ml.row_pass = 0;
         ml.ybase = ybase_c.as<int64_t>();
         ml.tfirst = tfirst_c.as<int32_t>();
         ml.nactive = na;
         ml.tiles = tiles_c.p;
-        ml.ntiles = ntc;
+        ml.ntiles = ntc_full;
         HIPCHK(hipEventRecord(evm[0], st));
         HIPCHK(launch_aca_mlp(ml, st));
         HIPCHK(hipEventRecord(evm[1], st));
+        su.tiles = tiles_c.p, su.tile0 = ntc_full, su.ntiles = ntc - ntc_full, su.row_pass = 0;
+        HIPCHK(launch_aca_sur(su, st));
         HIPCHK(launch_aca_steps(sl, 0, st));
         // row pass, residual, column pivot, norms, stop test
         ml.row_pass = 1;
@@ -674,10 +745,12 @@ struct hh_model {
         ml.tfirst = tfirst_r.as<int32_t>();
         ml.nactive = na;
         ml.tiles = tiles_r.p;
-        ml.ntiles = ntr;
+        ml.ntiles = ntr_full;
         HIPCHK(hipEventRecord(evm[2], st));
         HIPCHK(launch_aca_mlp(ml, st));
         HIPCHK(hipEventRecord(evm[3], st));
+        su.tiles = tiles_r.p, su.tile0 = ntr_full, su.ntiles = ntr - ntr_full, su.row_pass = 1;
+        HIPCHK(launch_aca_sur(su, st));
         HIPCHK(launch_aca_steps(sl, 1, st));
         HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -686,8 +759,10 @@ struct hh_model {
           HIPCHK(hipEventElapsedTime(&t0, evm[0], evm[1]));
           HIPCHK(hipEventElapsedTime(&t1, evm[2], evm[3]));
           prof_ms[1] += (double)t0 + (double)t1;
-          prof_pairs[1] += sc + sr;
+          prof_pairs[1] += full_items;
           prof_launches[1] += 2;
+          sur_stats[3] += sc + sr - full_items;
+          sur_stats[4] += full_items;
         }
         const int32_t *hs = h_stop.as<int32_t>();
         std::vector<int32_t> done, keep;
@@ -753,6 +828,7 @@ struct hh_model {
   }
 
   int resolved_far_mode() const {
+    if (sur) return 1;  // surrogate entries: batched iterations only
     return far_mode >= 0 ? far_mode : ((int64_t)far.size() <= (int64_t)g_cus ? 0 : 1);
   }
 
@@ -1247,6 +1323,61 @@ int hh_model_eval_pair(hh_model *m, const float *r, int64_t P, float *out9, int 
   });
 }
 
+int hh_model_surrogate_samples(hh_model *m, int64_t n, float r_lo, float r_hi, uint64_t seed,
+                               float *r_dev, float *t_dev, void *stream) {
+  return guarded([&]() {
+    if (!m || !r_dev || !t_dev) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n <= 0) throw HHError(HH_ERR_ARG, "need n > 0");
+    if (!(r_lo > 0) || !(r_hi > r_lo)) throw HHError(HH_ERR_ARG, "need 0 < r_lo < r_hi");
+    if (!m->has_weights) throw HHError(HH_ERR_STATE, "two-body model not loaded");
+    hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+    HIPCHK(launch_sur_samples(n, r_lo, r_hi, seed, r_dev, st));
+    // label in chunks through a bounded block buffer
+    const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 20);
+    DBuf m9;
+    m9.reserve(36 * (size_t)chunk);
+    try {
+      for (int64_t k0 = 0; k0 < n; k0 += chunk) {
+        const int64_t c = std::min(chunk, n - k0);
+        HIPCHK(launch_pair(PREC_FP32, r_dev + 3 * k0, c, m9.as<float>(), m->devw_f(false),
+                           m->devw_h(PREC_BF16), g_cus, st));
+        HIPCHK(launch_sur_targets(r_dev + 3 * k0, m9.as<float>(), c, t_dev + 6 * k0, st));
+      }
+      HIPCHK(hipStreamSynchronize(st));
+    } catch (...) {
+      m9.release();
+      throw;
+    }
+    m9.release();
+  });
+}
+
+int hh_model_set_surrogate(hh_model *m, hh_surrogate *s, double tol, int samples) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (s && !(tol >= 0)) throw HHError(HH_ERR_ARG, "tol must be >= 0");
+    if (s && (samples < 1 || samples > 4096)) throw HHError(HH_ERR_ARG, "samples must be in 1..4096");
+    m->sur = s;
+    if (s) m->sur_tol = tol, m->sur_samples = samples;
+  });
+}
+
+int hh_model_surrogate_stats(hh_model *m, int64_t stats[5]) {
+  return guarded([&]() {
+    if (!m || !stats) throw HHError(HH_ERR_ARG, "NULL argument");
+    for (int k = 0; k < 5; k++) stats[k] = m->sur_stats[k];
+  });
+}
+
+int hh_model_surrogate_errors(hh_model *m, float *err) {
+  return guarded([&]() {
+    if (!m || !err) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (m->sur_err.size() != m->far.size())
+      throw HHError(HH_ERR_STATE, "no judged matvec since the last tree build");
+    std::copy(m->sur_err.begin(), m->sur_err.end(), err);
+  });
+}
+
 struct hh_tree {
   TreeLists tl;
 };
@@ -1314,6 +1445,53 @@ int hh_model_last_timings(hh_model *m, double t[3]) {
     t[0] = m->t_close;
     t[1] = m->t_far;
     t[2] = m->t_total;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device memory (include/hh_device.h)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int hh_device_alloc(size_t bytes, void **p) {
+  return guarded([&]() {
+    if (!p) throw HHError(HH_ERR_ARG, "NULL argument");
+    *p = nullptr;
+    ensure_device();
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      throw HHError(HH_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+  });
+}
+
+int hh_device_free(void *p) {
+  return guarded([&]() {
+    if (p) HIPCHK(hipFree(p));
+  });
+}
+
+int hh_memcpy_to_device(void *dst_dev, const void *src_host, size_t bytes) {
+  return guarded([&]() {
+    if ((!dst_dev || !src_host) && bytes) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (bytes) HIPCHK(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int hh_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
+  return guarded([&]() {
+    if ((!dst_host || !src_dev) && bytes) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (bytes) HIPCHK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int hh_device_synchronize(void) {
+  return guarded([&]() {
+    ensure_device();
+    HIPCHK(hipDeviceSynchronize());
   });
 }
 

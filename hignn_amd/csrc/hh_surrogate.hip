@@ -1,11 +1,11 @@
-// Distilled far-field surrogate of the two-body pair function: a small net
-// s(r) ~ |r| M(r) with inputs (r/|r|, ln(|r| / r0)), trained on the GPU by
-// the kernels below (forward, backward, Adam) on pairs labelled by the full
-// MLP.  4 -> 32 -> 32 -> 9, tanh hidden layers, linear output; mean squared
-// error over the 9 outputs.
+// Distilled far-field surrogate of the two-body pair function (the net is
+// in hh_surrogate_net.h): training kernels and the C ABI of
+// include/hh_surrogate.h.
 //
-// Parameter vector (1,513 floats): W1 [32][4], b1 [32], W2 [32][32], b2 [32],
-// W3 [9][32], b3 [9].
+// Training is minibatch Adam on samples labelled by the full MLP: each step
+// runs one gradient pass over its batch (forward, backward and per-block
+// weight-gradient sums, no atomics) and one reduction that sums the block
+// partials in block order, so the gradient does not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,159 +16,153 @@
 #include "../../include/hh.h"
 #include "../../include/hh_surrogate.h"
 #include "hh_host.h"
+#include "hh_surrogate_internal.h"
+#include "hh_surrogate_net.h"
 
 namespace hh {
 namespace {
 
-constexpr int kIn = 4, kH = 32, kOut = 9;
-constexpr int kOW1 = 0, kOB1 = kOW1 + kH * kIn, kOW2 = kOB1 + kH, kOB2 = kOW2 + kH * kH,
-              kOW3 = kOB2 + kH, kOB3 = kOW3 + kOut * kH, kNP = kOB3 + kOut;
+using namespace sur;
+
 static_assert(kNP == HH_SURROGATE_PARAMS, "parameter count");
-constexpr int kTB = 128;  // samples (threads) per training block
-
-__device__ __forceinline__ void features(const float *r, float r0, float (&x)[kIn]) {
-  const float d = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-  const float inv = d > 0.0f ? 1.0f / d : 0.0f;
-  x[0] = r[0] * inv;
-  x[1] = r[1] * inv;
-  x[2] = r[2] * inv;
-  x[3] = logf(fmaxf(d, 1e-30f) / r0);
-}
-
-// Forward pass of one sample; a1, a2: hidden activations.
-__device__ __forceinline__ void forward(const float *P, const float (&x)[kIn], float (&a1)[kH],
-                                        float (&a2)[kH], float (&y)[kOut]) {
-#pragma unroll
-  for (int j = 0; j < kH; j++) {
-    float z = P[kOB1 + j];
-#pragma unroll
-    for (int k = 0; k < kIn; k++) z = fmaf(P[kOW1 + j * kIn + k], x[k], z);
-    a1[j] = tanhf(z);
-  }
-#pragma unroll
-  for (int i = 0; i < kH; i++) {
-    float z = P[kOB2 + i];
-#pragma unroll
-    for (int j = 0; j < kH; j++) z = fmaf(P[kOW2 + i * kH + j], a1[j], z);
-    a2[i] = tanhf(z);
-  }
-#pragma unroll
-  for (int o = 0; o < kOut; o++) {
-    float z = P[kOB3 + o];
-#pragma unroll
-    for (int i = 0; i < kH; i++) z = fmaf(P[kOW3 + o * kH + i], a2[i], z);
-    y[o] = z;
-  }
-}
+constexpr int kTB = 128;         // samples per chunk of a gradient block
+constexpr int kMaxGradBlocks = 1024;
+constexpr int kPartStride = kNP + 1;  // per block: gradient sums, then the loss sum
 
 __global__ __launch_bounds__(256) void surr_eval_kernel(const float *params, const float *r,
-                                                        int64_t n, float r0, float *out) {
+                                                        int64_t n, float inv_scale, float *out) {
   __shared__ float P[kNP];
-  for (int i = threadIdx.x; i < kNP; i += blockDim.x) P[i] = params[i];
-  __syncthreads();
+  stage(params, P);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  float x[kIn], a1[kH], a2[kH], y[kOut];
-  features(r + 3 * s, r0, x);
-  forward(P, x, a1, a2, y);
+  float y[kOut];
+  eval(P, r[3 * s], r[3 * s + 1], r[3 * s + 2], inv_scale, y);
 #pragma unroll
   for (int o = 0; o < kOut; o++) out[kOut * s + o] = y[o];
 }
 
-// Gradient of the loss L = (1 / (9 n)) sum_s |y_s - t_s|^2 over the n samples,
-// accumulated into grad (zeroed by the caller), and L into loss[0].  One
-// sample per thread; the block's activations and output errors go through
-// LDS, each thread sums a share of the weight gradients over the block's
-// samples and adds them to grad once.
+// Gradient partials of L = (1/n) sum_s sum_k w_k (y_sk - t_sk)^2 over samples
+// [s0, s0 + count): block b takes chunks b, b + gridDim.x, ... of kTB
+// samples (one per thread), keeps each chunk's activations and output
+// errors in LDS, and each thread adds its share of the parameters' sums over
+// the chunk to registers.  Block b writes part[b * kPartStride + p] (p <
+// kNP) and its loss sum at kNP.  scale = 1/n of the whole loss.
 __global__ __launch_bounds__(kTB) void surr_grad_kernel(const float *params, const float *r,
-                                                        const float *t, int64_t n, float r0,
-                                                        float *grad, float *loss) {
+                                                        const float *t, int64_t s0, int64_t count,
+                                                        float inv_scale, float scale, float *part) {
   __shared__ float P[kNP];
-  __shared__ float sx[kTB][kIn], sa1[kTB][kH + 1], sd1[kTB][kH + 1], sa2[kTB][kH + 1],
+  __shared__ float sz[kTB][kIn], sa1[kTB][kH + 1], sd1[kTB][kH + 1], sa2[kTB][kH + 1],
       sd2[kTB][kH + 1], sdy[kTB][kOut];
   __shared__ float sl[kTB];
-  for (int i = threadIdx.x; i < kNP; i += blockDim.x) P[i] = params[i];
-  __syncthreads();
+  stage(params, P);
   const int tid = threadIdx.x;
-  const int64_t s = (int64_t)blockIdx.x * kTB + tid;
-  const float scale = 2.0f / (9.0f * (float)n);
-  float x[kIn] = {0, 0, 0, 0}, a1[kH], a2[kH], y[kOut], dy[kOut];
-  float l = 0.0f;
-  if (s < n) {
-    features(r + 3 * s, r0, x);
-    forward(P, x, a1, a2, y);
+  constexpr int kPer = (kNP + kTB - 1) / kTB;  // parameters per thread
+  float acc[kPer];
 #pragma unroll
-    for (int o = 0; o < kOut; o++) {
-      const float e = y[o] - t[kOut * s + o];
-      l = fmaf(e, e, l);
-      dy[o] = scale * e;
+  for (int q = 0; q < kPer; q++) acc[q] = 0.0f;
+  float lsum = 0.0f;
+  const int64_t nchunks = (count + kTB - 1) / kTB;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int64_t s = c * kTB + tid;
+    float z[kIn] = {0, 0, 0}, x1[kH], x2[kH], y[kOut], dy[kOut];
+    float l = 0.0f;
+    const bool valid = s < count;
+    if (valid) {
+      const int64_t g = s0 + s;
+#pragma unroll
+      for (int k = 0; k < kIn; k++) z[k] = r[3 * g + k] * inv_scale;
+      forward(P, z, x1, x2, y);
+#pragma unroll
+      for (int o = 0; o < kOut; o++) {
+        const float e = y[o] - t[kOut * g + o];
+        const float we = entry_weight(o) * e;
+        l = __builtin_fmaf(we, e, l);
+        dy[o] = 2.0f * scale * we;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kH; j++) x1[j] = x2[j] = 0.0f;
+#pragma unroll
+      for (int o = 0; o < kOut; o++) dy[o] = 0.0f;
     }
-  } else {
+    // back through layers 3 and 2 (zero errors for padded samples)
+    float d2[kH], d1[kH];
 #pragma unroll
-    for (int j = 0; j < kH; j++) a1[j] = a2[j] = 0.0f;
+    for (int i = 0; i < kH; i++) {
+      float gsum = 0.0f;
 #pragma unroll
-    for (int o = 0; o < kOut; o++) dy[o] = 0.0f;
-  }
-  // back through layer 3 and 2
-  float d2[kH], d1[kH];
-#pragma unroll
-  for (int i = 0; i < kH; i++) {
-    float g = 0.0f;
-#pragma unroll
-    for (int o = 0; o < kOut; o++) g = fmaf(P[kOW3 + o * kH + i], dy[o], g);
-    d2[i] = g * (1.0f - a2[i] * a2[i]);
-  }
-#pragma unroll
-  for (int j = 0; j < kH; j++) {
-    float g = 0.0f;
-#pragma unroll
-    for (int i = 0; i < kH; i++) g = fmaf(P[kOW2 + i * kH + j], d2[i], g);
-    d1[j] = g * (1.0f - a1[j] * a1[j]);
-  }
-#pragma unroll
-  for (int k = 0; k < kIn; k++) sx[tid][k] = x[k];
-#pragma unroll
-  for (int j = 0; j < kH; j++) sa1[tid][j] = a1[j], sd1[tid][j] = d1[j];
-#pragma unroll
-  for (int i = 0; i < kH; i++) sa2[tid][i] = a2[i], sd2[tid][i] = d2[i];
-#pragma unroll
-  for (int o = 0; o < kOut; o++) sdy[tid][o] = dy[o];
-  sl[tid] = l;
-  __syncthreads();
-  // weight gradients: parameter p summed over the block's samples
-  for (int p = tid; p < kNP; p += kTB) {
-    float g = 0.0f;
-    if (p < kOB1) {  // W1[j][k]
-      const int j = p / kIn, k = p % kIn;
-      for (int q = 0; q < kTB; q++) g = fmaf(sd1[q][j], sx[q][k], g);
-    } else if (p < kOW2) {  // b1[j]
-      const int j = p - kOB1;
-      for (int q = 0; q < kTB; q++) g += sd1[q][j];
-    } else if (p < kOB2) {  // W2[i][j]
-      const int i = (p - kOW2) / kH, j = (p - kOW2) % kH;
-      for (int q = 0; q < kTB; q++) g = fmaf(sd2[q][i], sa1[q][j], g);
-    } else if (p < kOW3) {  // b2[i]
-      const int i = p - kOB2;
-      for (int q = 0; q < kTB; q++) g += sd2[q][i];
-    } else if (p < kOB3) {  // W3[o][i]
-      const int o = (p - kOW3) / kH, i = (p - kOW3) % kH;
-      for (int q = 0; q < kTB; q++) g = fmaf(sdy[q][o], sa2[q][i], g);
-    } else {  // b3[o]
-      const int o = p - kOB3;
-      for (int q = 0; q < kTB; q++) g += sdy[q][o];
+      for (int o = 0; o < kOut; o++) gsum = __builtin_fmaf(P[kOW3 + o * kH + i], dy[o], gsum);
+      d2[i] = gsum * dsilu(x2[i]);
     }
-    atomicAdd(&grad[p], g);
+#pragma unroll
+    for (int j = 0; j < kH; j++) {
+      float gsum = 0.0f;
+#pragma unroll
+      for (int i = 0; i < kH; i++) gsum = __builtin_fmaf(P[kOW2 + i * kH + j], d2[i], gsum);
+      d1[j] = gsum * dsilu(x1[j]);
+    }
+    __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+    for (int k = 0; k < kIn; k++) sz[tid][k] = z[k];
+#pragma unroll
+    for (int j = 0; j < kH; j++) sa1[tid][j] = valid ? silu(x1[j]) : 0.0f, sd1[tid][j] = d1[j];
+#pragma unroll
+    for (int i = 0; i < kH; i++) sa2[tid][i] = valid ? silu(x2[i]) : 0.0f, sd2[tid][i] = d2[i];
+#pragma unroll
+    for (int o = 0; o < kOut; o++) sdy[tid][o] = dy[o];
+    sl[tid] = l;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int p = tid + q * kTB;
+      if (p >= kNP) break;
+      float g = 0.0f;
+      if (p < kOB1) {  // W1[j][k]
+        const int j = p / kIn, k = p % kIn;
+        for (int u = 0; u < kTB; u++) g = __builtin_fmaf(sd1[u][j], sz[u][k], g);
+      } else if (p < kOW2) {  // b1[j]
+        const int j = p - kOB1;
+        for (int u = 0; u < kTB; u++) g += sd1[u][j];
+      } else if (p < kOB2) {  // W2[i][j]
+        const int i = (p - kOW2) / kH, j = (p - kOW2) % kH;
+        for (int u = 0; u < kTB; u++) g = __builtin_fmaf(sd2[u][i], sa1[u][j], g);
+      } else if (p < kOW3) {  // b2[i]
+        const int i = p - kOB2;
+        for (int u = 0; u < kTB; u++) g += sd2[u][i];
+      } else if (p < kOB3) {  // W3[o][i]
+        const int o = (p - kOW3) / kH, i = (p - kOW3) % kH;
+        for (int u = 0; u < kTB; u++) g = __builtin_fmaf(sdy[u][o], sa2[u][i], g);
+      } else {  // b3[o]
+        const int o = p - kOB3;
+        for (int u = 0; u < kTB; u++) g += sdy[u][o];
+      }
+      acc[q] += g;
+    }
+    if (tid == 0) {
+      float b = 0.0f;
+      for (int u = 0; u < kTB; u++) b += sl[u];
+      lsum += b;
+    }
   }
-  if (tid == 0) {
-    float b = 0.0f;
-    for (int q = 0; q < kTB; q++) b += sl[q];
-    atomicAdd(loss, b / (9.0f * (float)n));
+  float *dst = part + (size_t)blockIdx.x * kPartStride;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int p = tid + q * kTB;
+    if (p < kNP) dst[p] = acc[q];
   }
+  if (tid == 0) dst[kNP] = lsum * scale;
 }
 
-__global__ void surr_zero_kernel(float *g, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) g[i] = 0.0f;
+// grad[p] = sum over blocks of part[b][p], in block order; loss likewise.
+__global__ void surr_reduce_kernel(const float *part, int nblocks, float *grad, float *loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kNP) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblocks; b++) s += part[(size_t)b * kPartStride + p];
+  if (p < kNP)
+    grad[p] = s;
+  else
+    loss[0] = s;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction; step is 1-based.
@@ -187,28 +181,39 @@ __global__ void surr_adam_kernel(float *params, const float *grad, float *m, flo
 }
 
 }  // namespace
+
+int surrogate_grad_blocks(int64_t n) {
+  return (int)std::min<int64_t>((n + kTB - 1) / kTB, kMaxGradBlocks);
+}
+
+// Gradient of the loss on samples [s0, s0 + count) (scale = 1/n of the
+// loss being differentiated) into grad_dev / loss_dev, with part holding
+// surrogate_grad_blocks(count) * (HH_SURROGATE_PARAMS + 1) floats.
+void surrogate_gradient(const float *params, const float *r, const float *t, int64_t s0,
+                        int64_t count, float inv_scale, float scale, float *part, float *grad_dev,
+                        float *loss_dev, hipStream_t st) {
+  const int nb = surrogate_grad_blocks(count);
+  hipLaunchKernelGGL(surr_grad_kernel, dim3(nb), dim3(kTB), 0, st, params, r, t, s0, count,
+                     inv_scale, scale, part);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(surr_reduce_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, st,
+                     (const float *)part, nb, grad_dev, loss_dev);
+  HIPCHK(hipGetLastError());
+}
+
 }  // namespace hh
 
 using namespace hh;
 
-struct hh_surrogate {
-  float r0 = 1.0f;
-  DBuf params, grad, m, v, loss;
-  int step = 0;
-  hipStream_t st = nullptr;
-  ~hh_surrogate() {
-    for (DBuf *b : {&params, &grad, &m, &v, &loss}) b->release();
-    if (st) (void)hipStreamDestroy(st);
-  }
-  void run_grad(const float *r, const float *t, int64_t n) {
-    hipLaunchKernelGGL(surr_zero_kernel, dim3((kNP + 255) / 256), dim3(256), 0, st,
-                       grad.as<float>(), kNP);
-    hipLaunchKernelGGL(surr_zero_kernel, dim3(1), dim3(64), 0, st, loss.as<float>(), 1);
-    hipLaunchKernelGGL(surr_grad_kernel, dim3((unsigned)((n + kTB - 1) / kTB)), dim3(kTB), 0, st,
-                       params.as<float>(), r, t, n, r0, grad.as<float>(), loss.as<float>());
-    HIPCHK(hipGetLastError());
-  }
-};
+hh_surrogate::~hh_surrogate() {
+  for (DBuf *b : {&params, &grad, &m, &v, &loss, &part}) b->release();
+}
+
+void hh_surrogate::eval(const float *r, int64_t n, float *out, hipStream_t st) {
+  hipLaunchKernelGGL(surr_eval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     params.as<float>(), r, n, 1.0f / r_hi, out);
+  HIPCHK(hipGetLastError());
+}
 
 namespace {
 hh_surrogate *need(hh_surrogate *s) {
@@ -218,19 +223,21 @@ hh_surrogate *need(hh_surrogate *s) {
 void check_n(int64_t n) {
   if (n <= 0 || n > ((int64_t)1 << 31) - 1) throw HHError(HH_ERR_ARG, "sample count out of range");
 }
+hipStream_t stream_of(void *stream) { return (hipStream_t)stream; }
 }  // namespace
 
 extern "C" {
 
-int hh_surrogate_create(float r0, hh_surrogate **out) {
+int hh_surrogate_create(float r_lo, float r_hi, hh_surrogate **out) {
   return guarded([&]() {
     if (!out) throw HHError(HH_ERR_ARG, "NULL argument");
-    if (!(r0 > 0)) throw HHError(HH_ERR_ARG, "r0 must be positive");
+    *out = nullptr;
+    if (!(r_lo > 0) || !(r_hi > r_lo)) throw HHError(HH_ERR_ARG, "need 0 < r_lo < r_hi");
     ensure_device();
     hh_surrogate *s = new hh_surrogate();
-    s->r0 = r0;
+    s->r_lo = r_lo;
+    s->r_hi = r_hi;
     try {
-      HIPCHK(hipStreamCreate(&s->st));
       for (DBuf *b : {&s->params, &s->grad, &s->m, &s->v}) {
         b->reserve(4 * kNP);
         HIPCHK(hipMemset(b->p, 0, 4 * kNP));  // all-zero net and Adam state
@@ -247,10 +254,19 @@ int hh_surrogate_create(float r0, hh_surrogate **out) {
 
 void hh_surrogate_destroy(hh_surrogate *s) { delete s; }
 
+int hh_surrogate_range(hh_surrogate *s, float *r_lo, float *r_hi) {
+  return guarded([&]() {
+    need(s);
+    if (r_lo) *r_lo = s->r_lo;
+    if (r_hi) *r_hi = s->r_hi;
+  });
+}
+
 int hh_surrogate_set_params(hh_surrogate *s, const float *params) {
   return guarded([&]() {
     if (!params) throw HHError(HH_ERR_ARG, "NULL argument");
     need(s);
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(s->params.p, params, 4 * kNP, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(s->m.p, 0, 4 * kNP));
     HIPCHK(hipMemset(s->v.p, 0, 4 * kNP));
@@ -261,53 +277,74 @@ int hh_surrogate_set_params(hh_surrogate *s, const float *params) {
 int hh_surrogate_get_params(hh_surrogate *s, float *params) {
   return guarded([&]() {
     if (!params) throw HHError(HH_ERR_ARG, "NULL argument");
-    HIPCHK(hipStreamSynchronize(need(s)->st));
+    need(s);
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(params, s->params.p, 4 * kNP, hipMemcpyDeviceToHost));
   });
 }
 
-int hh_surrogate_eval_device(hh_surrogate *s, const float *r_dev, int64_t n, float *out_dev) {
+int hh_surrogate_eval_device(hh_surrogate *s, const float *r_dev, int64_t n, float *out_dev,
+                             void *stream) {
   return guarded([&]() {
     if (!r_dev || !out_dev) throw HHError(HH_ERR_ARG, "NULL argument");
     need(s);
     check_n(n);
-    hipLaunchKernelGGL(surr_eval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->st,
-                       s->params.as<float>(), r_dev, n, s->r0, out_dev);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->st));
+    s->eval(r_dev, n, out_dev, stream_of(stream));
+    HIPCHK(hipStreamSynchronize(stream_of(stream)));
   });
 }
 
 int hh_surrogate_gradient_device(hh_surrogate *s, const float *r_dev, const float *t_dev,
-                                 int64_t n, float *grad, float *loss) {
+                                 int64_t n, float *grad, float *loss, void *stream) {
   return guarded([&]() {
     if (!r_dev || !t_dev) throw HHError(HH_ERR_ARG, "NULL argument");
     need(s);
     check_n(n);
-    s->run_grad(r_dev, t_dev, n);
-    HIPCHK(hipStreamSynchronize(s->st));
+    hipStream_t st = stream_of(stream);
+    s->part.reserve((size_t)surrogate_grad_blocks(n) * kPartStride * 4);
+    surrogate_gradient(s->params.as<float>(), r_dev, t_dev, 0, n, 1.0f / s->r_hi,
+                       1.0f / (float)n, s->part.as<float>(), s->grad.as<float>(),
+                       s->loss.as<float>(), st);
+    HIPCHK(hipStreamSynchronize(st));
     if (grad) HIPCHK(hipMemcpy(grad, s->grad.p, 4 * kNP, hipMemcpyDeviceToHost));
     if (loss) HIPCHK(hipMemcpy(loss, s->loss.p, 4, hipMemcpyDeviceToHost));
   });
 }
 
 int hh_surrogate_train_device(hh_surrogate *s, const float *r_dev, const float *t_dev, int64_t n,
-                              int steps, float lr, float *final_loss) {
+                              int steps, int64_t batch, float lr0, float lr1, float *final_loss,
+                              void *stream) {
   return guarded([&]() {
     if (!r_dev || !t_dev) throw HHError(HH_ERR_ARG, "NULL argument");
     need(s);
     check_n(n);
-    if (steps < 0 || !(lr > 0)) throw HHError(HH_ERR_ARG, "steps >= 0 and lr > 0 required");
-    if (steps == 0) s->run_grad(r_dev, t_dev, n);  // the loss at the current parameters
+    if (steps < 0 || !(lr0 > 0) || !(lr1 > 0)) throw HHError(HH_ERR_ARG, "steps >= 0 and lr > 0 required");
+    if (batch <= 0 || batch > n) batch = n;
+    hipStream_t st = stream_of(stream);
+    s->part.reserve((size_t)surrogate_grad_blocks(batch) * kPartStride * 4);
+    const float inv_scale = 1.0f / s->r_hi;
+    if (steps == 0)  // the loss at the current parameters over all samples
+      surrogate_gradient(s->params.as<float>(), r_dev, t_dev, 0, n, inv_scale, 1.0f / (float)n,
+                         s->part.as<float>(), s->grad.as<float>(), s->loss.as<float>(), st);
+    int64_t off = 0;
     for (int k = 0; k < steps; k++) {
-      s->run_grad(r_dev, t_dev, n);
+      // consecutive batches, wrapping round the samples
+      if (off + batch > n) off = 0;
+      surrogate_gradient(s->params.as<float>(), r_dev, t_dev, off, batch, inv_scale,
+                         1.0f / (float)batch, s->part.as<float>(), s->grad.as<float>(),
+                         s->loss.as<float>(), st);
+      off += batch;
       s->step++;
-      hipLaunchKernelGGL(surr_adam_kernel, dim3((kNP + 255) / 256), dim3(256), 0, s->st,
-                         s->params.as<float>(), s->grad.as<float>(), s->m.as<float>(),
-                         s->v.as<float>(), kNP, lr, 0.9f, 0.999f, 1e-8f, s->step);
+      // cosine schedule from lr0 to lr1 over the call's steps
+      const double c = steps > 1 ? 0.5 * (1.0 + std::cos(M_PI * (double)k / (double)(steps - 1))) : 1.0;
+      const float lr = (float)(lr1 + (lr0 - lr1) * c);
+      hipLaunchKernelGGL(surr_adam_kernel, dim3((kNP + 255) / 256), dim3(256), 0, st,
+                         s->params.as<float>(), (const float *)s->grad.as<float>(),
+                         s->m.as<float>(), s->v.as<float>(), kNP, lr, 0.9f, 0.999f, 1e-8f,
+                         s->step);
       HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipStreamSynchronize(st));
     if (final_loss) HIPCHK(hipMemcpy(final_loss, s->loss.p, 4, hipMemcpyDeviceToHost));
   });
 }

@@ -66,8 +66,8 @@ class Model {
 
   // load_two_body_model(base): base_<device>.pt or base.pt (TorchScript, as
   // src/HignnModel.cpp:404-411), base.pkl (torch-saved Sequential), base.npy,
-  // read without unpickling; or "xavier:<seed>" for the seeded
-  // initialisation of the same layer shapes.
+  // read without unpickling; or "xavier:<seed>" / "smooth:<seed>" for the
+  // seeded weight sets of the same layer shapes (hignn_amd/weights.py).
   void load_two_body_model(const std::string &path) {
     py::object W = py::module_::import("hignn_amd.weights");
     py::array_t<float, py::array::forcecast | py::array::c_style> w;

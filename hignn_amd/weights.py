@@ -83,6 +83,29 @@ def xavier(seed: int = 0, shapes=None) -> np.ndarray:
     return pack(parts, shapes)
 
 
+def smooth(seed: int = 0, w1_scale: float = 0.01) -> np.ndarray:
+    """A smooth pair function of the same shapes: ``xavier(seed)`` with the
+    layer-1 weights W1 multiplied by ``w1_scale`` (biases unchanged).
+
+    With Xavier's W1 (|W1 r| ~ 0.2 |r|) the first tanh layer saturates a few
+    radii from the origin, so the pair function is close to a piecewise
+    constant function of the direction of r, cut by the 128 planes of W1,
+    divided by |r|: far blocks are far from low rank and no small net follows
+    it.  At w1_scale = 0.01 the first layer varies on the scale of a few
+    hundred radii (the size of a 10^6-particle suspension), so (y + I)/|r| is
+    a smooth function divided by |r|: far blocks keep the low ranks of an
+    Oseen-like kernel (0.5-0.9x those of the RPY tensor at the same
+    separation in the tolerance range 1e-4..1e-3), and the ACA tolerance test
+    ends the iterations.  Not a physical mobility: a seeded stand-in for a
+    trained net, whose far field is smooth in the same way.
+    """
+    w = xavier(seed)
+    p = unpack(w)
+    p = {k: v.copy() for k, v in p.items()}
+    p["W1"] = (p["W1"].astype(np.float64) * w1_scale).astype(np.float32)
+    return pack(p)
+
+
 class _Ref:
     """A global (module, name) named by the opcode stream; never imported."""
 
@@ -286,6 +309,12 @@ def resolve_model_path(base: str, device: int | None = None) -> str:
 
 
 def load(base: str) -> np.ndarray:
+    """Weights of load_two_body_model(base); also "xavier:<seed>" and
+    "smooth:<seed>" for the seeded weight sets above."""
+    if base.startswith("xavier:"):
+        return xavier(int(base.split(":", 1)[1]))
+    if base.startswith("smooth:"):
+        return smooth(int(base.split(":", 1)[1]))
     path = resolve_model_path(base)
     if path.endswith(".npy"):
         return pack(unpack(np.load(path, allow_pickle=False)))

@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# libhh.so and PyTorch each bring a HIP runtime of the same soname, and a
+# process keeps the copy it loads first; PyTorch only initialises its
+# devices on its own.  Tests that capture graphs through PyTorch therefore
+# import it before any test loads libhh.so.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -14,8 +23,7 @@ def pytest_configure(config):
 
 def gpu_available():
     try:
-        import torch
-        return torch.cuda.is_available()
+        return torch is not None and torch.cuda.is_available()
     except Exception:
         return False
 
