@@ -57,6 +57,7 @@ def lib():
             "hh_surrogate_get_params": [c_vp, c_vp],
             "hh_surrogate_eval_device": [c_vp, c_vp, c_i64, c_vp, c_vp],
             "hh_surrogate_gradient_device": [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+            "hh_surrogate_gradient_noalloc": [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
             "hh_surrogate_train_device": [c_vp, c_vp, c_vp, c_i64, c_int, c_i64, ctypes.c_float,
                                           ctypes.c_float, c_vp, c_vp],
             "hh_model_surrogate_samples": [c_vp, c_i64, ctypes.c_float, ctypes.c_float,
@@ -369,6 +370,12 @@ class Surrogate:
         _chk(lib().hh_surrogate_gradient_device(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n), _p(g),
                                                 _p(loss), c_vp(stream)))
         return float(loss[0]), g
+
+    def gradient_noalloc(self, r_ptr, t_ptr, n, grad_ptr, loss_ptr, stream=None):
+        """Enqueues the gradient into caller-owned device outputs; allocates
+        nothing (the library's fixed workspace holds the block partials)."""
+        _chk(lib().hh_surrogate_gradient_noalloc(self.h, c_vp(r_ptr), c_vp(t_ptr), int(n),
+                                                 c_vp(grad_ptr), c_vp(loss_ptr), c_vp(stream)))
 
     def train(self, r_ptr, t_ptr, n, steps, lr, batch=0, lr_final=None, stream=None):
         loss = np.zeros(1, np.float32)

@@ -1071,7 +1071,10 @@ extern "C" {
 const char *hh_last_error(void) { return g_last_error.c_str(); }
 
 int hh_init(void) {
-  return guarded([&]() { ensure_device(); });
+  return guarded([&]() {
+    ensure_device();
+    (void)scratch_ptr();  // the no-allocation gradient's workspace address
+  });
 }
 
 int hh_finalize(void) {

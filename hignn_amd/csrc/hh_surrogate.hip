@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,8 +27,15 @@ using namespace sur;
 
 static_assert(kNP == HH_SURROGATE_PARAMS, "parameter count");
 constexpr int kTB = 128;         // samples per chunk of a gradient block
-constexpr int kMaxGradBlocks = 1024;
+constexpr int kMaxGradBlocks = 512;
 constexpr int kPartStride = kNP + 1;  // per block: gradient sums, then the loss sum
+
+// Workspace of hh_surrogate_gradient_noalloc: the gradient pass's per-block
+// partial sums (512 x 1,383 floats = 2.8 MB), held by every GPU the library
+// runs on until the process ends.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the
+// device address from hipGetSymbolAddress, one per device (scratch_ptr).
+static __device__ __align__(256) float scratch[kMaxGradBlocks * kPartStride];
 
 __global__ __launch_bounds__(256) void surr_eval_kernel(const float *params, const float *r,
                                                         int64_t n, float inv_scale, float *out) {
@@ -201,6 +209,26 @@ void surrogate_gradient(const float *params, const float *r, const float *t, int
   HIPCHK(hipGetLastError());
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+constexpr int kMaxDevices = 64;
+static float *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+float *scratch_ptr() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw HHError(HH_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HHError(HH_ERR_DEVICE, "no device address for the gradient workspace");
+    g_scratch[dev] = (float *)p;
+  }
+  return g_scratch[dev];
+}
+
 }  // namespace hh
 
 using namespace hh;
@@ -308,6 +336,23 @@ int hh_surrogate_gradient_device(hh_surrogate *s, const float *r_dev, const floa
     HIPCHK(hipStreamSynchronize(st));
     if (grad) HIPCHK(hipMemcpy(grad, s->grad.p, 4 * kNP, hipMemcpyDeviceToHost));
     if (loss) HIPCHK(hipMemcpy(loss, s->loss.p, 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int hh_surrogate_gradient_noalloc(hh_surrogate *s, const float *r_dev, const float *t_dev,
+                                  int64_t n, float *grad_dev, float *loss_dev, void *stream) {
+  return guarded([&]() {
+    if (!r_dev || !t_dev || !grad_dev || !loss_dev) throw HHError(HH_ERR_ARG, "NULL argument");
+    need(s);
+    if (n <= 0 || n > HH_SURROGATE_NOALLOC_MAX_SAMPLES)
+      throw HHError(HH_ERR_ARG, "sample count must be in 1.." +
+                                    std::to_string((long long)HH_SURROGATE_NOALLOC_MAX_SAMPLES));
+    // the partials of at most kMaxGradBlocks blocks fit the workspace
+    static_assert(sizeof(scratch) == (size_t)kMaxGradBlocks * kPartStride * sizeof(float),
+                  "workspace size");
+    if (surrogate_grad_blocks(n) > kMaxGradBlocks) throw HHError(HH_ERR_ARG, "workspace too small");
+    surrogate_gradient(s->params.as<float>(), r_dev, t_dev, 0, n, 1.0f / s->r_hi, 1.0f / (float)n,
+                       scratch_ptr(), grad_dev, loss_dev, stream_of(stream));
   });
 }
 

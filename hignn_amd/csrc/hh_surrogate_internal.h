@@ -19,6 +19,9 @@ struct hh_surrogate {
 };
 
 namespace hh {
+// Device address of the no-allocation gradient's workspace on the current
+// device (looked up once per device).
+float *scratch_ptr();
 int surrogate_grad_blocks(int64_t n);
 void surrogate_gradient(const float *params, const float *r, const float *t, int64_t s0,
                         int64_t count, float inv_scale, float scale, float *part, float *grad_dev,

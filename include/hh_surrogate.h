@@ -57,6 +57,21 @@ int hh_surrogate_eval_device(hh_surrogate *s, const float *r_dev, int64_t n, flo
 int hh_surrogate_gradient_device(hh_surrogate *s, const float *r_dev, const float *t_dev,
                                  int64_t n, float *grad, float *loss, void *stream);
 
+/* As hh_surrogate_gradient_device, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training loops).  grad_dev (HH_SURROGATE_PARAMS
+ * floats) and loss_dev (1 float) are caller-owned device outputs; the
+ * gradient pass's per-block partial sums live in a fixed 2.8 MB device
+ * workspace of the library (512 blocks x (HH_SURROGATE_PARAMS + 1) floats,
+ * one per GPU, kept until the process ends).  Every call on a GPU shares
+ * that workspace: calls that overlap on one GPU (different streams or
+ * threads) race.  The work is enqueued on stream and not waited for; the
+ * result equals hh_surrogate_gradient_device's bit for bit.  n must be in
+ * 1..HH_SURROGATE_NOALLOC_MAX_SAMPLES. */
+#define HH_SURROGATE_NOALLOC_MAX_SAMPLES 2147483647LL
+int hh_surrogate_gradient_noalloc(hh_surrogate *s, const float *r_dev, const float *t_dev,
+                                  int64_t n, float *grad_dev, float *loss_dev, void *stream);
+
 /* steps minibatch Adam steps (beta 0.9 / 0.999, eps 1e-8) on consecutive
  * batches of `batch` samples (wrapping round; batch <= 0 or > n: all n), the
  * learning rate following a cosine from lr0 to lr1 over the call's steps.
