@@ -70,6 +70,7 @@ def lib():
             "hh_memcpy_to_device": [c_vp, c_vp, ctypes.c_size_t],
             "hh_memcpy_to_host": [c_vp, c_vp, ctypes.c_size_t],
             "hh_device_synchronize": [],
+            "hh_device_copy": [c_vp, c_vp, ctypes.c_size_t],
             "hh_model_tree_counts": [c_vp, c_vp],
             "hh_model_tree_export": [c_vp] * 7,
             "hh_model_far_ranks": [c_vp, c_vp],

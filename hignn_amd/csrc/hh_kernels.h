@@ -178,6 +178,8 @@ hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st);
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st);
 hipError_t launch_pair(int prec, const float *r, int64_t P, float *out, DevWeightsF wf,
                        DevWeightsH wh, int cus, hipStream_t st);
+// Copies n16 16-byte vectors with a kernel.
+hipError_t launch_copy16(void *dst, const void *src, int64_t n16, hipStream_t st);
 // Clears n doubles with a kernel (usable inside captured graphs).
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t st);
 hipError_t launch_gather_forces(const float *f, int64_t n, const int32_t *perm, float4 *fp,

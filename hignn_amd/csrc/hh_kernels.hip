@@ -25,6 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include <cstdint>
@@ -2199,6 +2200,20 @@ hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st) {
     else
       hipLaunchKernelGGL(persistent_aca_h_kernel<f16x8>, dim3(L.nslots), dim3(256), lds, st, A, W);
   }
+  return hipGetLastError();
+}
+
+__global__ void copy16_kernel(uint4 *dst, const uint4 *src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy16(void *dst, const void *src, int64_t n16, hipStream_t st) {
+  if (n16 <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, 65536);
+  hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint4 *)dst,
+                     (const uint4 *)src, n16);
   return hipGetLastError();
 }
 

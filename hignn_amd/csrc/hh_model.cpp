@@ -134,6 +134,10 @@ struct hh_model {
   double prof_ms[2] = {0, 0};
   int64_t prof_pairs[2] = {0, 0}, prof_launches[2] = {0, 0};
   hipEvent_t evm[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_all[2] = {nullptr, nullptr};  // whole matvec
+  // batched far field: the near field runs beside it on the side stream
+  // (HH_NEAR_OVERLAP=0 runs them one after the other)
+  bool near_overlap = true;
   // persistent far field: the near field runs beside it on a second stream
   // (forked and joined with events), on the compute units the far blocks
   // leave idle
@@ -171,6 +175,8 @@ struct hh_model {
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto &e : evm)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : ev_all)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {fork_ev, join_ev, far0_ev})
       if (e) (void)hipEventDestroy(e);
@@ -803,6 +809,7 @@ struct hh_model {
     if (!ev[0]) {
       for (auto &e : ev) HIPCHK(hipEventCreate(&e));
       for (auto &e : evm) HIPCHK(hipEventCreate(&e));
+      for (auto &e : ev_all) HIPCHK(hipEventCreate(&e));
       HIPCHK(hipEventCreate(&far0_ev));
       HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
       HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
@@ -815,6 +822,14 @@ struct hh_model {
   // both are done.  Both accumulate into u with FP64 atomics.  With timed,
   // ev[0..1] bracket the near field and far0_ev / ev[2] the far field.
   void near_beside_persistent_far(hipStream_t st, bool timed) {
+    near_beside_far(st, timed, true);
+  }
+
+  // Near field on the side stream, the far field (persistent kernel or
+  // batched iterations) on st; st continues once both are done.  The
+  // batched far field's per-iteration host waits are on st only, so the
+  // near kernel keeps running through them.
+  void near_beside_far(hipStream_t st, bool timed, bool persistent) {
     HIPCHK(hipEventRecord(fork_ev, st));
     HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
     if (timed) HIPCHK(hipEventRecord(ev[0], side));
@@ -822,7 +837,10 @@ struct hh_model {
     if (timed) HIPCHK(hipEventRecord(ev[1], side));
     HIPCHK(hipEventRecord(join_ev, side));
     if (timed) HIPCHK(hipEventRecord(far0_ev, st));
-    far_field_persistent(st);
+    if (persistent)
+      far_field_persistent(st);
+    else
+      far_field(st);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
   }
@@ -866,8 +884,9 @@ struct hh_model {
   // Far-field launch tuning on the caller's forces: each candidate (the
   // persistent kernel when there are at most 4 far blocks per compute unit;
   // the batched iterations with step chunks of 64, 128 and 256 items) runs
-  // one matvec untimed and one timed (HIP events around the far field); the
-  // fastest is kept.  Rows of `out`: {mode, chunk, far-field ms}.
+  // one matvec untimed and one timed (HIP events around the whole matvec,
+  // since the near field runs beside the far field); the fastest is kept.
+  // Rows of `out`: {mode, chunk, matvec ms}.
   void tune_far_field(const float *f_dev, hipStream_t st, std::vector<std::array<double, 3>> &out) {
     check_ready();
     std::vector<std::array<int, 2>> cand;
@@ -884,8 +903,9 @@ struct hh_model {
         aca_chunk = c[1];
         for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
         HIPCHK(hipStreamSynchronize(st));
-        out.push_back({(double)c[0], (double)c[1], t_far});
-        if (out.size() == 1 || t_far < best) best = t_far, pick = c;
+        // the whole matvec: the near field may run beside the far field
+        out.push_back({(double)c[0], (double)c[1], t_total});
+        if (out.size() == 1 || t_total < best) best = t_total, pick = c;
       }
     } catch (...) {  // a failed candidate leaves the settings as they were
       far_mode = mode0;
@@ -910,29 +930,33 @@ struct hh_model {
 
   void dot_device(float *u_dev, const float *f_dev, hipStream_t st) {
     check_ready();
+    HIPCHK(hipEventRecord(ev_all[0], st));
     HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
     HIPCHK(hipMemsetAsync(u.p, 0, 24 * (size_t)n, st));
     prof_ms[1] = 0;
     prof_pairs[1] = prof_launches[1] = 0;
-    const bool beside = resolved_far_mode() == 0 && !far.empty();
+    const bool persistent = resolved_far_mode() == 0;
+    const bool beside = !far.empty() && (persistent || near_overlap);
     if (beside) {
-      near_beside_persistent_far(st, true);
+      near_beside_far(st, true, persistent);
     } else {
       HIPCHK(hipEventRecord(ev[0], st));
       near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
       HIPCHK(hipEventRecord(ev[1], st));
+      HIPCHK(hipEventRecord(far0_ev, st));
       far_field(st);
       HIPCHK(hipEventRecord(ev[2], st));
     }
     HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
-    HIPCHK(hipEventSynchronize(ev[2]));
-    HIPCHK(hipEventSynchronize(ev[1]));
-    float a = 0, b = 0;
+    HIPCHK(hipEventRecord(ev_all[1], st));
+    HIPCHK(hipEventSynchronize(ev_all[1]));
+    float a = 0, b = 0, c = 0;
     HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
-    HIPCHK(hipEventElapsedTime(&b, beside ? far0_ev : ev[1], ev[2]));
+    HIPCHK(hipEventElapsedTime(&b, far0_ev, ev[2]));
+    HIPCHK(hipEventElapsedTime(&c, ev_all[0], ev_all[1]));
     t_close = a;
     t_far = b;
-    t_total = (double)a + (double)b;
+    t_total = c;  // the whole matvec, gather to scatter (near and far may overlap)
     prof_ms[0] = a;  // the near field is one kernel launch
     prof_pairs[0] = near_pairs;
     prof_launches[0] = n_near_tasks > 0 ? 1 : 0;
@@ -1109,6 +1133,7 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
     if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
     if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
+    if (const char *ov = std::getenv("HH_NEAR_OVERLAP")) m->near_overlap = std::atoi(ov) != 0;
     m->aca_chunk = chunk;
     if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
       m->far_pool_bytes = (size_t)(std::atof(gb) * (double)(1ull << 30));
@@ -1488,6 +1513,16 @@ int hh_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
   return guarded([&]() {
     if ((!dst_host || !src_dev) && bytes) throw HHError(HH_ERR_ARG, "NULL argument");
     if (bytes) HIPCHK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int hh_device_copy(void *dst_dev, const void *src_dev, size_t bytes) {
+  return guarded([&]() {
+    if ((!dst_dev || !src_dev) && bytes) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (bytes % 16) throw HHError(HH_ERR_ARG, "bytes must be a multiple of 16");
+    ensure_device();
+    HIPCHK(launch_copy16(dst_dev, src_dev, bytes / 16, nullptr));
+    HIPCHK(hipDeviceSynchronize());
   });
 }
 

@@ -122,7 +122,9 @@ int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int
                    int64_t *far_ij, int64_t *leaves);
 void hh_tree_free(hh_tree *t);
 
-/* Per-phase device times (ms) of the last dot: [close, far, total]. */
+/* Device times (ms) of the last dot: [close, far, total], total from the
+ * force gather to the velocity scatter (the near and far fields may run at
+ * the same time on two streams, so total <= close + far). */
 int hh_model_last_timings(hh_model *m, double t[3]);
 
 /* Pair-function kernels of the last dot (device time in ms from HIP events
@@ -138,7 +140,7 @@ int hh_model_last_profile(hh_model *m, double ms[2], int64_t pairs[2], int64_t l
  * ACA iterations with step chunks of 64, 128 and 256 items.  Each runs one
  * untimed and one timed matvec; the fastest is kept for the model's later
  * matvecs.  table receives up to
- * max_rows rows of {mode (0 persistent, 1 batched), chunk, far-field ms},
+ * max_rows rows of {mode (0 persistent, 1 batched), chunk, matvec ms},
  * *rows their number.  Every candidate computes the same matvec up to
  * rounding.  No reference counterpart (the reference has fixed launch
  * shapes, src/FarDot.cpp:64-120). */

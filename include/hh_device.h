@@ -22,6 +22,10 @@ int hh_device_free(void *p);
 int hh_memcpy_to_device(void *dst_dev, const void *src_host, size_t bytes);
 int hh_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes);
 int hh_device_synchronize(void);
+/* Device-to-device copy of bytes (a multiple of 16) by a copy kernel on the
+ * default stream, synchronous: a transfer of known size for calibrating HBM
+ * byte counters. */
+int hh_device_copy(void *dst_dev, const void *src_dev, size_t bytes);
 
 #ifdef __cplusplus
 }
