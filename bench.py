@@ -230,6 +230,8 @@ def stepper_bench(args):
         else:
             it.set_device_velocity(m, fc)
             it.set_graphs(mode != "device_eager")
+            if mode == "device_reuse":
+                it.set_tree_rebuild(args.rebuild_every)
         state = np.zeros((n, 6), np.float32)
         state[:, :3] = x
         res = []
@@ -251,6 +253,8 @@ def stepper_bench(args):
         out[mode] = timed(mode, args.steps)
     xs = {k: v[2] for k, v in out.items()}
     diff = float(np.abs(xs["device_graphs"] - xs["callback"]).max())
+    diff_reuse = float(np.abs(xs["device_reuse"] - xs["device_graphs"]).max())
+    moved = float(np.abs(xs["device_graphs"] - x).max())
     rate = {k: v[0] / v[1] for k, v in out.items()}
     steps, el = out["device_graphs"][0], out["device_graphs"][1]
     line = {
@@ -273,6 +277,9 @@ def stepper_bench(args):
             "callback_steps_per_s": round(rate["callback"], 3),
             "device_eager_steps_per_s": round(rate["device_eager"], 3),
             "device_tree_reuse_steps_per_s": round(rate["device_reuse"], 3),
+            "device_tree_reuse_rebuild_every": args.rebuild_every,
+            "device_tree_reuse_max_abs_position_diff_vs_rebuilt": diff_reuse,
+            "max_displacement": moved,
             "device_tree_reuse_whole_stage_graph_launches": out["device_reuse"][3],
             "speedup_vs_callback": round(rate["device_graphs"] / rate["callback"], 3),
             "far_field_tuning": ("off" if args.no_tune_far else
@@ -317,6 +324,8 @@ def main():
     ap.add_argument("--stepper", action="store_true",
                     help="time Explicit Euler steps of the filament_cloud suspension instead "
                          "(Python velocity callback vs the device-resident integrator)")
+    ap.add_argument("--rebuild-every", type=int, default=10,
+                    help="stepper, tree kept across stages: full rebuild every this many steps")
     ap.add_argument("--filaments", type=int, default=1024,
                     help="filaments of 21 beads in the --stepper workload")
     ap.add_argument("--three-body", action="store_true",

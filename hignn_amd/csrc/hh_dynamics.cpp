@@ -27,6 +27,8 @@ struct hh_forcing {
   int64_t n_fil = 0, n_chain = 0;
   float rest = 0, k_t = 0;
   double k_b = 0;
+  int64_t bond_fil = 0, bond_chain = 0;
+  double bond_D = 0, bond_alpha = 0, bond_r0 = 0;
   float eps = 0;
   bool periodic = false, has_domain = false;
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -40,6 +42,8 @@ struct hh_forcing {
       throw HHError(HH_ERR_STATE, "periodic Morse force without a domain (set_domain)");
     if (n_fil > 0 && n_fil * n_chain > n)
       throw HHError(HH_ERR_ARG, "filaments cover more particles than given");
+    if (bond_fil > 0 && bond_fil * bond_chain > n)
+      throw HHError(HH_ERR_ARG, "bond chains cover more particles than given");
     ForcingLaunch L{};
     L.x = x;
     L.f = f;
@@ -50,6 +54,11 @@ struct hh_forcing {
     L.rest = rest;
     L.k_t = k_t;
     L.k_b = k_b;
+    L.bond_fil = bond_fil;
+    L.bond_chain = bond_chain;
+    L.bond_D = bond_D;
+    L.bond_alpha = bond_alpha;
+    L.bond_r0 = bond_r0;
     L.morse = eps > 0 ? 1 : 0;
     L.periodic = periodic ? 1 : 0;
     L.eps = eps;
@@ -218,12 +227,22 @@ struct hh_stepper {
 
   // Velocity of a stage: forcing at the stage input, update_coord, u = M f.
   void velocity(int row, DBuf &out) {
-    if (run_full(row, out)) {
+    const bool rebuild = pending_rebuild;
+    pending_rebuild = false;
+    if (!rebuild && run_full(row, out)) {
       evals++;
       return;
     }
     run_prep(row);
-    model_call(hh_model_update_coord_device(m, XIN.as<float>(), n, st));
+    if (rebuild) {
+      model_call(hh_model_set_tree_reuse(m, 0));
+      const int rc = hh_model_update_coord_device(m, XIN.as<float>(), n, st);
+      model_call(hh_model_set_tree_reuse(m, 1));
+      model_call(rc);
+      rebuilds++;
+    } else {
+      model_call(hh_model_update_coord_device(m, XIN.as<float>(), n, st));
+    }
     model_call(hh_model_dot_device(m, out.as<float>(), F.as<float>(), n, st));
     if (tb && hh_three_body_velocity_device(tb, XIN.as<float>(), F.as<float>(), n, out.as<float>(), 1,
                                             st) != HH_OK)
@@ -237,12 +256,24 @@ struct hh_stepper {
     out_fn(out_user, count, t, X.as<float>(), v.as<float>());
   }
 
+  // With the model's tree kept across stages (hh_model_set_tree_reuse), a
+  // full rebuild at the start of every rebuild_every-th accepted step
+  // (0: never): the positions have moved, and admissibility is re-checked.
+  // The rebuild uses the first stage's input positions, as the first build
+  // of the run does.
+  int64_t rebuild_every = 0, rebuilds = 0;
+  bool pending_rebuild = false;
+  void maybe_rebuild() {
+    if (rebuild_every > 0 && count > 0 && count % rebuild_every == 0) pending_rebuild = true;
+  }
+
   // ExplicitEuler::Run (TimeIntegrator.hpp:431-488)
   void run_euler(int64_t max_steps) {
     const double h0 = dt;
     int64_t taken = 0;
     while (t < tf - 1e-5 && (max_steps <= 0 || taken < max_steps)) {
       const double h = std::min(h0, tf - t);
+      maybe_rebuild();
       velocity(0, V);
       HIPCHK(launch_set_f32(coef.as<float>() + 7 * 8, (float)h, st));
       HIPCHK(launch_euler_update(X.as<float>(), V.as<float>(), OFF.as<float>(), n,
@@ -289,6 +320,7 @@ struct hh_stepper {
     int64_t taken = 0;
     while (t < tf - 1e-5 && (max_steps <= 0 || taken < max_steps)) {
       double h = std::min(h0, tf - t);
+      maybe_rebuild();
       // x0 = x (+ periodic offset of x)
       StageLaunch S0 = stage(0);
       S0.xin = X0.as<float>();
@@ -371,6 +403,22 @@ int hh_forcing_set_filaments(hh_forcing *fc, int64_t n_filament, int64_t n_chain
     fc->rest = rest_length;
     fc->k_t = k_t;
     fc->k_b = k_b;
+    fc->version++;
+  });
+}
+
+int hh_forcing_set_bonds(hh_forcing *fc, int64_t n_filament, int64_t n_chain, double D,
+                         double alpha, double r0) {
+  return guarded([&]() {
+    need(fc);
+    if (n_filament < 0 || (n_filament > 0 && n_chain < 1))
+      throw HHError(HH_ERR_ARG, "invalid bond-chain layout");
+    if (n_filament > 0 && !(alpha > 0 && r0 > 0)) throw HHError(HH_ERR_ARG, "need alpha, r0 > 0");
+    fc->bond_fil = n_filament;
+    fc->bond_chain = n_filament > 0 ? n_chain : 0;
+    fc->bond_D = D;
+    fc->bond_alpha = alpha;
+    fc->bond_r0 = r0;
     fc->version++;
   });
 }
@@ -465,6 +513,13 @@ int hh_stepper_set_lim(hh_stepper *s, int axis, float lo, float hi) {
 
 int hh_stepper_set_three_body(hh_stepper *s, hh_three_body *tb) {
   return guarded([&]() { need(s)->tb = tb; });
+}
+
+int hh_stepper_set_tree_rebuild(hh_stepper *s, int64_t every) {
+  return guarded([&]() {
+    if (every < 0) throw HHError(HH_ERR_ARG, "every must be >= 0");
+    need(s)->rebuild_every = every;
+  });
 }
 
 int hh_stepper_set_graphs(hh_stepper *s, int flag) {

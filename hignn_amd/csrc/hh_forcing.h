@@ -20,6 +20,11 @@ struct ForcingLaunch {
   int64_t n_fil, n_chain;
   float rest, k_t;
   double k_b;
+  // Morse bonds between consecutive beads of bond chains: particles
+  // [0, bond_fil * bond_chain) in chains of bond_chain; U(r) = D (1 -
+  // exp(-alpha (r - r0)))^2 per bond
+  int64_t bond_fil, bond_chain;
+  double bond_D, bond_alpha, bond_r0;
   // Morse neighbours within eps
   int morse, periodic;
   float eps;

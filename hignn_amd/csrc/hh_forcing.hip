@@ -381,6 +381,28 @@ __global__ void force_kernel(ForcingLaunch L) {
     for (int j = 0; j < 3; j++) f[j] = (float)((double)f[j] + ft[j]);
   }
 
+  if (L.bond_chain > 1 && i < L.bond_fil * L.bond_chain) {
+    // bonded Morse: the bond (k, k+1) pulls bead k along d = x_{k+1} - x_k
+    // with dU/dr and bead k+1 the other way; FP64, added once
+    const int64_t q = i % L.bond_chain, base = i - q;
+    double acc[3] = {0, 0, 0};
+    for (int side = 0; side < 2; side++) {
+      if (side == 0 ? q > L.bond_chain - 2 : q < 1) continue;
+      float p[3];
+      load3(L.x, side == 0 ? base + q + 1 : base + q - 1, p);
+      // d = x_{k+1} - x_k of the bond
+      double d[3];
+      for (int j = 0; j < 3; j++)
+        d[j] = side == 0 ? (double)p[j] - (double)xi[j] : (double)xi[j] - (double)p[j];
+      const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double e = exp(-L.bond_alpha * (r - L.bond_r0));
+      const double g = 2.0 * L.bond_D * L.bond_alpha * e * (1.0 - e) / r;  // dU/dr / r
+      const double sg = side == 0 ? g : -g;
+      for (int j = 0; j < 3; j++) acc[j] += sg * d[j];
+    }
+    for (int j = 0; j < 3; j++) f[j] = (float)((double)f[j] + acc[j]);
+  }
+
   if (L.morse) {
     double acc[3] = {0, 0, 0};
     const int cx = cell_of(xi[0], L.inv_h), cy = cell_of(xi[1], L.inv_h),

@@ -1644,15 +1644,24 @@ __device__ void persistent_aca(const PAcaArgs &A, Core core) {
         const double a = mm[0], bb = mm[1], c = mm[2], d = mm[3], e = mm[4], f = mm[5], g = mm[6],
                      h = mm[7], ii = mm[8];
         const double det = a * (e * ii - f * h) - bb * (d * ii - f * g) + c * (d * h - e * g);
-        sh_inv[0] = (e * ii - f * h) / det;
-        sh_inv[1] = -(bb * ii - c * h) / det;
-        sh_inv[2] = (bb * f - c * e) / det;
-        sh_inv[3] = -(d * ii - f * g) / det;
-        sh_inv[4] = (a * ii - c * g) / det;
-        sh_inv[5] = -(a * f - c * d) / det;
-        sh_inv[6] = (d * h - e * g) / det;
-        sh_inv[7] = -(a * h - bb * g) / det;
-        sh_inv[8] = (a * e - bb * d) / det;
+        double iv[9];
+        iv[0] = (e * ii - f * h) / det;
+        iv[1] = -(bb * ii - c * h) / det;
+        iv[2] = (bb * f - c * e) / det;
+        iv[3] = -(d * ii - f * g) / det;
+        iv[4] = (a * ii - c * g) / det;
+        iv[5] = -(a * f - c * d) / det;
+        iv[6] = (d * h - e * g) / det;
+        iv[7] = -(a * h - bb * g) / det;
+        iv[8] = (a * e - bb * d) / det;
+        // as the batched path: an exactly singular or non-finite pivot
+        // inverse becomes zero, so the new term vanishes, nu2 = 0 ends the
+        // block at the norm test, and no inf/NaN reaches the pools or u
+        bool finite = det != 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) finite = finite && isfinite(iv[q]);
+#pragma unroll
+        for (int q = 0; q < 9; q++) sh_inv[q] = finite ? iv[q] : 0.0;
       }
       __syncthreads();
       {

@@ -308,7 +308,23 @@ struct hh_model {
   }
 
   // ---- update_coord ------------------------------------------------------
+  // The host coordinates after device-side updates on a kept tree: the
+  // positions live only in xp (tree order) until a rebuild needs them.
+  bool x_stale = false;
+  void refresh_host_x() {
+    if (!x_stale) return;
+    std::vector<float> xph(4 * (size_t)n);
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(xph.data(), xp.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t s = tl.perm[(size_t)i];
+      for (int d = 0; d < 3; d++) x[3 * (size_t)s + d] = xph[4 * (size_t)i + d];
+    }
+    x_stale = false;
+  }
+
   void update() {
+    if (!(tree_reuse && built)) refresh_host_x();
     if (tree_reuse && built) {
       // keep the permutation, tree and block lists of the last build; only
       // the positions (in that tree order) are refreshed
@@ -1213,6 +1229,7 @@ int hh_model_update_coord(hh_model *m, const float *coord, int64_t n) {
     if (!m || !coord) throw HHError(HH_ERR_ARG, "NULL argument");
     if (n != m->n) throw HHError(HH_ERR_ARG, "particle count differs from the model's");
     m->x.assign(coord, coord + 3 * n);
+    m->x_stale = false;
     m->update();
   });
 }
@@ -1225,11 +1242,13 @@ int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n,
     if (m->tree_reuse && m->built) {
       // kept tree: positions in tree order straight from the device array
       HIPCHK(launch_gather_forces(coord_dev, n, m->perm.as<int32_t>(), m->xp.as<float4>(), st));
+      m->x_stale = true;  // the host copy catches up before the next rebuild
       return;
     }
     m->x.resize(3 * (size_t)n);
     HIPCHK(hipMemcpyAsync(m->x.data(), coord_dev, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    m->x_stale = false;
     m->update();
   });
 }

@@ -233,6 +233,9 @@ class Forcing {
                      double k_b) {
     check(hh_forcing_set_filaments(f_, n_filament, n_chain, rest_length, k_t, k_b));
   }
+  void set_bonds(int64_t n_filament, int64_t n_chain, double D, double alpha, double r0) {
+    check(hh_forcing_set_bonds(f_, n_filament, n_chain, D, alpha, r0));
+  }
   py::array_t<float> eval(py::array_t<float, py::array::forcecast> coord) {
     if (coord.ndim() != 2) throw std::runtime_error("Rank of target sites must be two");
     auto x = rows3(coord, "coord");
@@ -465,6 +468,11 @@ class Integrator {
     }
   }
   void set_graphs(bool flag) { graphs_ = flag; }
+  // kept tree rebuilt every `every` accepted steps (0: never)
+  void set_tree_rebuild(int64_t every) {
+    if (every < 0) throw std::runtime_error("every must be >= 0");
+    rebuild_every_ = every;
+  }
   py::array_t<float> positions() const {
     py::array_t<float> p({(py::ssize_t)n_, (py::ssize_t)3});
     auto a = p.mutable_unchecked<2>();
@@ -545,6 +553,7 @@ class Integrator {
   ThreeBody *dt3_ = nullptr;
   hh_stepper *stepper_ = nullptr;
   bool graphs_ = true;
+  int64_t rebuild_every_ = 0;
   int64_t stats_[4] = {0, 0, 0, 0};
 
   bool device_mode() const { return dm_ != nullptr; }
@@ -567,6 +576,7 @@ class Integrator {
     if (periodic_)
       for (int d = 0; d < 3; d++) check(hh_stepper_set_lim(stepper_, d, lim_[0][d], lim_[1][d]));
     check(hh_stepper_set_graphs(stepper_, graphs_));
+    check(hh_stepper_set_tree_rebuild(stepper_, rebuild_every_));
     std::vector<float> x(3 * n_);
     for (size_t i = 0; i < n_; i++)
       for (int j = 0; j < 3; j++) x[3 * i + j] = pos_[i].v[j];
@@ -748,6 +758,7 @@ PYBIND11_MODULE(hignn, m) {
            py::arg("forcing"), py::arg("three_body") = nullptr, py::keep_alive<1, 2>(),
            py::keep_alive<1, 3>(), py::keep_alive<1, 4>())
       .def("set_graphs", &Euler::set_graphs)
+      .def("set_tree_rebuild", &Euler::set_tree_rebuild, py::arg("every"))
       .def("positions", &Euler::positions)
       .def("device_stats", &Euler::device_stats);
 
@@ -770,6 +781,7 @@ PYBIND11_MODULE(hignn, m) {
            py::arg("forcing"), py::arg("three_body") = nullptr, py::keep_alive<1, 2>(),
            py::keep_alive<1, 3>(), py::keep_alive<1, 4>())
       .def("set_graphs", &Rk4::set_graphs)
+      .def("set_tree_rebuild", &Rk4::set_tree_rebuild, py::arg("every"))
       .def("positions", &Rk4::positions)
       .def("device_stats", &Rk4::device_stats);
 
@@ -778,6 +790,8 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_body_force", &Forcing::set_body_force)
       .def("set_filaments", &Forcing::set_filaments, py::arg("n_filament"), py::arg("n_chain"),
            py::arg("rest_length"), py::arg("k_t"), py::arg("k_b"))
+      .def("set_bonds", &Forcing::set_bonds, py::arg("n_filament"), py::arg("n_chain"),
+           py::arg("D") = 1.0, py::arg("alpha") = 1.0, py::arg("r0") = 2.2)
       .def("set_two_body_epsilon", &Forcing::set_two_body_epsilon)
       .def("set_periodic", &Forcing::set_periodic, py::arg("is_periodic") = false)
       .def("set_domain", &Forcing::set_domain)

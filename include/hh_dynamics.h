@@ -23,7 +23,7 @@ extern "C" {
 typedef struct hh_forcing hh_forcing;
 
 /* A forcing program; all terms are off until set.  f = b + bending + tension
- * + Morse, each term as the reference computes it. */
+ * + Morse bonds + Morse neighbours, each term as the reference computes it. */
 int hh_forcing_create(hh_forcing **out);
 void hh_forcing_destroy(hh_forcing *fc);
 
@@ -38,6 +38,16 @@ int hh_forcing_set_body(hh_forcing *fc, const float b[3]);
  * removes the term. */
 int hh_forcing_set_filaments(hh_forcing *fc, int64_t n_filament, int64_t n_chain,
                              float rest_length, float k_t, double k_b);
+
+/* Morse bonds between consecutive beads: particles [0, n_filament *
+ * n_chain) form chains of n_chain; each bond (k, k+1) of a chain adds
+ * -dU/dr along the bond to both beads, U(r) = D (1 - exp(-alpha (r - r0)))^2
+ * (the Morse form of CalculatePotentialForce, include/PotentialForce.hpp:
+ * 115-140, with the bond's own D, alpha, r0 and neighbours given by the
+ * chain instead of a radius search).  Sums in FP64, added to the float force
+ * once.  n_filament = 0 removes the term. */
+int hh_forcing_set_bonds(hh_forcing *fc, int64_t n_filament, int64_t n_chain, double D,
+                         double alpha, double r0);
 
 /* Morse pair force over neighbours within epsilon: PotentialForce
  * (include/PotentialForce.hpp): set_two_body_epsilon (:229-231),
@@ -103,6 +113,13 @@ int hh_stepper_set_lim(hh_stepper *s, int axis, float lo, float hi);
  * outlive the stepper.  No reference counterpart as a native call. */
 struct hh_three_body;
 int hh_stepper_set_three_body(hh_stepper *s, struct hh_three_body *tb);
+
+/* With the model's tree kept across stages (hh_model_set_tree_reuse), a full
+ * tree and list rebuild at the first stage of every `every`-th accepted step
+ * (0, the default: the first build is kept for the whole run).  No reference
+ * counterpart (the reference rebuilds on every update_coord,
+ * src/HignnModel.cpp:744-762). */
+int hh_stepper_set_tree_rebuild(hh_stepper *s, int64_t every);
 
 /* Per-stage HIP graphs for the stage combination and forcing (default on). */
 int hh_stepper_set_graphs(hh_stepper *s, int flag);

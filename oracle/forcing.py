@@ -10,6 +10,9 @@ Only tests/ import this module, as the checker of the device forcing kernels.
   force += bending; force += tension``.
 - ``morse_pair``: CalculatePotentialForce (include/PotentialForce.hpp:115-140),
   De = 1, a = 1, re = 2.5.
+- ``bond_forces``: the same Morse form with the bond parameters (D, alpha,
+  r0) between consecutive beads of each chain (the filament workload's
+  bonds), float64.
 - ``source_images``: BuildSourceSites (include/PotentialForce.hpp:247-392),
   written as its loop over duplicated axes with halving block strides.
 - ``morse_force``: GetPotentialForce (include/PotentialForce.hpp:468-580) with
@@ -23,8 +26,8 @@ Only tests/ import this module, as the checker of the device forcing kernels.
 
 Pinned by tests/test_forcing_oracle.py: closed forms (two-particle tension,
 right-angle bending, zero bending of a straight chain), Newton's third law,
-rotation covariance, the Morse force as minus the finite-difference gradient
-of the Morse potential, the brute-force neighbour set against scipy's
+rotation covariance, the Morse force and the bond forces as minus the
+finite-difference gradient of their potentials, the brute-force neighbour set against scipy's
 cKDTree, and the image layout against an independent bit-by-bit reading.
 """
 import numpy as np
@@ -88,9 +91,38 @@ def filament_terms(x, n_fil, n_chain, rest, k_t, k_b):
     return bend, tens
 
 
-def velocity_force(x, b=(0, 0, 0), n_fil=0, n_chain=0, rest=0.0, k_t=0.0, k_b=0.0, morse=None):
-    """The float32 force of velocity_update; ``morse`` (float64 (n,3)) is
-    added last when given, as the device forcing does."""
+def bond_forces(x, n_fil, n_chain, D=1.0, alpha=1.0, r0=2.2):
+    """Morse bonds between consecutive beads of n_fil chains of n_chain:
+    U(r) = D (1 - exp(-alpha (r - r0)))^2 per bond (the Morse form of
+    CalculatePotentialForce, include/PotentialForce.hpp:115-140, with the
+    bond's parameters), float64 (n,3) from float32 positions; particles past
+    the chains get zero."""
+    x = np.asarray(x, dtype=f32).astype(np.float64)
+    out = np.zeros_like(x)
+    m = n_fil * n_chain
+    X = x[:m].reshape(n_fil, n_chain, 3)
+    d = X[:, 1:] - X[:, :-1]
+    r = np.sqrt(np.sum(d * d, axis=2, keepdims=True))
+    e = np.exp(-alpha * (r - r0))
+    fb = 2.0 * D * alpha * e * (1.0 - e) / r * d  # on bead k of bond (k, k+1)
+    F = np.zeros_like(X)
+    F[:, :-1] += fb
+    F[:, 1:] -= fb
+    out[:m] = F.reshape(m, 3)
+    return out
+
+
+def bond_energy(x, n_fil, n_chain, D=1.0, alpha=1.0, r0=2.2):
+    """Total bond energy sum_bonds D (1 - exp(-alpha (r - r0)))^2 (float64)."""
+    X = np.asarray(x, dtype=np.float64)[:n_fil * n_chain].reshape(n_fil, n_chain, 3)
+    r = np.linalg.norm(X[:, 1:] - X[:, :-1], axis=2)
+    return float(np.sum(D * (1.0 - np.exp(-alpha * (r - r0))) ** 2))
+
+
+def velocity_force(x, b=(0, 0, 0), n_fil=0, n_chain=0, rest=0.0, k_t=0.0, k_b=0.0, morse=None,
+                   bonds=None):
+    """The float32 force of velocity_update; ``bonds`` and then ``morse``
+    (float64 (n,3)) are added last when given, as the device forcing does."""
     x = np.asarray(x, dtype=f32)
     f = np.zeros(x.shape, dtype=f32)
     f += np.asarray(b, dtype=f32)
@@ -98,6 +130,8 @@ def velocity_force(x, b=(0, 0, 0), n_fil=0, n_chain=0, rest=0.0, k_t=0.0, k_b=0.
         bend, tens = filament_terms(x, n_fil, n_chain, rest, k_t, k_b)
         f = (f.astype(np.float64) + bend).astype(f32)
         f = (f.astype(np.float64) + tens).astype(f32)
+    if bonds is not None:
+        f = (f.astype(np.float64) + bonds).astype(f32)
     if morse is not None:
         f = (f.astype(np.float64) + morse).astype(f32)
     return f

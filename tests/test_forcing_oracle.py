@@ -148,3 +148,25 @@ def test_periodic_morse_reads_source_array_at_target_index():
     # against source entry 1 (x0 + 10), as GetPotentialForce indexes it
     np.testing.assert_array_equal(f[0], F.morse_pair((x[0] - src[1])[None].astype(f32))[0])
     np.testing.assert_array_equal(f[1], F.morse_pair((x[1] - src[0])[None].astype(f32))[0])
+
+
+def test_bond_forces_are_minus_the_energy_gradient():
+    """Bonded Morse forces against central differences of the total bond
+    energy, Newton's third law per chain, and zero force at the rest length."""
+    from hignn_amd import workloads
+    x = workloads.filaments(3, 6, 2.2, 0.08, 5).astype(np.float64)
+    x += np.random.default_rng(0).normal(scale=0.3, size=x.shape)
+    Fb = F.bond_forces(x.astype(np.float32), 3, 6, D=1.3, alpha=0.9, r0=2.1)
+    xs = x.astype(np.float32).astype(np.float64)
+    h = 1e-6
+    for i in range(x.shape[0]):
+        for j in range(3):
+            xp, xm = xs.copy(), xs.copy()
+            xp[i, j] += h
+            xm[i, j] -= h
+            g = (F.bond_energy(xp, 3, 6, 1.3, 0.9, 2.1) - F.bond_energy(xm, 3, 6, 1.3, 0.9, 2.1)) / (2 * h)
+            assert abs(Fb[i, j] + g) < 1e-6, (i, j, Fb[i, j], -g)
+    np.testing.assert_allclose(Fb.reshape(3, 6, 3).sum(axis=1), 0.0, atol=1e-12)
+    # a straight chain at the rest length feels no bond force
+    st = workloads.filaments(2, 5, 2.2, 0.08, 1)
+    assert np.abs(F.bond_forces(st, 2, 5)).max() < 1e-6

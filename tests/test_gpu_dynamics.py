@@ -345,3 +345,51 @@ def test_whole_stage_graphs_recapture_after_model_settings(xavier_w, monkeypatch
     assert close(c, d, 1e-6)
     # one eager evaluation sets the persistent kernel up for the new cap
     assert sc["whole_stage_graphs"] == sc["velocity_evaluations"] - 1
+
+
+def test_tree_rebuild_every_step_equals_rebuilding_stepper(xavier_w):
+    """With the tree kept across stages but rebuilt at every accepted step,
+    an Euler run (one stage per step) builds the same trees from the same
+    positions as the stepper without tree reuse: identical trajectories."""
+    h = hignn()
+    x = suspension(3)
+    kw = dict(dt=0.02, tf=0.1)
+    xa, ca, _ = integrate(h, h.ExplicitEuler, x, xavier_w, device=True, **kw)
+    m = model(h, x, xavier_w, reuse=True)
+    it = h.ExplicitEuler()
+    it.set_time_step(kw["dt"])
+    it.set_final_time(kw["tf"])
+    it.set_num_rigid_body(x.shape[0])
+    it.set_device_velocity(m, forcing(h))
+    it.set_tree_rebuild(1)
+    state = np.zeros((x.shape[0], 6), f32)
+    state[:, :3] = x
+    it.initialize(state)
+    it.run()
+    assert it.func_count() == ca
+    np.testing.assert_array_equal(it.positions(), xa)
+
+
+def test_rebuild_after_device_updates_reads_current_positions(xavier_w):
+    """Device-side coordinate updates on a kept tree leave the host copy
+    behind; a later rebuild (here forced by a settings change) must build
+    from the current positions."""
+    from hignn_amd import abi, workloads
+    x = workloads.uniform_cube(900, 0.05, 4)
+    x2 = (x + np.random.default_rng(4).normal(scale=3.0, size=x.shape)).astype(f32)
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-3, max_iter=20, tree_reuse=True)
+    m.update_coord()
+    xd = abi.DeviceArray.of(x2)
+    m.update_coord_device(xd.ptr)
+    m.set(use_symmetry=False)  # clears the built tree: the next dot rebuilds
+    f = np.random.default_rng(5).normal(size=x.shape).astype(f32)
+    m.dot(f)
+    fresh = abi.Model(x2, 64)
+    fresh.load_weights(xavier_w)
+    fresh.set(epsilon=1e-3, max_iter=20, use_symmetry=False)
+    fresh.update_coord()
+    a, b = m.export(), fresh.export()
+    for k in ("perm", "tree", "close", "far"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)

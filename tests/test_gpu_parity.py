@@ -409,3 +409,33 @@ def test_dot_fp32_parity_below_and_near_one_leaf(xavier_w, n):
     om.update_coord()
     assert O.rel_l2(m.dot(f), om.dot(xavier_w, f)) <= 1e-5
     assert O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f)) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_singular_pivot_blocks_stay_finite(xavier_w, prec, monkeypatch):
+    """A pair function that vanishes away from r = 0 (W3 = 0, b3 = -I, so
+    y + I = 0) makes every far block's pivot block exactly singular.  Both
+    far-field paths give the singular pivot a zero inverse: no inf/NaN
+    reaches u, which is then the self-interaction alone (u_i = f_i), and the
+    persistent kernel stops every block at the same rank as the batched
+    iterations."""
+    p = weights.unpack(xavier_w)
+    p = {k: v.copy() for k, v in p.items()}
+    p["W3"][:] = 0.0
+    p["b3"][:] = [-1, 0, 0, 0, -1, 0, 0, 0, -1]
+    w = weights.pack(p)
+    x = workloads.uniform_cube(400, 0.05, 9)
+    f = np.random.default_rng(9).normal(size=x.shape).astype(np.float32)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("HH_FAR_MODE", mode)
+        m = abi.Model(x, 32)
+        m.load_weights(w)
+        m.set(epsilon=1e-4, max_iter=12, precision=prec)
+        m.update_coord()
+        u = m.dot(f)
+        assert len(m.far_ranks()) > 0
+        assert np.isfinite(u).all(), mode
+        np.testing.assert_allclose(u, f, rtol=1e-6, atol=1e-6)
+        out[mode] = m.far_ranks()
+    np.testing.assert_array_equal(out["0"], out["1"])
