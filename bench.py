@@ -331,6 +331,9 @@ def main():
     ap.add_argument("--three-body", action="store_true",
                     help="time the three-body and self corrections on the gravity_field lattice "
                          "instead")
+    ap.add_argument("--instances", type=int, default=0,
+                    help="time the many-instance workload with this many suspensions (64) "
+                         "instead")
     ap.add_argument("--lattice", type=int, default=50,
                     help="lattice points per side in the --three-body workload")
     args = ap.parse_args()
@@ -342,8 +345,115 @@ def main():
         return stepper_bench(args)
     if args.three_body:
         return three_body_bench(args)
+    if args.instances:
+        return instances_bench(args)
 
     return matvec_bench(args)
+
+
+def instances_bench(args):
+    """Many independent suspensions (the many-instance workload): 64 uniform
+    cubes with N log-uniform in [5,000, 200,000] and volume fraction uniform
+    in [5%, 15%] (seed 4), gravity, in one model: per-instance trees and
+    lists, one near-field launch and one far field over all instances per
+    matvec.  Chosen form as the 1M benchmark (BF16 MLP + trained surrogate);
+    value = aggregate particle-updates/s; per-instance relative error against
+    the FP64 evaluation of the same blocks."""
+    from hignn_amd import abi, weights, workloads
+    from oracle import surrogate as S
+
+    t0 = time.time()
+    x, ns = workloads.instances(count=args.instances)
+    n = x.shape[0]
+    log(f"[instances] {len(ns)} instances, N {ns.min()}..{ns.max()}, total {n} in "
+        f"{time.time() - t0:.1f}s")
+    f = workloads.gravity(n)
+    w = weights.smooth(0)
+    abi.lib().hh_init()
+    m = abi.Model(x, args.leaf, instances=ns)
+    m.load_weights(w)
+    m.set(epsilon=args.epsilon, max_iter=args.max_iter,
+          max_relative_coord=args.max_relative_coord)
+    t0 = time.time()
+    m.update_coord()
+    t_update = time.time() - t0
+    counts = m.counts()
+    log(f"[instances] update_coord {t_update:.2f}s: {counts}")
+    t0 = time.time()
+    r_s, t_s = m.surrogate_samples(args.sur_samples, args.sur_rlo, args.sur_rhi, seed=1)
+    sur = abi.Surrogate(args.sur_rlo, args.sur_rhi)
+    sur.set_params(S.init(0))
+    loss1 = sur.train(r_s.ptr, t_s.ptr, args.sur_samples, args.sur_steps, 3e-3, batch=16384,
+                      lr_final=3e-5)
+    abi.synchronize()
+    del r_s, t_s
+    t_train = time.time() - t0
+    f_dev = abi.DeviceArray.of(f)
+    u_dev = abi.DeviceArray((n, 3))
+
+    def run(prec, use_sur):
+        m.set(precision=prec)
+        m.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
+        abi.synchronize()
+        t = time.perf_counter()
+        m.dot_device(u_dev.ptr, f_dev.ptr)
+        abi.synchronize()
+        return time.perf_counter() - t
+
+    for i in range(args.warmup):
+        log(f"[instances] warmup {i}: {run(args.prec, True):.3f}s")
+    abi.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        dt = run(args.prec, True)
+        log(f"[instances] step {i}: {dt:.3f}s")
+    t_total = time.perf_counter() - t_start
+    u = u_dev.numpy()
+    stats = m.surrogate_stats()
+    t32 = None
+    if args.fp32_steps > 0:
+        t32 = min(run("fp32", False) for _ in range(args.fp32_steps))
+        log(f"[instances] fp32: {t32:.3f}s")
+    off = np.concatenate([[0], np.cumsum(ns)])
+    g = np.random.default_rng(5)
+    tg = np.concatenate([np.sort(g.choice(np.arange(off[k], off[k + 1]), size=min(150, ns[k]),
+                                          replace=False)) for k in range(len(ns))])
+    t0 = time.time()
+    ref = m.check_fp64(f, tg, 0)
+    errs = []
+    for k in range(len(ns)):
+        sel = (tg >= off[k]) & (tg < off[k + 1])
+        d = u[tg[sel]] - ref[sel]
+        errs.append(float(np.sqrt(np.sum(d ** 2) / np.sum(ref[sel] ** 2))))
+    log(f"[instances] fp64 check ({time.time() - t0:.1f}s): max error {max(errs):.2e}")
+    ms = 1000.0 * t_total / args.steps
+    value = n / (t_total / args.steps)
+    line = {
+        "metric": "many-instance matvec: aggregate particle-updates/s (64 suspensions, ~4.3M "
+                  "particles), per-instance rel err <= 1e-3",
+        "value": round(value, 2), "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.prec,
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(ns)} uniform cubes, N log-uniform in [5000, 200000] (total {n}), "
+                        f"volume fraction uniform in [5%, 15%], seed 4, gravity; leaf {args.leaf}, "
+                        f"eps {args.epsilon}, max_iter {args.max_iter}; smooth seeded weights",
+            "instances": len(ns), "n_min": int(ns.min()), "n_max": int(ns.max()),
+            "tree_rows": counts[0], "close_pairs": counts[1], "far_blocks": counts[2],
+            "update_coord_s": round(t_update, 3),
+            "surrogate_train_s": round(t_train, 2), "surrogate_loss": loss1,
+            "surrogate_stats": dict(zip(["far_blocks_judged", "on_surrogate", "fell_back",
+                                         "surrogate_entries", "full_mlp_far_entries"], stats)),
+            "per_instance_rel_err_fp64_max": max(errs),
+            "per_instance_rel_err_fp64_median": float(np.median(errs)),
+            "per_instance_rel_err_fp64": [round(e, 8) for e in errs],
+            "check_targets_per_instance": 150,
+            "fp32_value": round(n / t32, 2) if t32 else None,
+            "speedup_vs_fp32": round(value / (n / t32), 3) if t32 else None,
+        },
+    }
+    print(json.dumps(line), flush=True)
 
 
 def host_info():

@@ -32,6 +32,7 @@ def lib():
         sig = {
             "hh_init": [], "hh_finalize": [],
             "hh_model_create": [c_vp, c_i64, c_int, c_vp],
+            "hh_model_create_instances": [c_vp, c_vp, c_i64, c_int, c_vp],
             "hh_model_load_two_body": [c_vp, c_vp, c_i64],
             "hh_model_set_epsilon": [c_vp, c_dbl], "hh_model_set_eta": [c_vp, c_dbl],
             "hh_model_set_max_iter": [c_vp, c_int],
@@ -136,12 +137,21 @@ def tree_build(x, leaf, use_symmetry=True, max_relative_coord=500000):
 class Model:
     """Thin object wrapper over an hh_model handle."""
 
-    def __init__(self, x, leaf):
+    def __init__(self, x, leaf, instances=None):
+        """instances: particle counts of independent systems whose rows are
+        concatenated in x (hh_model_create_instances); None: one system."""
         _chk(lib().hh_init())
         self.x = _f32(x)
         self.n = self.x.shape[0]
         h = c_vp()
-        _chk(lib().hh_model_create(_p(self.x), self.n, int(leaf), ctypes.byref(h)))
+        if instances is None:
+            _chk(lib().hh_model_create(_p(self.x), self.n, int(leaf), ctypes.byref(h)))
+        else:
+            counts = np.ascontiguousarray(instances, dtype=np.int64)
+            if int(counts.sum()) != self.n:
+                raise ValueError("instance counts do not add up to the particle count")
+            _chk(lib().hh_model_create_instances(_p(self.x), _p(counts), counts.size, int(leaf),
+                                                 ctypes.byref(h)))
         self.h = h
         # held by the object: at interpreter exit the module's globals may be
         # gone before the last model is collected

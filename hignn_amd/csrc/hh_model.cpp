@@ -19,6 +19,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "../../include/hh.h"
 #include "../../include/hh_dynamics.h"
@@ -83,6 +85,9 @@ struct hh_model {
   bool post_check = false, use_symmetry = true;
   bool tree_reuse = false;  // update_coord keeps the last tree and lists (opt-in)
   uint64_t builds = 0, loads = 0;  // tree builds and weight loads (graph signatures)
+  // independent instances (hh_model_create_instances): particle counts, in
+  // row order of x; empty for a single system
+  std::vector<int64_t> inst_counts;
   int prec = PREC_FP32;
   // byte budget of one far-field batch (factor pools + Gram histories); 0:
   // half the device memory free at the first far field, at most 192 GiB
@@ -337,7 +342,10 @@ struct hh_model {
       upload(xp, xph.data(), xph.size() * 4);
       return;
     }
-    build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
+    if (inst_counts.size() > 1)
+      build_instances();
+    else
+      build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
     builds++;
     // positions in tree order
     std::vector<float> xph(4 * (size_t)n);
@@ -375,6 +383,61 @@ struct hh_model {
     far_rank.assign(far.size(), 0);
     p_ready = false;
     built = true;
+  }
+
+  // Trees and block lists of independent instances, built per instance on
+  // host threads and merged: instance k's node ids move up by the nodes of
+  // instances 0..k-1 and its particle ranges by their particles, so the
+  // merged lists drive one near-field launch and one far field over all
+  // instances with no block between two instances.  Leaves keep child id 0.
+  void build_instances() {
+    const size_t K = inst_counts.size();
+    std::vector<TreeLists> parts(K);
+    std::vector<int64_t> pbase(K + 1, 0);
+    for (size_t k = 0; k < K; k++) pbase[k + 1] = pbase[k] + inst_counts[k];
+    const int nt = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> th;
+    std::vector<std::string> errs(K);
+    for (int t = 0; t < std::min<int>(nt, (int)K); t++)
+      th.emplace_back([&]() {
+        for (size_t k; (k = next.fetch_add(1)) < K;) {
+          try {
+            build_tree_lists(x.data() + 3 * pbase[k], inst_counts[k], leaf, use_symmetry,
+                             max_rel_coord, 1, parts[k]);
+          } catch (const std::exception &e) {
+            errs[k] = e.what();
+          }
+        }
+      });
+    for (auto &t : th) t.join();
+    for (size_t k = 0; k < K; k++)
+      if (!errs[k].empty()) throw HHError(HH_ERR_ARG, "instance " + std::to_string(k) + ": " + errs[k]);
+    TreeLists out;
+    out.n = n;
+    int64_t nbase = 0;
+    for (size_t k = 0; k < K; k++) {
+      const TreeLists &p = parts[k];
+      const uint64_t nb = (uint64_t)nbase, pb = (uint64_t)pbase[k];
+      for (int64_t nd = 0; nd < p.T; nd++) {
+        const uint64_t *r = &p.rows[5 * (size_t)nd];
+        out.rows.push_back(r[0] ? r[0] + nb : 0);
+        out.rows.push_back(r[0] ? r[1] + nb : r[1]);
+        out.rows.push_back(r[2] + pb);
+        out.rows.push_back(r[3] + pb);
+        out.rows.push_back(r[4]);
+      }
+      out.aux.insert(out.aux.end(), p.aux.begin(), p.aux.end());
+      for (int64_t v : p.perm) out.perm.push_back(v + pbase[k]);
+      for (int64_t v : p.leaves) out.leaves.push_back(v + nbase);
+      for (int64_t v : p.closeI) out.closeI.push_back(v + nbase);
+      for (int64_t v : p.closeJ) out.closeJ.push_back(v + nbase);
+      for (int64_t v : p.farI) out.farI.push_back(v + nbase);
+      for (int64_t v : p.farJ) out.farJ.push_back(v + nbase);
+      nbase += p.T;
+    }
+    out.T = nbase;
+    tl = std::move(out);
   }
 
   void ensure_buffers() {
@@ -1120,6 +1183,25 @@ int hh_init(void) {
 int hh_finalize(void) {
   return guarded([&]() {
     if (g_device >= 0) HIPCHK(hipDeviceSynchronize());
+  });
+}
+
+int hh_model_create_instances(const float *coord, const int64_t *counts, int64_t ninst,
+                              int block_size, hh_model **out) {
+  return guarded([&]() {
+    if (!out || !counts) throw HHError(HH_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (ninst <= 0) throw HHError(HH_ERR_ARG, "need at least one instance");
+    int64_t n = 0;
+    for (int64_t k = 0; k < ninst; k++) {
+      if (counts[k] <= 0) throw HHError(HH_ERR_ARG, "every instance needs particles");
+      n += counts[k];
+    }
+    hh_model *m = nullptr;
+    const int rc = hh_model_create(coord, n, block_size, &m);
+    if (rc != HH_OK) throw HHError(rc, g_last_error);
+    m->inst_counts.assign(counts, counts + ninst);
+    *out = m;
   });
 }
 

@@ -83,9 +83,19 @@ def xavier(seed: int = 0, shapes=None) -> np.ndarray:
     return pack(parts, shapes)
 
 
-def smooth(seed: int = 0, w1_scale: float = 0.01) -> np.ndarray:
+def round_bf16(a: np.ndarray) -> np.ndarray:
+    """float32 values rounded to the nearest BF16 value (ties to even)."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def smooth(seed: int = 0, w1_scale: float = 0.01, bf16_exact: bool = True) -> np.ndarray:
     """A smooth pair function of the same shapes: ``xavier(seed)`` with the
-    layer-1 weights W1 multiplied by ``w1_scale`` (biases unchanged).
+    layer-1 weights W1 multiplied by ``w1_scale`` (biases unchanged) and,
+    with ``bf16_exact``, W2 and W3 rounded to BF16 values (the weights of a
+    BF16 checkpoint; every path, FP32 and FP64 included, then computes with
+    the same numbers).
 
     With Xavier's W1 (|W1 r| ~ 0.2 |r|) the first tanh layer saturates a few
     radii from the origin, so the pair function is close to a piecewise
@@ -98,11 +108,20 @@ def smooth(seed: int = 0, w1_scale: float = 0.01) -> np.ndarray:
     separation in the tolerance range 1e-4..1e-3), and the ACA tolerance test
     ends the iterations.  Not a physical mobility: a seeded stand-in for a
     trained net, whose far field is smooth in the same way.
+
+    Why BF16-exact W2 and W3: the BF16 MFMA path rounds its operands to BF16.
+    Rounding FP32 weights moves every block entry by a fixed relative amount
+    (measured on this net for |r| in [2, 30]: 9e-4 from W2, 7e-4 from W3,
+    the same sign for every pair), which sums of many pairs do not average
+    out; the activation rounding (about 8e-4 per layer) is random and does.
     """
     w = xavier(seed)
     p = unpack(w)
     p = {k: v.copy() for k, v in p.items()}
     p["W1"] = (p["W1"].astype(np.float64) * w1_scale).astype(np.float32)
+    if bf16_exact:
+        p["W2"] = round_bf16(p["W2"])
+        p["W3"] = round_bf16(p["W3"])
     return pack(p)
 
 

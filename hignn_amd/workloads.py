@@ -135,7 +135,22 @@ CONFIGS = {
                         epsilon=1e-4, max_iter=32),
     "c4_filaments_100k": dict(nfil=2000, nbead=50, bond=2.2, phi=0.08, seed=3, leaf=256,
                               epsilon=1e-4, max_iter=32),
+    "c5_instances_64": dict(count=64, nmin=5000, nmax=200_000, seed=4, phi=(0.05, 0.15)),
 }
+
+
+def instances(count: int = 64, nmin: int = 5000, nmax: int = 200_000, seed: int = 4,
+              phi_range=(0.05, 0.15)):
+    """Independent suspensions of the many-instance workload: N log-uniform
+    in [nmin, nmax] and volume fraction uniform in phi_range per instance
+    (drawn from one PCG64 stream seeded with ``seed``), each a uniform cube
+    with its own seed (seed * 1000 + k).  Returns (positions of all instances
+    concatenated, per-instance counts)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count)).astype(np.int64)
+    phis = rng.uniform(phi_range[0], phi_range[1], size=count)
+    xs = [uniform_cube(int(ns[k]), float(phis[k]), seed * 1000 + k) for k in range(count)]
+    return np.ascontiguousarray(np.concatenate(xs, axis=0)), ns
 
 
 def positions(name: str, n: int | None = None) -> np.ndarray:

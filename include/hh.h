@@ -44,6 +44,18 @@ int hh_finalize(void);
 
 /* HignnModel(coord, blockSize) -- src/HignnModel.cpp:335-386. */
 int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **out);
+
+/* Many independent suspensions in one model (no reference counterpart; the
+ * reference runs one system per HignnModel).  Instance k holds rows
+ * [o_k, o_k + counts[k]) of coord (o_k the sum of the earlier counts); each
+ * instance gets its own cluster tree and close/far lists (built on host
+ * threads, one instance per thread), and no block couples two instances.
+ * The merged lists run as one near-field launch and one far field over all
+ * instances, so u of every instance equals that of its own model.  Every
+ * other call works as for a single system on the concatenated arrays
+ * (tree export: instance k's node ids follow those of instances 0..k-1). */
+int hh_model_create_instances(const float *coord, const int64_t *counts, int64_t ninst,
+                              int block_size, hh_model **out);
 void hh_model_destroy(hh_model *m);
 
 /* load_two_body_model(path) -- src/HignnModel.cpp:402-433.  The host layer

@@ -1,0 +1,216 @@
+"""The benchmark configurations C1-C4 against committed oracle fixtures
+(tests/golden/*.npz, written by tests/golden/make_fixtures.py), and C3 at its
+full size through size-independent checks (GPU).
+
+Bars: the plain FP32 path within 1e-5 (relative L2) of the oracle's
+velocities with the tree and block lists bit-exact and the far ranks equal;
+the 16-bit forms within 1e-3 of the FP64 evaluation of the same blocks.
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+from hignn_amd import abi, weights, workloads
+from oracle import forcing as OF
+from oracle import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+_spec = importlib.util.spec_from_file_location("make_fixtures", os.path.join(GOLD, "make_fixtures.py"))
+MF = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(MF)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def fixture(name):
+    d = np.load(os.path.join(GOLD, f"{name}.npz"), allow_pickle=False)
+    x, leaf, eps, mi, mrc, spec = MF.setup(name)
+    np.testing.assert_array_equal(MF.digest(x), d["x_sha256"])  # the recipe is unchanged
+    assert str(d["weights"]) == f"{spec[0]}:{spec[1]}"
+    return d, x, leaf, eps, mi, mrc, MF.weight_set(spec)
+
+
+def model(x, leaf, eps, mi, mrc, w, prec="fp32"):
+    m = abi.Model(x, leaf)
+    m.load_weights(w)
+    m.set(epsilon=eps, max_iter=mi, max_relative_coord=mrc, precision=prec)
+    m.update_coord()
+    return m
+
+
+@pytest.fixture(scope="module")
+def surrogate():
+    """The benchmark's surrogate recipe (|r| in [2, 650]) on 4M samples of
+    the smooth weight set, trained on the GPU."""
+    m = abi.Model(workloads.uniform_cube(200, 0.05, 0), 64)
+    m.load_weights(weights.smooth(0))
+    n = 1 << 22
+    r, t = m.surrogate_samples(n, 2.0, 650.0, seed=1)
+    s = abi.Surrogate(2.0, 650.0)
+    s.set_params(S.init(0))
+    s.train(r.ptr, t.ptr, n, 40000, 3e-3, batch=16384, lr_final=3e-5)
+    return s
+
+
+def check_fp32(name, f=None):
+    d, x, leaf, eps, mi, mrc, w = fixture(name)
+    m = model(x, leaf, eps, mi, mrc, w)
+    e = m.export()
+    for k in ("perm", "tree", "close", "far", "leaves"):
+        np.testing.assert_array_equal(e[k], d[k], err_msg=k)
+    f = d["f"] if f is None else f
+    u = m.dot(f)
+    err = rel(u, d["u"])
+    ranks = m.far_ranks()
+    same = float(np.mean(ranks == d["ranks"])) if ranks.size else 1.0
+    print(f"{name}: N={x.shape[0]}, {len(d['close'])} close, {len(d['far'])} far blocks; FP32 vs "
+          f"oracle {err:.2e}; ranks equal {100 * same:.1f}% (mean {ranks.mean():.1f})")
+    assert err <= 1e-5
+    # tolerance stops near the threshold may move by the FP32 pair-function
+    # rounding (about 1e-7 relative against the oracle's summation order)
+    assert same >= 0.97
+    return d, x, m, f
+
+
+def check_16bit(name, m, f, surrogate, forms=(("bf16", True), ("fp16", False))):
+    x_n = m.n
+    tg = np.arange(0, x_n, max(1, x_n // 1500))
+    ref = m.check_fp64(f, tg)
+    out = {}
+    for prec, sur in forms:
+        m.set(precision=prec)
+        m.set_surrogate(surrogate if sur else None)
+        e = rel(m.dot(f)[tg], ref)
+        st = m.surrogate_stats() if sur else None
+        out[(prec, sur)] = e
+        print(f"{name}: {prec}{'+surrogate' if sur else ''} vs FP64 {e:.2e}"
+              + (f" (blocks on surrogate {st[1]} of {st[0]})" if sur else ""))
+        assert e <= 1e-3
+    m.set_surrogate(None)
+    return out
+
+
+def test_c1_matvec_lists_and_euler_steps():
+    """C1 exactly: N = 2,000, one matvec against the oracle, then 10 explicit
+    Euler steps (dt 0.005) with the device-resident integrator, each step
+    rebuilding the tree, against the oracle integrator's positions."""
+    from hignn_amd import hignn as h
+    d, x, m, f = check_fp32("c1")
+    _, _, leaf, eps, mi, mrc, w = fixture("c1")
+    h.Init()
+    hm = h.HignnModel(x, leaf)
+    hm.load_two_body_weights(w)
+    hm.set_epsilon(eps)
+    hm.set_max_iter(mi)
+    fc = h.DeviceForcing()
+    fc.set_body_force([0.0, 0.0, -1.0])
+    it = h.ExplicitEuler()
+    it.set_time_step(0.005)
+    it.set_final_time(0.05)
+    it.set_num_rigid_body(x.shape[0])
+    it.set_device_velocity(hm, fc)
+    state = np.zeros((x.shape[0], 6), np.float32)
+    state[:, :3] = x
+    it.initialize(state)
+    it.run()
+    xf = it.positions()
+    assert it.func_count() == int(d["euler_calls"]) == 10
+    moved = np.abs(d["x_euler"] - x).max()
+    diff = np.abs(xf - d["x_euler"]).max()
+    print(f"c1 Euler: 10 steps, particles moved up to {moved:.3e}, max |x - x_oracle| {diff:.2e}")
+    assert diff <= 1e-5 * moved
+
+
+def test_c2_matvec(surrogate):
+    d, x, m, f = check_fp32("c2")
+    check_16bit("c2", m, f, surrogate)
+
+
+def test_c2_dopri_100_steps():
+    """The C2 recipe at N = 500 (leaf 32): 100 steps of the reference's
+    "Rk4" (Dormand-Prince 5(4), dt 0.002) with the device-resident
+    integrator against the oracle integrator driven by the oracle's matvec."""
+    from hignn_amd import hignn as h
+    d, x, leaf, eps, mi, mrc, w = fixture("c2_dopri")
+    h.Init()
+    hm = h.HignnModel(x, leaf)
+    hm.load_two_body_weights(w)
+    hm.set_epsilon(eps)
+    hm.set_max_iter(mi)
+    fc = h.DeviceForcing()
+    fc.set_body_force([0.0, 0.0, -1.0])
+    it = h.ExplicitRk4()
+    it.set_time_step(0.002)
+    it.set_final_time(0.2)
+    it.set_num_rigid_body(x.shape[0])
+    it.set_device_velocity(hm, fc)
+    state = np.zeros((x.shape[0], 6), np.float32)
+    state[:, :3] = x
+    it.initialize(state)
+    it.run()
+    xf = it.positions()
+    moved = np.abs(d["x_final"] - x).max()
+    diff = np.abs(xf - d["x_final"]).max()
+    print(f"c2 DOPRI: {int(d['steps'])} steps, {it.func_count()} velocity calls "
+          f"(oracle {int(d['calls'])}), moved up to {moved:.3e}, max |x - x_oracle| {diff:.2e}")
+    assert it.func_count() == int(d["calls"])
+    assert diff <= 1e-5 * moved
+
+
+def test_c3_matvec(surrogate):
+    d, x, m, f = check_fp32("c3")
+    check_16bit("c3", m, f, surrogate)
+
+
+def test_c4_forces_and_matvec(surrogate):
+    """C4 recipe at 200 filaments x 50 beads: the device forcing (bonded
+    Morse D 1, alpha 1, r0 2.2, plus gravity) against the oracle's, then the
+    matvec on those forces."""
+    from hignn_amd import hignn as h
+    d, x, leaf, eps, mi, mrc, w = fixture("c4")
+    h.Init()
+    fc = h.DeviceForcing()
+    fc.set_body_force([0.0, 0.0, -1.0])
+    fc.set_bonds(200, 50, 1.0, 1.0, 2.2)
+    fd = fc.eval(x)
+    want = OF.velocity_force(x, b=(0, 0, -1), bonds=OF.bond_forces(x, 200, 50, 1.0, 1.0, 2.2))
+    np.testing.assert_array_equal(want, d["f"])
+    ef = np.abs(fd - want).max() / np.abs(want).max()
+    print(f"c4 forcing: max rel diff {ef:.2e}")
+    assert ef <= 1e-6
+    _, _, m, f = check_fp32("c4", fd)
+    check_16bit("c4", m, f, surrogate)
+
+
+def test_c3_full_size(surrogate):
+    """C3 at N = 1,000,000 (the benchmark's configuration): the chosen form
+    (BF16 + surrogate) and FP16 with full-MLP far entries within 1e-3 of the
+    FP64 evaluation at 600 random targets; ranks within [1, max_iter]; every
+    far block judged."""
+    x = workloads.positions("c3_blobs_1m")
+    n = x.shape[0]
+    f = workloads.gravity(n)
+    m = model(x, 128, 1e-4, 60, 500000, weights.smooth(0), prec="bf16")
+    tg = np.sort(np.random.default_rng(11).choice(n, 600, replace=False))
+    ref = m.check_fp64(f, tg)
+    for prec, sur in (("bf16", True), ("fp16", False)):
+        m.set(precision=prec)
+        m.set_surrogate(surrogate if sur else None)
+        u = m.dot(f)
+        e = rel(u[tg], ref)
+        ranks = m.far_ranks()
+        print(f"c3 full: {prec}{'+surrogate' if sur else ''} vs FP64 {e:.2e}, ranks "
+              f"{ranks.min()}..{ranks.max()} (mean {ranks.mean():.1f})")
+        assert np.isfinite(u).all()
+        assert e <= 1e-3
+        assert ranks.min() >= 1 and ranks.max() <= 60
+        if sur:
+            st = m.surrogate_stats()
+            assert st[0] == ranks.size and st[1] + st[2] == st[0] and st[1] > 0.9 * st[0]
+    m.set_surrogate(None)
