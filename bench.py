@@ -651,7 +651,8 @@ def matvec_bench(args):
     form = (f"{args.prec} MLP on MFMA (FP32 accumulate) for the near field"
             + (" and the far blocks the judge rejects; far ACA entries of the other blocks from "
                "the FP32 surrogate (3-32-32-6 SiLU)" if use_sur else "; far entries from the full MLP")
-            + "; ACA factors stored in BF16 (FP32 arithmetic), FP64 ACA norms and velocity sums")
+            + ("; ACA factors stored in FP32" if use_sur else "; ACA factors stored in BF16")
+            + " (FP32 arithmetic), FP64 ACA norms and velocity sums")
     line = {
         "metric": "matvec particle-updates/s @N=1M: BF16+surrogate vs plain FP32, rel err<=1e-3",
         "value": round(value, 2),

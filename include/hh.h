@@ -29,8 +29,9 @@ extern "C" {
 
 /* precision modes (hh_model_set_precision) */
 #define HH_PREC_FP32 0 /* plain: FP32 MLP, FP64 ACA */
-#define HH_PREC_BF16 1 /* BF16 MLP operands, FP32 accumulate, FP32 ACA factors */
-#define HH_PREC_FP16 2 /* FP16 MLP operands, FP32 accumulate, FP32 ACA factors */
+#define HH_PREC_BF16 1 /* BF16 MLP operands, FP32 accumulate; ACA arithmetic in FP32 on
+                          factors stored in BF16 (FP32 with a surrogate attached) */
+#define HH_PREC_FP16 2 /* FP16 MLP operands, otherwise as HH_PREC_BF16 */
 
 typedef struct hh_model hh_model;
 
