@@ -140,6 +140,7 @@ struct AcaSurLaunch {
   float *Y;
   const float *params;
   float inv_scale;
+  int cus;
 };
 
 // Surrogate judge of nblk far blocks: samples pairs per block, the full FP32
@@ -151,7 +152,7 @@ struct JudgeLaunch {
   int64_t nblk;
   const float4 *xp;
   int samples;
-  float *r, *m9;
+  float *r, *m9, *s6;  // workspace: samples' displacements, full-MLP blocks, surrogate outputs
   const float *params;
   float r_lo, r_hi;
   double tol;

@@ -625,36 +625,135 @@ __global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, 
 }
 
 // ---- surrogate-generated ACA entries -------------------------------------
-// The tiles of blocks that passed the surrogate judge: one thread per item,
-// Y receives the symmetric block s(r)/|r| in the same row-major layout as
-// the full MLP's entries (the step kernels' symmetrisation leaves it
+// The tiles of blocks that passed the surrogate judge.  One wave per 16-item
+// tile; the surrogate's three layers run on v_mfma_f32_16x16x4f32 (FP32
+// products and sums: the ACA needs entries without 16-bit noise), with the
+// tile's items on the column axis and the hidden features on the row axis,
+// so each layer's output registers are the next layer's B operand:
+//   layer 1: A = [W1 | b1] rows 16t..16t+15 (lane g, p: W1[16t+p][g], b1 for
+//            g = 3), B = (z, 1) of item p; D lane (g, p) register r =
+//            feature 16t + 4g + r, t = 0, 1;
+//   layer 2: K-step (t, r) takes B = SiLU of layer-1 register r of tile t,
+//            i.e. feature 16t + 4g + r, so its A operand is W2[16mt + p][16t
+//            + 4g + r]; the bias enters as C;
+//   layer 3: the same with W3 (rows 6..15 zero); output 4g + r of item p
+//            lands in lane (g, p), register r.
+// Each lane gathers its 26 weight fragments and 12 biases from the
+// parameters once.  Y receives the symmetric block s(r)/|r| in the full
+// MLP's row-major layout (the step kernels' symmetrisation leaves it
 // unchanged).  Same displacement convention as aca_mlp_body: x_j - x_i.
+struct SurMfma {
+  float w1[2], w2[2][2][4], w3[2][4];
+  f32x4 b2v[2], b3v;
+
+  // The lane's weight fragments and biases, gathered from the parameters.
+  __device__ __forceinline__ void init(const float *params, int g, int p) {
+    using namespace sur;
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+      w1[t] = g < 3 ? params[kOW1 + (16 * t + p) * kIn + g] : params[kOB1 + 16 * t + p];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) w2[mt][t][r] = params[kOW2 + (16 * mt + p) * kH + 16 * t + 4 * g + r];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) w3[t][r] = p < kOut ? params[kOW3 + p * kH + 16 * t + 4 * g + r] : 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+      b2v[mt] = f32x4{params[kOB2 + 16 * mt + 4 * g], params[kOB2 + 16 * mt + 4 * g + 1],
+                      params[kOB2 + 16 * mt + 4 * g + 2], params[kOB2 + 16 * mt + 4 * g + 3]};
+#pragma unroll
+    for (int r = 0; r < 4; r++) b3v[r] = 4 * g + r < kOut ? params[kOB3 + 4 * g + r] : 0.0f;
+  }
+
+  // Outputs 4g + r (r = 0..3) of the tile's item p; bz is z_g of item p
+  // (g < 3) or 1 (g = 3).  Every lane of the wave takes part.
+  __device__ __forceinline__ f32x4 eval(float bz) const {
+    f32x4 a1[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      a1[t] = mma(w1[t], bz, f32x4{0, 0, 0, 0});
+#pragma unroll
+      for (int r = 0; r < 4; r++) a1[t][r] = sur::silu(a1[t][r]);
+    }
+    f32x4 a2[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+      f32x4 c = b2v[mt];
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) c = mma(w2[mt][t][r], a1[t][r], c);
+#pragma unroll
+      for (int r = 0; r < 4; r++) c[r] = sur::silu(c[r]);
+      a2[mt] = c;
+    }
+    f32x4 y = b3v;
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) y = mma(w3[t][r], a2[t][r], y);
+    return y;
+  }
+};
+
 __global__ __launch_bounds__(256) void aca_sur_kernel(const float4 *xp, const AcaPassTile *tiles,
                                                       int64_t ntiles, int row_pass, float *Y,
                                                       const float *params, float inv_scale) {
-  __shared__ float P[sur::kNP];
-  sur::stage(params, P);
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t q = t >> 4;
-  const int p = (int)(t & 15);
-  if (q >= ntiles) return;
-  const AcaPassTile T = tiles[q];
-  if (p >= T.nvalid) return;
-  const float4 xi = xp[T.item0 + p], xv = xp[T.piv];
-  float rx, ry, rz;
-  if (!row_pass) {
-    rx = xv.x - xi.x, ry = xv.y - xi.y, rz = xv.z - xi.z;
-  } else {
-    rx = xi.x - xv.x, ry = xi.y - xv.y, rz = xi.z - xv.z;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, p = lane & 15;
+  SurMfma S;
+  S.init(params, g, p);
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < ntiles; q += nwaves) {
+    const AcaPassTile T = tiles[q];
+    const bool valid = p < T.nvalid;
+    const float4 xi = xp[T.item0 + (valid ? p : 0)], xv = xp[T.piv];
+    float rx, ry, rz;
+    if (!row_pass) {
+      rx = xv.x - xi.x, ry = xv.y - xi.y, rz = xv.z - xi.z;
+    } else {
+      rx = xi.x - xv.x, ry = xi.y - xv.y, rz = xi.z - xv.z;
+    }
+    const float bz = g == 0 ? rx * inv_scale : g == 1 ? ry * inv_scale : g == 2 ? rz * inv_scale : 1.0f;
+    const f32x4 y = S.eval(bz);
+    if (!valid || g > 1) continue;
+    const float inv = rsqrt_r(rx * rx + ry * ry + rz * rz);
+    float *o = Y + 9 * (T.yrow0 + p);
+    if (g == 0) {
+      o[0] = y[0] * inv, o[4] = y[1] * inv, o[8] = y[2] * inv;
+      o[1] = o[3] = y[3] * inv;
+    } else {
+      o[5] = o[7] = y[0] * inv;
+      o[2] = o[6] = y[1] * inv;
+    }
   }
-  float y[sur::kOut];
-  sur::eval(P, rx, ry, rz, inv_scale, y);
-  const float inv = rsqrt_r(rx * rx + ry * ry + rz * rz);
-  float *o = Y + 9 * (T.yrow0 + p);
-  o[0] = y[0] * inv, o[4] = y[1] * inv, o[8] = y[2] * inv;
-  o[1] = o[3] = y[3] * inv;
-  o[5] = o[7] = y[4] * inv;
-  o[2] = o[6] = y[5] * inv;
+}
+
+// Surrogate outputs (n,6) for n displacements, 16 per wave on the MFMA path
+// of aca_sur_kernel (the judge's samples).
+__global__ __launch_bounds__(256) void sur_pairs_kernel(const float *r, int64_t n, float *out,
+                                                        const float *params, float inv_scale) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, p = lane & 15;
+  SurMfma S;
+  S.init(params, g, p);
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; 16 * q < n; q += nwaves) {
+    const int64_t k = 16 * q + p;
+    const bool valid = k < n;
+    const int64_t kc = valid ? k : n - 1;
+    const float bz = g < 3 ? r[3 * kc + g] * inv_scale : 1.0f;
+    const f32x4 y = S.eval(bz);
+    if (!valid || g > 1) continue;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+      if (4 * g + rr < sur::kOut) out[sur::kOut * k + 4 * g + rr] = y[rr];
+  }
 }
 
 // ---- surrogate judge --------------------------------------------------------
@@ -688,15 +787,13 @@ __global__ void judge_pairs_kernel(const FarBlock *blk, int64_t nblk, const floa
 // as the ACA uses it); the block passes when the sampled Frobenius error is
 // at most tol times the sampled Frobenius norm and every sampled |r| lies in
 // [r_lo, r_hi].  err[b] receives the relative error (inf when out of range).
-__global__ __launch_bounds__(256) void judge_kernel(const float *r, const float *m9, int64_t nblk,
-                                                    int samples, const float *params, float r_lo,
+__global__ __launch_bounds__(256) void judge_kernel(const float *r, const float *m9, const float *s6,
+                                                    int64_t nblk, int samples, float r_lo,
                                                     float r_hi, double tol, int32_t *flag,
                                                     float *err) {
-  __shared__ float P[sur::kNP];
-  sur::stage(params, P);
   const int lane = threadIdx.x & 63;
-  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (b >= nblk) return;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; b < nblk; b += nwaves) {
   double e2 = 0.0, n2 = 0.0;
   int out = 0;
   for (int t = lane; t < samples; t += 64) {
@@ -704,8 +801,7 @@ __global__ __launch_bounds__(256) void judge_kernel(const float *r, const float 
     const float rx = r[3 * k], ry = r[3 * k + 1], rz = r[3 * k + 2];
     const float d = sqrtf(rx * rx + ry * ry + rz * rz);
     if (!(d >= r_lo && d <= r_hi)) out = 1;
-    float y[sur::kOut];
-    sur::eval(P, rx, ry, rz, 1.0f / r_hi, y);
+    const float *y = s6 + sur::kOut * k;
     const float *M = m9 + 9 * k;
 #pragma unroll
     for (int o = 0; o < sur::kOut; o++) {
@@ -725,6 +821,7 @@ __global__ __launch_bounds__(256) void judge_kernel(const float *r, const float 
     const bool ok = !out && rel <= tol;
     flag[b] = ok ? 1 : 0;
     err[b] = out ? INFINITY : (float)rel;
+  }
   }
 }
 
@@ -2051,8 +2148,8 @@ hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
 
 hipError_t launch_aca_sur(const AcaSurLaunch &L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
-  const int64_t threads = 16 * L.ntiles;
-  hipLaunchKernelGGL(aca_sur_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+  const int64_t blocks = std::min<int64_t>((L.ntiles + 3) / 4, 16 * (int64_t)L.cus);
+  hipLaunchKernelGGL(aca_sur_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
                      L.xp, (const AcaPassTile *)L.tiles + L.tile0, L.ntiles, L.row_pass, L.Y,
                      L.params, L.inv_scale);
   return hipGetLastError();
@@ -2065,9 +2162,12 @@ hipError_t launch_judge(const JudgeLaunch &L, hipStream_t st) {
                      L.blk, L.nblk, L.xp, L.samples, L.r);
   hipError_t e = launch_pair(PREC_FP32, L.r, k, L.m9, L.wf, L.wh, L.cus, st);
   if (e != hipSuccess) return e;
+  const int64_t sblocks = std::min<int64_t>((k + 63) / 64, 16 * (int64_t)L.cus);
+  hipLaunchKernelGGL(sur_pairs_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, (const float *)L.r,
+                     k, L.s6, L.params, 1.0f / L.r_hi);
   hipLaunchKernelGGL(judge_kernel, dim3((unsigned)((L.nblk + 3) / 4)), dim3(256), 0, st, L.r,
-                     (const float *)L.m9, L.nblk, L.samples, L.params, L.r_lo, L.r_hi, L.tol,
-                     L.flag, L.err);
+                     (const float *)L.m9, (const float *)L.s6, L.nblk, L.samples, L.r_lo, L.r_hi,
+                     L.tol, L.flag, L.err);
   return hipGetLastError();
 }
 

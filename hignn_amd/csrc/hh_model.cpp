@@ -160,7 +160,7 @@ struct hh_model {
   hh_surrogate *sur = nullptr;
   double sur_tol = 2e-3;
   int sur_samples = 32;
-  DBuf jd_blk, jd_r, jd_m9, jd_flag, jd_err;
+  DBuf jd_blk, jd_r, jd_m9, jd_s6, jd_flag, jd_err;
   std::vector<int32_t> sur_flag;  // per far block: 1 = surrogate entries
   std::vector<float> sur_err;     // per far block: sampled relative error
   // [judged, on surrogate, fell back, surrogate entries, full-MLP far entries]
@@ -168,7 +168,7 @@ struct hh_model {
 
   ~hh_model() {
     for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y, &jd_blk, &jd_r, &jd_m9,
-                    &jd_flag, &jd_err})
+                    &jd_s6, &jd_flag, &jd_err})
       b->release();
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
                     &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
@@ -602,6 +602,7 @@ struct hh_model {
     upload(jd_blk, far.data(), far.size() * sizeof(FarBlock));
     jd_r.reserve((size_t)nfar * S * 12);
     jd_m9.reserve((size_t)nfar * S * 36);
+    jd_s6.reserve((size_t)nfar * S * 24);
     jd_flag.reserve((size_t)nfar * 4);
     jd_err.reserve((size_t)nfar * 4);
     JudgeLaunch L{};
@@ -611,6 +612,7 @@ struct hh_model {
     L.samples = S;
     L.r = jd_r.as<float>();
     L.m9 = jd_m9.as<float>();
+    L.s6 = jd_s6.as<float>();
     L.params = sur->params.as<float>();
     L.r_lo = sur->r_lo;
     L.r_hi = sur->r_hi;
@@ -819,6 +821,7 @@ struct hh_model {
         AcaSurLaunch su{};
         su.xp = xp.as<float4>();
         su.Y = Y.as<float>();
+        su.cus = g_cus;
         if (sur) su.params = sur->params.as<float>(), su.inv_scale = 1.0f / sur->r_hi;
         ml.row_pass = 0;
         ml.ybase = ybase_c.as<int64_t>();
