@@ -40,9 +40,10 @@ MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
 # HBM bytes per pair-function evaluation (read + written), measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this benchmark at N=1M, each
 # scaled by the counter's reading for a 1 GiB device copy (x2.0 and x1.0):
-# the ACA passes write 36 B of entries and read 9.5 B; the near kernel's
-# traffic is mostly its FP64 velocity atomics.
-TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": 7.25}
+# the near kernel reads 0.63 B and writes 7.49 B per evaluation (its FP64
+# velocity atomics, as the L2 issues them); the ACA passes of the full MLP
+# (leaf 256, FP16, earlier setting) wrote 36 B of entries and read 9.5 B.
+TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": 8.12}
 PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # dense MFMA TFLOP/s
 
 
