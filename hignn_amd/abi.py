@@ -45,6 +45,7 @@ def lib():
             "hh_model_set_precision": [c_vp, c_int],
             "hh_model_update_coord": [c_vp, c_vp, c_i64],
             "hh_model_set_tree_reuse": [c_vp, c_int],
+            "hh_model_set_tree_build": [c_vp, c_int],
             "hh_model_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dense_dot": [c_vp, c_vp, c_vp, c_i64],
             "hh_model_dot_device": [c_vp, c_vp, c_vp, c_i64, c_vp],
@@ -80,6 +81,7 @@ def lib():
             "hh_model_last_timings": [c_vp, c_vp],
             "hh_model_last_profile": [c_vp, c_vp, c_vp, c_vp],
             "hh_tree_build": [c_vp, c_i64, c_int, c_int, c_i64, c_vp],
+            "hh_tree_build_device": [c_vp, c_vp, c_i64, c_int, c_int, c_i64, c_vp, c_vp],
             "hh_tree_counts": [c_vp, c_vp],
             "hh_tree_export": [c_vp] * 7,
         }
@@ -134,6 +136,22 @@ def tree_build(x, leaf, use_symmetry=True, max_relative_coord=500000):
         lib().hh_tree_free(h)
 
 
+def tree_build_device(x, leaf, use_symmetry=True, max_relative_coord=500000, counts=None):
+    """Tree + block lists built on the device from positions uploaded to it;
+    counts: particles per independent instance (consecutive rows of x)."""
+    x = _f32(x)
+    counts = np.asarray([x.shape[0]] if counts is None else counts, dtype=np.int64)
+    xd = DeviceArray.of(x)
+    h = c_vp()
+    _chk(lib().hh_tree_build_device(xd.ptr, _p(counts), counts.size, int(leaf),
+                                    int(bool(use_symmetry)), int(max_relative_coord), None,
+                                    ctypes.byref(h)))
+    try:
+        return _export(lib().hh_tree_counts, lib().hh_tree_export, h, x.shape[0])
+    finally:
+        lib().hh_tree_free(h)
+
+
 class Model:
     """Thin object wrapper over an hh_model handle."""
 
@@ -167,8 +185,10 @@ class Model:
         _chk(lib().hh_model_load_two_body(self.h, _p(w), w.size))
 
     def set(self, epsilon=None, max_iter=None, use_symmetry=None, max_relative_coord=None,
-            precision=None, tree_reuse=None):
+            precision=None, tree_reuse=None, tree_build=None):
         L = lib()
+        if tree_build is not None:
+            _chk(L.hh_model_set_tree_build(self.h, {"host": 0, "device": 1}[tree_build]))
         if tree_reuse is not None:
             _chk(L.hh_model_set_tree_reuse(self.h, int(bool(tree_reuse))))
         if epsilon is not None:

@@ -36,12 +36,22 @@ def _run(cmd):
     return out
 
 
-def build_libhh():
-    srcs = [os.path.join(CSRC, f) for f in ("hh_kernels.hip", "hh_forcing.hip", "hh_three_body.hip",
-                                             "hh_surrogate.hip", "hh_model.cpp", "hh_tree.cpp",
-                                             "hh_dynamics.cpp", "hh_three_body.cpp")]
+SOURCES = ("hh_kernels.hip", "hh_forcing.hip", "hh_three_body.hip", "hh_surrogate.hip",
+           "hh_tree_gpu.hip", "hh_model.cpp", "hh_tree.cpp", "hh_dynamics.cpp", "hh_three_body.cpp")
+
+
+def build_libhh(jobs=None):
+    """Compiles every source to an object (in parallel), then links libhh.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, f + ".o") for f in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, min(os.cpu_count() or 1, 16)))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, so[0]), "-o", so[1]]),
+                    zip(SOURCES, objs)))
     tmp = LIBHH + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", *srcs, "-o", tmp, "-lpthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", *objs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIBHH)
     return LIBHH
 

@@ -29,6 +29,24 @@ struct TreeLists {
 void build_tree_lists(const float *x, int64_t n, int leaf, bool use_symmetry,
                       int64_t max_rel_coord, int nthreads, TreeLists &out);
 
+// The divide step's principal axis (cyclic Jacobi on the FP64 covariance
+// [xx, yy, zz, xy, yz, zx]), shared by the host and the device builds.
+void tree_principal_axis(const double cov[6], double axis[3]);
+
+// The same tree and lists built on the device (hh_tree_gpu.hip) for one or
+// more independent instances: instance k holds the particles
+// [pbase[k], pbase[k+1]) of x_dev (n,3 floats, original order, device
+// memory).  Node ids, particle ranges and list order are those of
+// build_tree_lists per instance, merged as hh_model_create_instances merges
+// them.  Device work buffers live in the builder and are reused.
+struct DeviceTreeBuilder;
+DeviceTreeBuilder *device_tree_builder_new();
+void device_tree_builder_free(DeviceTreeBuilder *b);
+void build_tree_lists_device(DeviceTreeBuilder *b, const float *x_dev,
+                             const std::vector<int64_t> &pbase, int leaf, bool use_symmetry,
+                             int64_t max_rel_coord, void *stream, TreeLists &out,
+                             int32_t *perm_dev_out);
+
 // ---------------------------------------------------------------------------
 // Device-side work descriptors.
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -84,6 +85,11 @@ struct hh_model {
   int64_t max_rel_coord = 500000;
   bool post_check = false, use_symmetry = true;
   bool tree_reuse = false;  // update_coord keeps the last tree and lists (opt-in)
+  // tree and block lists built on the device (hh_tree_gpu.hip) or on host
+  // threads (hh_tree.cpp); both give the same result bit for bit
+  bool tree_device = false;
+  DeviceTreeBuilder *tbuild = nullptr;
+  DBuf xdev;  // coordinates (n,3) uploaded for a device build from host data
   uint64_t builds = 0, loads = 0;  // tree builds and weight loads (graph signatures)
   // independent instances (hh_model_create_instances): particle counts, in
   // row order of x; empty for a single system
@@ -167,6 +173,8 @@ struct hh_model {
   int64_t sur_stats[5] = {0, 0, 0, 0, 0};
 
   ~hh_model() {
+    if (tbuild) device_tree_builder_free(tbuild);
+    xdev.release();
     for (auto *b : {&p_blk, &p_order, &p_rank, &p_poolC, &p_poolQ, &p_Y, &jd_blk, &jd_r, &jd_m9,
                     &jd_s6, &jd_flag, &jd_err})
       b->release();
@@ -336,8 +344,10 @@ struct hh_model {
     x_stale = false;
   }
 
-  void update() {
-    if (!(tree_reuse && built)) refresh_host_x();
+  // xdev_in: the new coordinates (n,3, original order) in device memory, or
+  // NULL when they are in x
+  void update(const float *xdev_in = nullptr) {
+    if (!(tree_reuse && built) && !xdev_in) refresh_host_x();
     if (tree_reuse && built) {
       // keep the permutation, tree and block lists of the last build; only
       // the positions (in that tree order) are refreshed
@@ -350,22 +360,45 @@ struct hh_model {
       upload(xp, xph.data(), xph.size() * 4);
       return;
     }
-    if (inst_counts.size() > 1)
-      build_instances();
-    else
-      build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
-    builds++;
-    // positions in tree order
-    std::vector<float> xph(4 * (size_t)n);
-    std::vector<int32_t> ph((size_t)n);
-    for (int64_t i = 0; i < n; i++) {
-      const int64_t s = tl.perm[(size_t)i];
-      ph[(size_t)i] = (int32_t)s;
-      for (int d = 0; d < 3; d++) xph[4 * (size_t)i + d] = x[3 * (size_t)s + d];
-      xph[4 * (size_t)i + 3] = 0.0f;
+    if (tree_device) {
+      const float *xd = xdev_in;
+      if (!xd) {
+        xdev.reserve(12 * (size_t)n);
+        HIPCHK(hipMemcpy(xdev.p, x.data(), 12 * (size_t)n, hipMemcpyHostToDevice));
+        xd = xdev.as<float>();
+      }
+      std::vector<int64_t> pb{0};
+      if (inst_counts.size() > 1)
+        for (int64_t c : inst_counts) pb.push_back(pb.back() + c);
+      else
+        pb.push_back(n);
+      if (!tbuild) tbuild = device_tree_builder_new();
+      perm.reserve(4 * (size_t)n);
+      xp.reserve(16 * (size_t)n);
+      build_tree_lists_device(tbuild, xd, pb, leaf, use_symmetry, max_rel_coord, stream, tl,
+                              perm.as<int32_t>());
+      // positions in tree order
+      HIPCHK(launch_gather_forces(xd, n, perm.as<int32_t>(), xp.as<float4>(), stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (xdev_in) x_stale = true;  // the host copy catches up when a host path needs it
+    } else {
+      if (inst_counts.size() > 1)
+        build_instances();
+      else
+        build_tree_lists(x.data(), n, leaf, use_symmetry, max_rel_coord, 0, tl);
+      // positions in tree order
+      std::vector<float> xph(4 * (size_t)n);
+      std::vector<int32_t> ph((size_t)n);
+      for (int64_t i = 0; i < n; i++) {
+        const int64_t s = tl.perm[(size_t)i];
+        ph[(size_t)i] = (int32_t)s;
+        for (int d = 0; d < 3; d++) xph[4 * (size_t)i + d] = x[3 * (size_t)s + d];
+        xph[4 * (size_t)i + 3] = 0.0f;
+      }
+      upload(xp, xph.data(), xph.size() * 4);
+      upload(perm, ph.data(), ph.size() * 4);
     }
-    upload(xp, xph.data(), xph.size() * 4);
-    upload(perm, ph.data(), ph.size() * 4);
+    builds++;
     // near-field row blocks and tasks (src/CloseDot.cpp)
     std::vector<RowBlock> rb;
     std::vector<NearTask> tk;
@@ -1239,6 +1272,7 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
     }
     if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
+    if (const char *tb = std::getenv("HH_TREE_BUILD")) m->tree_device = std::strcmp(tb, "device") == 0;
     if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
     if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
@@ -1338,11 +1372,26 @@ int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n,
       m->x_stale = true;  // the host copy catches up before the next rebuild
       return;
     }
+    if (m->tree_device) {
+      // build from the device coordinates; the host copy is refreshed from
+      // xp when a host path needs it
+      HIPCHK(hipStreamSynchronize(st));
+      m->update(coord_dev);
+      return;
+    }
     m->x.resize(3 * (size_t)n);
     HIPCHK(hipMemcpyAsync(m->x.data(), coord_dev, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     m->x_stale = false;
     m->update();
+  });
+}
+
+int hh_model_set_tree_build(hh_model *m, int device) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (device != 0 && device != 1) throw HHError(HH_ERR_ARG, "tree build: 0 host, 1 device");
+    m->tree_device = device != 0;
   });
 }
 
@@ -1563,6 +1612,38 @@ int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int
   return guarded([&]() {
     if (!t) throw HHError(HH_ERR_ARG, "NULL tree");
     export_lists(t->tl, tree5, perm, aux6, close_ij, far_ij, leaves);
+  });
+}
+
+int hh_tree_build_device(const float *x_dev, const int64_t *counts, int64_t ninst, int block_size,
+                         int use_symmetry, int64_t max_relative_coord, void *stream, hh_tree **out) {
+  return guarded([&]() {
+    if (!x_dev || !counts || !out) throw HHError(HH_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (ninst <= 0 || block_size <= 0 || max_relative_coord <= 0)
+      throw HHError(HH_ERR_ARG, "need ninst, block_size, max_relative_coord > 0");
+    std::vector<int64_t> pb{0};
+    for (int64_t k = 0; k < ninst; k++) {
+      if (counts[k] <= 0) throw HHError(HH_ERR_ARG, "instance sizes must be > 0");
+      pb.push_back(pb.back() + counts[k]);
+    }
+    if (pb.back() >= ((int64_t)1 << 31)) throw HHError(HH_ERR_ARG, "too many particles");
+    ensure_device();
+    std::unique_ptr<hh_tree> t(new hh_tree());
+    DeviceTreeBuilder *b = device_tree_builder_new();
+    DBuf permd;
+    try {
+      permd.reserve(4 * (size_t)pb.back());
+      build_tree_lists_device(b, x_dev, pb, block_size, use_symmetry != 0, max_relative_coord,
+                              stream, t->tl, permd.as<int32_t>());
+    } catch (...) {
+      permd.release();
+      device_tree_builder_free(b);
+      throw;
+    }
+    permd.release();
+    device_tree_builder_free(b);
+    *out = t.release();
   });
 }
 

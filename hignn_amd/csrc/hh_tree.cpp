@@ -363,6 +363,10 @@ void traverse(TreeLists &tl, bool sym, int64_t R, int nthreads) {
 
 }  // namespace
 
+void tree_principal_axis(const double cov[6], double axis[3]) {
+  dominant_axis(cov[0], cov[1], cov[2], cov[3], cov[4], cov[5], axis);
+}
+
 void build_tree_lists(const float *x, int64_t n, int leaf, bool use_symmetry,
                       int64_t max_rel_coord, int nthreads, TreeLists &out) {
   out = TreeLists();

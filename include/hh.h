@@ -86,6 +86,12 @@ int hh_model_update_coord(hh_model *m, const float *coord, int64_t n);
  * caller decides when to rebuild (flag 0, then update_coord). */
 int hh_model_set_tree_reuse(hh_model *m, int flag);
 
+/* Where update_coord builds the tree and lists (the reference builds them on
+ * the host, src/HignnModel.cpp:457-461): 0 on host threads, 1 on the device
+ * (see hh_tree_build_device; the same result bit for bit).  The environment
+ * variable HH_TREE_BUILD=device|host sets the default of new models. */
+int hh_model_set_tree_build(hh_model *m, int device);
+
 /* dot(u, f) -- src/HignnModel.cpp:545-634: u = M(x) f, host arrays. */
 int hh_model_dot(hh_model *m, float *u, const float *f, int64_t n);
 
@@ -133,6 +139,17 @@ int hh_tree_build(const float *x, int64_t n, int block_size, int use_symmetry,
 int hh_tree_counts(hh_tree *t, int64_t counts[4]);
 int hh_tree_export(hh_tree *t, uint64_t *tree5, uint64_t *perm, float *aux6, int64_t *close_ij,
                    int64_t *far_ij, int64_t *leaves);
+
+/* The same tree and lists built on the device (Build + CloseFarCheck,
+ * src/Build.cpp:17-265, src/CloseFarCheck.cpp:181-454, as HignnModel::Update
+ * runs them, src/HignnModel.cpp:457-461) for ninst independent systems whose
+ * particles are consecutive rows of x_dev ((sum counts, 3) floats, device
+ * memory): instance k's node ids follow those of instances 0..k-1 and its
+ * particle ranges theirs, as hh_model_create_instances merges them.  With
+ * ninst = 1 the result equals hh_tree_build's bit for bit.  stream may be
+ * NULL (the null stream). */
+int hh_tree_build_device(const float *x_dev, const int64_t *counts, int64_t ninst, int block_size,
+                         int use_symmetry, int64_t max_relative_coord, void *stream, hh_tree **out);
 void hh_tree_free(hh_tree *t);
 
 /* Device times (ms) of the last dot: [close, far, total], total from the
