@@ -375,11 +375,9 @@ def instances_bench(args):
     m.load_weights(w)
     m.set(epsilon=args.epsilon, max_iter=args.max_iter,
           max_relative_coord=args.max_relative_coord)
-    t0 = time.time()
-    m.update_coord()
-    t_update = time.time() - t0
+    t_update, t_update_host = time_update_coord(m)
     counts = m.counts()
-    log(f"[instances] update_coord {t_update:.2f}s: {counts}")
+    log(f"[instances] update_coord {t_update:.3f}s (host build {t_update_host:.3f}s): {counts}")
     t0 = time.time()
     r_s, t_s = m.surrogate_samples(args.sur_samples, args.sur_rlo, args.sur_rhi, seed=1)
     sur = abi.Surrogate(args.sur_rlo, args.sur_rhi)
@@ -443,6 +441,7 @@ def instances_bench(args):
             "instances": len(ns), "n_min": int(ns.min()), "n_max": int(ns.max()),
             "tree_rows": counts[0], "close_pairs": counts[1], "far_blocks": counts[2],
             "update_coord_s": round(t_update, 3),
+            "update_coord_host_build_s": round(t_update_host, 3),
             "surrogate_train_s": round(t_train, 2), "surrogate_loss": loss1,
             "surrogate_stats": dict(zip(["far_blocks_judged", "on_surrogate", "fell_back",
                                          "surrogate_entries", "full_mlp_far_entries"], stats)),
@@ -472,6 +471,21 @@ def host_info():
     return {"affinity_cores": cores, "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
+def time_update_coord(model):
+    """update_coord (tree + lists + work descriptors) from host coordinates
+    with the device build (warm: the second call) and with the host build;
+    the model ends on the device build."""
+    model.set(tree_build="host")
+    t0 = time.time()
+    model.update_coord()
+    t_host = time.time() - t0
+    model.set(tree_build="device")
+    model.update_coord()
+    t0 = time.time()
+    model.update_coord()
+    return time.time() - t0, t_host
+
+
 def matvec_bench(args):
     from hignn_amd import abi, weights, workloads
     from oracle import surrogate as S
@@ -488,11 +502,10 @@ def matvec_bench(args):
     model = abi.Model(x, leaf)
     model.load_weights(w)
     model.set(epsilon=eps, max_iter=max_iter, max_relative_coord=mrc)
-    t0 = time.time()
-    model.update_coord()
-    t_update = time.time() - t0
+    t_update, t_update_host = time_update_coord(model)
     exp = model.export()
-    log(f"[bench] update_coord {t_update:.2f}s: {len(exp['close'])} close, {len(exp['far'])} far")
+    log(f"[bench] update_coord {t_update:.3f}s (host build {t_update_host:.3f}s): "
+        f"{len(exp['close'])} close, {len(exp['far'])} far")
     f_dev = abi.DeviceArray.of(f)
     u_dev = abi.DeviceArray((n, 3))
 
@@ -690,6 +703,7 @@ def matvec_bench(args):
             "e_near": e_near, "e_far": e_far,
             "mean_far_rank": float(ranks.mean()) if ranks.size else 0.0,
             "update_coord_s": round(t_update, 3),
+            "update_coord_host_build_s": round(t_update_host, 3),
             "close_ms": round(float(np.median(tcl)), 2), "far_ms": round(float(np.median(tfar)), 2),
             "far_mlp_ms": round(far_mlp_ms, 2),
         },

@@ -33,20 +33,47 @@ struct NearLaunch {
   int cus;         // compute units (grid of the LDS-resident 16-bit kernel)
 };
 
-struct AcaMlpLaunch {
-  int prec;
-  const float4 *xp;
+// Counts of one batched ACA iteration.  They live in device memory, so the
+// kernels run on counts that either the host uploaded or the device plan of
+// the iteration computed (no host round trip per iteration); launch grids
+// come from host-side upper bounds and the kernels loop grid-stride.
+struct AcaCounts {
+  int32_t na;           // active blocks (slots)
+  int32_t nl;           // blocks to apply (stopped in the previous iteration)
+  int32_t ntc[2];       // column-pass 16-item tiles: [0] full MLP, [1] surrogate
+  int32_t ntr[2];       // row-pass tiles
+  int32_t ncc, ncr;     // step chunks over I / J of the active blocks
+  int32_t nci, ncj;     // apply chunks over I / J of the blocks to apply
+  int64_t ysum_c, ysum_r;  // items of the active blocks (rows of Y per pass)
+  int64_t entries[2];   // running totals of pass entries: [0] full MLP, [1] surrogate
+};
+
+// Tile lists of one ACA pass: every active slot writes its ceil(items/16)
+// tiles at toff[slot] of the list of its kind (full-MLP or surrogate
+// entries).  Column pass: pivot x[js + jpiv[k]], items x[is + i]; row pass:
+// pivot x[is + ipiv[k]], items x[js + j].
+struct AcaExpandLaunch {
   const FarBlock *blk;
   const int32_t *active;
   const int32_t *it;
   const int32_t *ipiv;
   const int32_t *jpiv;
   const int64_t *ybase;
-  const int32_t *tfirst;  // slot -> first 16-item tile (nactive + 1)
-  int nactive;
-  void *tiles;            // ntiles resolved tiles (24 B each), filled from tfirst
-  int64_t ntiles;
+  const int32_t *toff;
+  const int32_t *kind;    // per batch block: 1 = surrogate entries; NULL: all full MLP
+  const AcaCounts *cnt;
+  int64_t grid;           // slots bound (grid size)
   int max_iter;
+  int row_pass;
+  void *tiles_full, *tiles_sur;  // 24 B per tile
+};
+
+struct AcaMlpLaunch {
+  int prec;
+  const float4 *xp;
+  const void *tiles;      // resolved tiles (24 B each)
+  const int32_t *dntiles; // device count of the tiles
+  int64_t ntiles;         // bound of the count (grid size)
   int row_pass;
   float *Y;
   DevWeightsF wf;
@@ -80,8 +107,10 @@ struct AcaStepLaunch {
   double *x_c, *x_r;                // max_iter x 9 per chunk
   double *inv;                      // 9 per slot
   int chunk;                        // items per chunk, 1..kAcaChunk
-  int nactive;
+  const AcaCounts *cnt;             // na, ncc, ncr
+  int64_t nactive;                  // bounds of the counts (grid sizes)
   int64_t nchunks_c, nchunks_r;
+  int64_t grid_cap;                 // at most this many workgroups per kernel
   int pool_f64;
   int pool_c16;  // BF16 C pool (16-bit modes)
   int pool_q16;  // BF16 Q pool (16-bit modes)
@@ -96,7 +125,9 @@ struct AcaApplyLaunch {
   const float4 *fp;
   double *u;
   int use_symmetry;
-  int nlist;
+  const AcaCounts *cnt;             // nl, nci, ncj
+  int64_t nlist;                    // bounds of the counts (grid sizes)
+  int64_t grid_cap;
   int pool_f64;
   int pool_c16;
   int pool_q16;
@@ -130,12 +161,13 @@ struct PAcaLaunch {
   DevWeightsH wh;
 };
 
-// Surrogate entries for tiles [tile0, tile0 + ntiles) of an expanded ACA
-// pass tile list (blocks that passed the judge).
+// Surrogate entries for the surrogate tile list of an ACA pass (blocks that
+// passed the judge).
 struct AcaSurLaunch {
   const float4 *xp;
   const void *tiles;
-  int64_t tile0, ntiles;
+  const int32_t *dntiles;  // device count of the tiles
+  int64_t ntiles;          // bound of the count (grid size)
   int row_pass;
   float *Y;
   const float *params;
@@ -172,6 +204,40 @@ hipError_t launch_sur_targets(const float *r, const float *m9, int64_t n, float 
                               hipStream_t st);
 hipError_t launch_persistent_aca(const PAcaLaunch &L, hipStream_t st);
 hipError_t launch_near(const NearLaunch &L, hipStream_t st);
+// Device plan of one batched ACA iteration: from the previous active list
+// and the stop flags its finalize step wrote (or, for the first plan, all nb
+// blocks of the batch), the new active list with its per-slot prefix counts
+// (Y rows, tile offsets per kind, step chunks) and the list of blocks that
+// stopped, with their apply chunk offsets; counts into cnt_out.  The order of
+// the input list is kept.  Three launches (group sums, one scan of the
+// groups, ordered write), no host involvement.
+constexpr int kPlanGroups = 256;
+constexpr int kPlanFields = 14;
+struct AcaPlanLaunch {
+  const FarBlock *blk;
+  const int32_t *kind;     // per batch block: 1 = surrogate entries; NULL: all full MLP
+  const int32_t *act_in;   // previous active list; NULL: blocks 0..nb-1
+  const int32_t *stop_in;  // per previous slot; NULL: none stopped
+  const AcaCounts *cnt_in; // previous counts; NULL: first plan (nb blocks, zero totals)
+  int nb;
+  int chunk;
+  int32_t *act_out;
+  int64_t *ybase_c, *ybase_r;
+  int32_t *toff_c, *toff_r;
+  int32_t *cfirst_c, *cfirst_r;  // na + 1 entries
+  int32_t *alist, *first_i, *first_j;  // nl (+ 1) entries
+  AcaCounts *cnt_out;
+  int64_t *part;           // kPlanGroups x kPlanFields workspace
+};
+hipError_t launch_aca_plan(const AcaPlanLaunch &L, hipStream_t st);
+// Batch start: it = 0, mu2 = nu2 = 0, jpiv[b][0] = n_b / 2 for nb blocks.
+hipError_t launch_aca_init(const FarBlock *blk, int nb, int max_iter, int32_t *it, int32_t *jpiv,
+                           double *mu2, double *nu2, hipStream_t st);
+// acc[0..1] += cnt->entries[0..1] (entries per kind of a finished batch).
+hipError_t launch_aca_entries(const AcaCounts *cnt, int64_t *acc, hipStream_t st);
+// Final ranks of a batch into the per-far-block rank array.
+hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, hipStream_t st);
+hipError_t launch_aca_expand(const AcaExpandLaunch &L, hipStream_t st);
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);
 // which = 0: column residual, row pivot, column normalisation;
 // which = 1: row residual, column pivot, norms and stop test.

@@ -510,39 +510,27 @@ struct AcaMlpArgs {
   float *Y;               // (items, 9) block entries M (not symmetrised)
 };
 
-// Tile list of an ACA pass from the per-slot prefix counts: slot s owns
-// tiles first[s] .. first[s + 1] - 1, tile u of a slot starts at item 16u.
-// Column pass: pivot x[js + jpiv[k]], items x[is + i]; row pass: pivot
-// x[is + ipiv[k]], items x[js + j] (src/FarDot.cpp:346-458, 667-782).
-struct AcaExpandArgs {
-  const FarBlock *blk;
-  const int32_t *active;
-  const int32_t *it;
-  const int32_t *ipiv;
-  const int32_t *jpiv;
-  const int64_t *ybase;
-  const int32_t *first;
-  int nslots;
-  int max_iter;
-  int row_pass;
-  AcaPassTile *tiles;
-};
-
-__global__ void expand_tiles_kernel(AcaExpandArgs A) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.nslots) return;
-  const int b = A.active[s];
-  const FarBlock fb = A.blk[b];
-  const int k = A.it[b];
-  const int piv = A.row_pass ? fb.is + A.ipiv[(size_t)b * A.max_iter + k]
-                             : fb.js + A.jpiv[(size_t)b * (A.max_iter + 1) + k];
-  const int base = A.row_pass ? fb.js : fb.is;
-  const int lim = A.row_pass ? fb.n : fb.m;
-  const int64_t y0 = A.ybase[s];
-  const int t0 = A.first[s], t1 = A.first[s + 1];
-  for (int t = t0; t < t1; t++) {
-    const int off = 16 * (t - t0);
-    A.tiles[t] = AcaPassTile{piv, base + off, min(16, lim - off), 0, y0 + off};
+// Tile lists of an ACA pass (src/FarDot.cpp:346-458, 667-782): active slot s
+// writes its ceil(items / 16) tiles at toff[s] of the list of its kind; tile
+// u of a slot starts at item 16u.  Column pass: pivot x[js + jpiv[k]], items
+// x[is + i]; row pass: pivot x[is + ipiv[k]], items x[js + j].
+__global__ void expand_tiles_kernel(AcaExpandLaunch A) {
+  const int na = A.cnt->na;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < na; s += gridDim.x * blockDim.x) {
+    const int b = A.active[s];
+    const FarBlock fb = A.blk[b];
+    const int k = A.it[b];
+    const int piv = A.row_pass ? fb.is + A.ipiv[(size_t)b * A.max_iter + k]
+                               : fb.js + A.jpiv[(size_t)b * (A.max_iter + 1) + k];
+    const int base = A.row_pass ? fb.js : fb.is;
+    const int lim = A.row_pass ? fb.n : fb.m;
+    const int64_t y0 = A.ybase[s];
+    AcaPassTile *tl = (AcaPassTile *)((A.kind && A.kind[b]) ? A.tiles_sur : A.tiles_full);
+    const int t0 = A.toff[s], nt = (lim + 15) / 16;
+    for (int t = 0; t < nt; t++) {
+      const int off = 16 * t;
+      tl[t0 + t] = AcaPassTile{piv, base + off, min(16, lim - off), 0, y0 + off};
+    }
   }
 }
 
@@ -609,13 +597,18 @@ __device__ __forceinline__ void aca_mlp_body(const AcaMlpArgs &A, Core core) {
   }
 }
 
-__global__ __launch_bounds__(256, 2) void aca_mlp_fp32_kernel(AcaMlpArgs A, WF<float> W) {
+__global__ __launch_bounds__(256, 2) void aca_mlp_fp32_kernel(AcaMlpArgs A, const int32_t *dntiles,
+                                                             WF<float> W) {
+  A.ntiles = *dntiles;
   aca_mlp_body(A, [&](const float(&rx)[4], const float(&ry)[4], const float(&rz)[4],
                       f32x4(&y)[4]) { mlp_tiles_f<float, 4>(W, rx, ry, rz, y); });
 }
 
 template <class H>
-__global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, WH W) {
+__global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, const int32_t *dntiles,
+                                                                 WH W) {
+  A.ntiles = *dntiles;
+  if (A.ntiles <= 0) return;  // uniform: no fragments to stage
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
@@ -702,8 +695,10 @@ struct SurMfma {
 };
 
 __global__ __launch_bounds__(256) void aca_sur_kernel(const float4 *xp, const AcaPassTile *tiles,
-                                                      int64_t ntiles, int row_pass, float *Y,
+                                                      const int32_t *dntiles, int row_pass, float *Y,
                                                       const float *params, float inv_scale) {
+  const int64_t ntiles = *dntiles;
+  if (ntiles <= 0) return;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, p = lane & 15;
   SurMfma S;
@@ -906,6 +901,7 @@ struct AcaStepArgs {
   double *x_r;              // per chunk: max_iter x 9 row cross-term sums
   double *inv;              // per slot: inverse of the pivot block (9)
   int chunk;                // items per chunk, 1..kAcaChunk
+  const AcaCounts *cnt;     // na, ncc, ncr of the iteration
 };
 
 template <class S, class P>
@@ -1194,11 +1190,9 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
 // the row pivot candidates of one chunk (src/FarDot.cpp:474-580).  One wave
 // per chunk.
 template <class P, class SC, class SQ>
-__global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp, const SQ *Qp) {
-  __shared__ P piv[128][9];
-  __shared__ int32_t used[128];
-  __shared__ P sh[kCrossN * 65];
-  const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
+__device__ __forceinline__ void col_resid_chunk(const AcaStepArgs &A, SC *Cp, const SQ *Qp, int q,
+                                                int na, P (*piv)[9], int32_t *used, P *sh) {
+  const AcaTile ch = locate(A.cfirst_c, na, q, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -1213,7 +1207,7 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < iend;
   P c[kAcaIpt][9];
   residual_cross<P, SC, 0>(c, valid, Cp, fb.coff + ch.off + threadIdx.x, fb.m, k, piv, sh,
-                           A.x_c + (size_t)blockIdx.x * A.max_iter * 9);
+                           A.x_c + (size_t)q * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     if (!valid[s]) continue;
@@ -1248,16 +1242,28 @@ __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp
   }
   const ArgMax am = wave_argmax(best);
   if (threadIdx.x == 0) {
-    A.part_c[blockIdx.x].val = am.val;
-    A.part_c[blockIdx.x].idx = am.idx;
+    A.part_c[q].val = am.val;
+    A.part_c[q].idx = am.idx;
+  }
+}
+
+// One wave per chunk, grid-stride over the iteration's chunks.
+template <class P, class SC, class SQ>
+__global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp, const SQ *Qp) {
+  __shared__ P piv[128][9];
+  __shared__ int32_t used[128];
+  __shared__ P sh[kCrossN * 65];
+  const int na = A.cnt->na, nc = A.cnt->ncc;
+  for (int q = blockIdx.x; q < nc; q += gridDim.x) {
+    col_resid_chunk<P, SC, SQ>(A, Cp, Qp, q, na, piv, used, sh);
+    __syncthreads();  // piv / used of this chunk are read until here
   }
 }
 
 // Row pivot of each active block and the inverse of its pivot block
 // (src/FarDot.cpp:537-656).  One wave per block.
 template <class SC>
-__global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const SC *Cp) {
-  const int slot = blockIdx.x;
+__device__ __forceinline__ void col_pivot_slot(const AcaStepArgs &A, const SC *Cp, int slot) {
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -1301,11 +1307,17 @@ __global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const 
   }
 }
 
+template <class SC>
+__global__ __launch_bounds__(64) void aca_col_pivot_kernel(AcaStepArgs A, const SC *Cp) {
+  const int na = A.cnt->na;
+  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) col_pivot_slot<SC>(A, Cp, slot);
+}
+
 // Column normalisation C_k = R_k inv (src/FarDot.cpp:630-656) and the Gram
 // sums of C_k for one chunk.  One wave per chunk.
 template <class P, class SC>
-__global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp) {
-  const AcaTile ch = locate(A.cfirst_c, A.nactive, blockIdx.x, A.chunk);
+__device__ __forceinline__ void col_norm_chunk(const AcaStepArgs &A, SC *Cp, int q, int na) {
+  const AcaTile ch = locate(A.cfirst_c, na, q, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -1342,19 +1354,23 @@ __global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp)
 #pragma unroll
   for (int e = 0; e < 6; e++) {
     const double s = wave_sum(g[e]);
-    if (threadIdx.x == 0) A.part_c[blockIdx.x].g[e] = s;
+    if (threadIdx.x == 0) A.part_c[q].g[e] = s;
   }
+}
+
+template <class P, class SC>
+__global__ __launch_bounds__(64) void aca_col_norm_kernel(AcaStepArgs A, SC *Cp) {
+  const int na = A.cnt->na, nc = A.cnt->ncc;
+  for (int q = blockIdx.x; q < nc; q += gridDim.x) col_norm_chunk<P, SC>(A, Cp, q, na);
 }
 
 // Row pass residual Q_k, the row cross-term sums of iteration k-1, column
 // pivot candidates and the Gram sums of Q_k for one chunk
 // (src/FarDot.cpp:797-1078).  One wave per chunk.
 template <class P, class SC, class SQ>
-__global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const SC *Cp, SQ *Qp) {
-  __shared__ P piv[128][9];
-  __shared__ int32_t used[129];
-  __shared__ P sh[kCrossN * 65];
-  const AcaTile ch = locate(A.cfirst_r, A.nactive, blockIdx.x, A.chunk);
+__device__ __forceinline__ void row_resid_chunk(const AcaStepArgs &A, const SC *Cp, SQ *Qp, int q,
+                                                int na, P (*piv)[9], int32_t *used, P *sh) {
+  const AcaTile ch = locate(A.cfirst_r, na, q, A.chunk);
   const int b = A.active[ch.slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -1367,16 +1383,16 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
   bool valid[kAcaIpt];
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) valid[s] = ch.off + (int)threadIdx.x + 64 * s < jend;
-  P q[kAcaIpt][9];
-  residual_cross<P, SQ, 1>(q, valid, Qp, fb.qoff + ch.off + threadIdx.x, fb.n, k, piv, sh,
-                           A.x_r + (size_t)blockIdx.x * A.max_iter * 9);
+  P qv[kAcaIpt][9];
+  residual_cross<P, SQ, 1>(qv, valid, Qp, fb.qoff + ch.off + threadIdx.x, fb.n, k, piv, sh,
+                           A.x_r + (size_t)q * A.max_iter * 9);
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
     if (!valid[s]) continue;
     P raw[9];
     sym_load<P>(A.Y + 9 * (A.ybase_r[ch.slot] + ch.off + threadIdx.x + 64 * s), raw);
 #pragma unroll
-    for (int e = 0; e < 9; e++) q[s][e] = raw[e] - q[s][e];
+    for (int e = 0; e < 9; e++) qv[s][e] = raw[e] - qv[s][e];
   }
   const int64_t qk0 = fb.qoff + (int64_t)k * fb.n;  // first item of Q_k
   ArgMax best{kNegHuge, -1};
@@ -1389,24 +1405,36 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
     SQ st[9];  // pivot and norms see the stored factor
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      st[e] = (SQ)q[s][e];
-      q[s][e] = (P)st[e];
+      st[e] = (SQ)qv[s][e];
+      qv[s][e] = (P)st[e];
       a[e] = (double)st[e];
     }
     PoolAt<SQ>::store(Qp, qk0 + j, st);
     bool u = false;
     for (int l = 0; l <= k; l++) u |= (used[l] == j);
-    const double dt = (double)det3(q[s]);
+    const double dt = (double)det3(qv[s]);
     if (!u && dt > best.val) best = ArgMax{dt, j};
     gram_row(a, g);
   }
   const ArgMax am = wave_argmax(best);
-  AcaPart &pt = A.part_r[blockIdx.x];
+  AcaPart &pt = A.part_r[q];
   if (threadIdx.x == 0) pt.val = am.val, pt.idx = am.idx;
 #pragma unroll
   for (int e = 0; e < 6; e++) {
     const double s = wave_sum(g[e]);
     if (threadIdx.x == 0) pt.g[e] = s;
+  }
+}
+
+template <class P, class SC, class SQ>
+__global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const SC *Cp, SQ *Qp) {
+  __shared__ P piv[128][9];
+  __shared__ int32_t used[129];
+  __shared__ P sh[kCrossN * 65];
+  const int na = A.cnt->na, nc = A.cnt->ncr;
+  for (int q = blockIdx.x; q < nc; q += gridDim.x) {
+    row_resid_chunk<P, SC, SQ>(A, Cp, Qp, q, na, piv, used, sh);
+    __syncthreads();  // piv / used of this chunk are read until here
   }
 }
 
@@ -1416,10 +1444,8 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 // the hard stop tests of iteration k (src/FarDot.cpp:1002-1108).  mu2[b]
 // holds mu2 of iteration k-1 without its cross terms, nu2[b] its nu2; it[b]
 // ends as the block's rank when it stops.
-__global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
-  __shared__ double cs[126 * 9], qs[126 * 9];
-  __shared__ ArgMax sh_am[4];
-  const int slot = blockIdx.x;
+__device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, double *cs, double *qs,
+                                              ArgMax *sh_am) {
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
   const int k = A.it[b];
@@ -1440,29 +1466,39 @@ __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
   __syncthreads();
   if (threadIdx.x == 0) {
     double mu2 = A.mu2[b];
+    bool soft = false;
     if (k >= 1) {
       for (int t = 0; t < ncross; t++) mu2 += 2.0 * fabs(cs[t]) * fabs(qs[t]);
-      if (A.nu2[b] < mu2 * A.eps2) {  // iteration k-1 met the tolerance: rank k
-        A.mu2[b] = mu2;
-        A.stop[slot] = 1;
-        return;
-      }
+      soft = A.nu2[b] < mu2 * A.eps2;
     }
-    double GC[6] = {0, 0, 0, 0, 0, 0}, GQ[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = c0; c < c1; c++)
-      for (int e = 0; e < 6; e++) GC[e] += A.part_c[c].g[e];
-    for (int c = r0; c < r1; c++)
-      for (int e = 0; e < 6; e++) GQ[e] += A.part_r[c].g[e];
-    const double nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
-                       2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
-    A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
-    A.mu2[b] = mu2 + nu2;
-    A.nu2[b] = nu2;
-    const int itn = k + 1;
-    A.it[b] = itn;
-    const int mn = fb.m < fb.n ? fb.m : fb.n;
-    A.stop[slot] = (itn >= A.max_iter || itn >= mn) ? 1 : 0;
+    if (soft) {  // iteration k-1 met the tolerance: rank k
+      A.mu2[b] = mu2;
+      A.stop[slot] = 1;
+    } else {
+      double GC[6] = {0, 0, 0, 0, 0, 0}, GQ[6] = {0, 0, 0, 0, 0, 0};
+      for (int c = c0; c < c1; c++)
+        for (int e = 0; e < 6; e++) GC[e] += A.part_c[c].g[e];
+      for (int c = r0; c < r1; c++)
+        for (int e = 0; e < 6; e++) GQ[e] += A.part_r[c].g[e];
+      const double nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
+                         2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
+      A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
+      A.mu2[b] = mu2 + nu2;
+      A.nu2[b] = nu2;
+      const int itn = k + 1;
+      A.it[b] = itn;
+      const int mn = fb.m < fb.n ? fb.m : fb.n;
+      A.stop[slot] = (itn >= A.max_iter || itn >= mn) ? 1 : 0;
+    }
   }
+  __syncthreads();  // cs / qs / sh_am are reused by the next slot
+}
+
+__global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
+  __shared__ double cs[126 * 9], qs[126 * 9];
+  __shared__ ArgMax sh_am[4];
+  const int na = A.cnt->na;
+  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, cs, qs, sh_am);
 }
 
 // ---- chunked apply of stopped blocks (src/FarDot.cpp:1200-1388) -----------
@@ -1484,16 +1520,15 @@ struct AcaApplyArgs {
   double *w_i;               // per I chunk: max_iter x 3 (side 1 partials)
   int max_iter;
   int chunk;                 // items per chunk, 1..kAcaChunk
+  const AcaCounts *cnt;      // nl, nci, ncj
 };
 
 template <class SC, class SQ, int SIDE>
-__global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const SC *Cp,
-                                                              const SQ *Qp) {
+__device__ __forceinline__ void apply_reduce_chunk(const AcaApplyArgs &A, const SC *Cp, const SQ *Qp,
+                                                   int q, int nl, double *sh) {
   // SIDE 0 reduces over the Q pool, SIDE 1 over the C pool
   typedef typename std::conditional<SIDE == 0, SQ, SC>::type S;
-  __shared__ double sh[32 * 65];
-  const AcaTile ch = SIDE == 0 ? locate(A.first_j, A.nlist, blockIdx.x, A.chunk)
-                                : locate(A.first_i, A.nlist, blockIdx.x, A.chunk);
+  const AcaTile ch = SIDE == 0 ? locate(A.first_j, nl, q, A.chunk) : locate(A.first_i, nl, q, A.chunk);
   const int b = A.list[ch.slot];
   const FarBlock fb = A.blk[b];
   const int K = min(A.it[b], 128);
@@ -1506,7 +1541,7 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
   else
     Src = Cp, item0 = fb.coff;
   const int send = min(ch.off + A.chunk, ns);
-  double *W = (SIDE == 0 ? A.w_j : A.w_i) + (size_t)blockIdx.x * A.max_iter * 3;
+  double *W = (SIDE == 0 ? A.w_j : A.w_i) + (size_t)q * A.max_iter * 3;
   float4 f[kAcaIpt];
 #pragma unroll
   for (int s = 0; s < kAcaIpt; s++) {
@@ -1550,13 +1585,19 @@ __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, co
 }
 
 template <class SC, class SQ, int SIDE>
-__global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, const SC *Cp,
-                                                               const SQ *Qp) {
+__global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const SC *Cp,
+                                                              const SQ *Qp) {
+  __shared__ double sh[32 * 65];
+  const int nl = A.cnt->nl, nc = SIDE == 0 ? A.cnt->ncj : A.cnt->nci;
+  for (int q = blockIdx.x; q < nc; q += gridDim.x) apply_reduce_chunk<SC, SQ, SIDE>(A, Cp, Qp, q, nl, sh);
+}
+
+template <class SC, class SQ, int SIDE>
+__device__ __forceinline__ void apply_update_chunk(const AcaApplyArgs &A, const SC *Cp, const SQ *Qp,
+                                                   int q, int nl, double *wsh) {
   // SIDE 0 updates from the C pool, SIDE 1 from the Q pool
   typedef typename std::conditional<SIDE == 0, SC, SQ>::type S;
-  __shared__ double wsh[128 * 3];
-  const AcaTile ch = SIDE == 0 ? locate(A.first_i, A.nlist, blockIdx.x, A.chunk)
-                                : locate(A.first_j, A.nlist, blockIdx.x, A.chunk);
+  const AcaTile ch = SIDE == 0 ? locate(A.first_i, nl, q, A.chunk) : locate(A.first_j, nl, q, A.chunk);
   const int b = A.list[ch.slot];
   const FarBlock fb = A.blk[b];
   const int K = min(A.it[b], 128);
@@ -1599,6 +1640,211 @@ __global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, co
     unsafeAtomicAdd(u + 1, o1);
     unsafeAtomicAdd(u + 2, o2);
   }
+}
+
+template <class SC, class SQ, int SIDE>
+__global__ __launch_bounds__(64) void aca_apply_update_kernel(AcaApplyArgs A, const SC *Cp,
+                                                               const SQ *Qp) {
+  __shared__ double wsh[128 * 3];
+  const int nl = A.cnt->nl, nc = SIDE == 0 ? A.cnt->nci : A.cnt->ncj;
+  for (int q = blockIdx.x; q < nc; q += gridDim.x) {
+    apply_update_chunk<SC, SQ, SIDE>(A, Cp, Qp, q, nl, wsh);
+    __syncthreads();  // wsh of this chunk is read until here
+  }
+}
+
+// ---- device plan of a batched ACA iteration --------------------------------
+// Per input element (a block of the previous active list): the values whose
+// exclusive prefix sums place it in the new active list (kept) or in the
+// apply list (stopped), and two running totals (entries per kind).
+__device__ __forceinline__ void plan_values(const AcaPlanLaunch &L, int e, int64_t (&v)[kPlanFields],
+                                            int &b, bool &keep, bool &full) {
+  b = L.act_in ? L.act_in[e] : e;
+  keep = !(L.stop_in && L.stop_in[e]);
+  full = !(L.kind && L.kind[b]);
+  const FarBlock fb = L.blk[b];
+  const int64_t m = fb.m, n = fb.n;
+  const int64_t tm = (m + 15) / 16, tn = (n + 15) / 16;
+  const int64_t cm = (m + L.chunk - 1) / L.chunk, cn = (n + L.chunk - 1) / L.chunk;
+  const int64_t k = keep ? 1 : 0, d = keep ? 0 : 1, f = full ? 1 : 0;
+  v[0] = k;
+  v[1] = k * m;
+  v[2] = k * n;
+  v[3] = k * f * tm;
+  v[4] = k * (1 - f) * tm;
+  v[5] = k * f * tn;
+  v[6] = k * (1 - f) * tn;
+  v[7] = k * cm;
+  v[8] = k * cn;
+  v[9] = d;
+  v[10] = d * cm;
+  v[11] = d * cn;
+  v[12] = k * f * (m + n);
+  v[13] = k * (1 - f) * (m + n);
+}
+
+__device__ __forceinline__ void plan_range(const AcaPlanLaunch &L, int &e0, int &e1) {
+  const int nin = L.cnt_in ? L.cnt_in->na : L.nb;
+  const int per = (nin + kPlanGroups - 1) / kPlanGroups;
+  e0 = min(nin, (int)blockIdx.x * per);
+  e1 = min(nin, e0 + per);
+}
+
+__global__ __launch_bounds__(256) void aca_plan_reduce_kernel(AcaPlanLaunch L) {
+  __shared__ int64_t sh[4][kPlanFields];
+  int e0, e1;
+  plan_range(L, e0, e1);
+  int64_t acc[kPlanFields];
+#pragma unroll
+  for (int f = 0; f < kPlanFields; f++) acc[f] = 0;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    int64_t v[kPlanFields];
+    int b;
+    bool keep, full;
+    plan_values(L, e, v, b, keep, full);
+#pragma unroll
+    for (int f = 0; f < kPlanFields; f++) acc[f] += v[f];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int f = 0; f < kPlanFields; f++) {
+    int64_t t = acc[f];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) sh[wv][f] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kPlanFields) {
+    const int f = threadIdx.x;
+    L.part[(size_t)blockIdx.x * kPlanFields + f] = sh[0][f] + sh[1][f] + sh[2][f] + sh[3][f];
+  }
+}
+
+// One workgroup of kPlanGroups threads: exclusive scan of the group sums,
+// totals into cnt_out, sentinels of the prefix arrays.
+__global__ __launch_bounds__(kPlanGroups) void aca_plan_scan_kernel(AcaPlanLaunch L) {
+  __shared__ int64_t sh[kPlanGroups];
+  __shared__ int64_t tot[kPlanFields];
+  const int g = threadIdx.x;
+  for (int f = 0; f < kPlanFields; f++) {
+    const int64_t x = L.part[(size_t)g * kPlanFields + f];
+    sh[g] = x;
+    __syncthreads();
+    for (int o = 1; o < kPlanGroups; o <<= 1) {  // inclusive Hillis-Steele scan
+      const int64_t y = g >= o ? sh[g - o] : 0;
+      __syncthreads();
+      sh[g] += y;
+      __syncthreads();
+    }
+    L.part[(size_t)g * kPlanFields + f] = sh[g] - x;  // exclusive
+    if (g == kPlanGroups - 1) tot[f] = sh[g];
+    __syncthreads();
+  }
+  if (g == 0) {
+    AcaCounts c;
+    c.na = (int32_t)tot[0];
+    c.ysum_c = tot[1];
+    c.ysum_r = tot[2];
+    c.ntc[0] = (int32_t)tot[3];
+    c.ntc[1] = (int32_t)tot[4];
+    c.ntr[0] = (int32_t)tot[5];
+    c.ntr[1] = (int32_t)tot[6];
+    c.ncc = (int32_t)tot[7];
+    c.ncr = (int32_t)tot[8];
+    c.nl = (int32_t)tot[9];
+    c.nci = (int32_t)tot[10];
+    c.ncj = (int32_t)tot[11];
+    c.entries[0] = (L.cnt_in ? L.cnt_in->entries[0] : 0) + tot[12];
+    c.entries[1] = (L.cnt_in ? L.cnt_in->entries[1] : 0) + tot[13];
+    *L.cnt_out = c;
+    L.cfirst_c[c.na] = c.ncc;
+    L.cfirst_r[c.na] = c.ncr;
+    L.first_i[c.nl] = c.nci;
+    L.first_j[c.nl] = c.ncj;
+  }
+}
+
+// The group's range in order, 256 elements per round: block-wide exclusive
+// scan of the placing values, plus the group's offset and the rounds before.
+__global__ __launch_bounds__(256) void aca_plan_write_kernel(AcaPlanLaunch L) {
+  constexpr int F = 12;  // fields that place an element
+  __shared__ int64_t wsum[4][F];
+  int e0, e1;
+  plan_range(L, e0, e1);
+  int64_t carry[F];
+#pragma unroll
+  for (int f = 0; f < F; f++) carry[f] = L.part[(size_t)blockIdx.x * kPlanFields + f];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int r0 = e0; r0 < e1; r0 += blockDim.x) {  // uniform
+    const int e = r0 + (int)threadIdx.x;
+    int64_t v[kPlanFields];
+    int b = 0;
+    bool keep = false, full = true;
+    if (e < e1) {
+      plan_values(L, e, v, b, keep, full);
+    } else {
+#pragma unroll
+      for (int f = 0; f < kPlanFields; f++) v[f] = 0;
+    }
+    int64_t incl[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      int64_t t = v[f];
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(t, o, 64);
+        if (lane >= o) t += y;
+      }
+      incl[f] = t;
+      if (lane == 63) wsum[wv][f] = t;
+    }
+    __syncthreads();
+    int64_t pre[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      int64_t w = 0;
+      for (int q = 0; q < wv; q++) w += wsum[q][f];
+      pre[f] = carry[f] + w + incl[f] - v[f];
+      carry[f] += wsum[0][f] + wsum[1][f] + wsum[2][f] + wsum[3][f];
+    }
+    __syncthreads();  // wsum is rewritten by the next round
+    if (e < e1) {
+      if (keep) {
+        const int s = (int)pre[0];
+        L.act_out[s] = b;
+        L.ybase_c[s] = pre[1];
+        L.ybase_r[s] = pre[2];
+        L.toff_c[s] = (int32_t)(full ? pre[3] : pre[4]);
+        L.toff_r[s] = (int32_t)(full ? pre[5] : pre[6]);
+        L.cfirst_c[s] = (int32_t)pre[7];
+        L.cfirst_r[s] = (int32_t)pre[8];
+      } else {
+        const int a = (int)pre[9];
+        L.alist[a] = b;
+        L.first_i[a] = (int32_t)pre[10];
+        L.first_j[a] = (int32_t)pre[11];
+      }
+    }
+  }
+}
+
+__global__ void aca_init_kernel(const FarBlock *blk, int nb, int max_iter, int32_t *it, int32_t *jpiv,
+                                double *mu2, double *nu2) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  it[b] = 0;
+  jpiv[(size_t)b * (max_iter + 1)] = blk[b].n / 2;
+  mu2[b] = 0.0;
+  nu2[b] = 0.0;
+}
+
+__global__ void aca_entries_kernel(const AcaCounts *cnt, int64_t *acc) {
+  if (threadIdx.x < 2) acc[threadIdx.x] += cnt->entries[threadIdx.x];
+}
+
+__global__ void copy_ranks_kernel(const int32_t *it, int nb, int32_t *rank_out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) rank_out[b] = it[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -2119,29 +2365,60 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_aca_plan(const AcaPlanLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(aca_plan_reduce_kernel, dim3(kPlanGroups), dim3(256), 0, st, L);
+  hipLaunchKernelGGL(aca_plan_scan_kernel, dim3(1), dim3(kPlanGroups), 0, st, L);
+  hipLaunchKernelGGL(aca_plan_write_kernel, dim3(kPlanGroups), dim3(256), 0, st, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_init(const FarBlock *blk, int nb, int max_iter, int32_t *it, int32_t *jpiv,
+                           double *mu2, double *nu2, hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(aca_init_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, blk, nb,
+                     max_iter, it, jpiv, mu2, nu2);
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_entries(const AcaCounts *cnt, int64_t *acc, hipStream_t st) {
+  hipLaunchKernelGGL(aca_entries_kernel, dim3(1), dim3(64), 0, st, cnt, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_ranks_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, it, nb,
+                     rank_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_aca_expand(const AcaExpandLaunch &L, hipStream_t st) {
+  if (L.grid <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((L.grid + 255) / 256, 4096);
+  hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)blocks), dim3(256), 0, st, L);
+  return hipGetLastError();
+}
+
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st) {
-  if (L.nactive <= 0) return hipSuccess;
-  AcaExpandArgs E{L.blk, L.active, L.it, L.ipiv, L.jpiv, L.ybase, L.tfirst, L.nactive,
-                  L.max_iter, L.row_pass, (AcaPassTile *)L.tiles};
-  hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)((L.nactive + 255) / 256)), dim3(256), 0,
-                     st, E);
-  if (L.ntiles <= 0) return hipGetLastError();
+  if (L.ntiles <= 0) return hipSuccess;
   AcaMlpArgs A{L.xp, (const AcaPassTile *)L.tiles, L.ntiles, L.row_pass, L.Y};
   const int64_t waves = (L.ntiles + 3) / 4;
   if (L.prec == PREC_FP32) {
     WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,
                 (const float *)L.wf.w3f, (const float *)L.wf.b3};
     hipLaunchKernelGGL(aca_mlp_fp32_kernel, dim3(grid_for(waves, 4, L.max_blocks)), dim3(256), 0,
-                       st, A, W);
+                       st, A, L.dntiles, W);
   } else {
     allow_big_lds();
     WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
     const size_t lds = kW2FragBytes + kW3FragBytes;
     const int grid = grid_for(waves, kNearHThreads / 64, L.cus);
     if (L.prec == PREC_BF16)
-      hipLaunchKernelGGL(aca_mlp_h_kernel<bf16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A, W);
+      hipLaunchKernelGGL(aca_mlp_h_kernel<bf16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A,
+                         L.dntiles, W);
     else
-      hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A, W);
+      hipLaunchKernelGGL(aca_mlp_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, A,
+                         L.dntiles, W);
   }
   return hipGetLastError();
 }
@@ -2150,7 +2427,7 @@ hipError_t launch_aca_sur(const AcaSurLaunch &L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
   const int64_t blocks = std::min<int64_t>((L.ntiles + 3) / 4, 16 * (int64_t)L.cus);
   hipLaunchKernelGGL(aca_sur_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     L.xp, (const AcaPassTile *)L.tiles + L.tile0, L.ntiles, L.row_pass, L.Y,
+                     L.xp, (const AcaPassTile *)L.tiles, L.dntiles, L.row_pass, L.Y,
                      L.params, L.inv_scale);
   return hipGetLastError();
 }
@@ -2203,12 +2480,15 @@ static void step_launch(const AcaStepArgs &A, const AcaStepLaunch &L, int which,
   }
 }
 
+static unsigned capped(int64_t n, int64_t cap) { return (unsigned)std::max<int64_t>(1, std::min(n, cap)); }
+
 hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
   if (L.nactive <= 0) return hipSuccess;
   AcaStepArgs A{L.blk, L.active, L.ybase_c, L.ybase_r, L.Y, L.it, L.ipiv, L.jpiv, L.mu2, L.nu2,
-                L.stop, L.max_iter, L.eps2, L.nactive, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
-                (AcaPart *)L.part_r, L.x_c, L.x_r, L.inv, L.chunk};
-  const dim3 gc((unsigned)L.nchunks_c), gr((unsigned)L.nchunks_r), ga((unsigned)L.nactive);
+                L.stop, L.max_iter, L.eps2, (int)L.nactive, L.cfirst_c, L.cfirst_r, (AcaPart *)L.part_c,
+                (AcaPart *)L.part_r, L.x_c, L.x_r, L.inv, L.chunk, L.cnt};
+  const dim3 gc(capped(L.nchunks_c, L.grid_cap)), gr(capped(L.nchunks_r, L.grid_cap)),
+      ga(capped(L.nactive, L.grid_cap));
   typedef __bf16 B;
   if (L.pool_f64)
     step_launch<double, double, double>(A, L, which, gc, gr, ga, st);
@@ -2230,7 +2510,7 @@ template <class SC, class SQ>
 static void apply_launch(const AcaApplyArgs &A, const AcaApplyLaunch &L, hipStream_t st) {
   const SC *C = (const SC *)L.C;
   const SQ *Q = (const SQ *)L.Q;
-  const dim3 gi((unsigned)L.nchunks_i), gj((unsigned)L.nchunks_j);
+  const dim3 gi(capped(L.nchunks_i, L.grid_cap)), gj(capped(L.nchunks_j, L.grid_cap));
   hipLaunchKernelGGL((aca_apply_reduce_kernel<SC, SQ, 0>), gj, dim3(64), 0, st, A, C, Q);
   if (L.use_symmetry)
     hipLaunchKernelGGL((aca_apply_reduce_kernel<SC, SQ, 1>), gi, dim3(64), 0, st, A, C, Q);
@@ -2241,8 +2521,8 @@ static void apply_launch(const AcaApplyArgs &A, const AcaApplyLaunch &L, hipStre
 
 hipError_t launch_aca_apply(const AcaApplyLaunch &L, hipStream_t st) {
   if (L.nlist <= 0) return hipSuccess;
-  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, L.nlist, L.first_i, L.first_j, L.w_j, L.w_i,
-                 L.max_iter, L.chunk};
+  AcaApplyArgs A{L.blk, L.list, L.it, L.fp, L.u, (int)L.nlist, L.first_i, L.first_j, L.w_j, L.w_i,
+                 L.max_iter, L.chunk, L.cnt};
   typedef __bf16 B;
   if (L.pool_f64)
     apply_launch<double, double>(A, L, st);

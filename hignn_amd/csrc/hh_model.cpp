@@ -85,9 +85,9 @@ struct hh_model {
   int64_t max_rel_coord = 500000;
   bool post_check = false, use_symmetry = true;
   bool tree_reuse = false;  // update_coord keeps the last tree and lists (opt-in)
-  // tree and block lists built on the device (hh_tree_gpu.hip) or on host
-  // threads (hh_tree.cpp); both give the same result bit for bit
-  bool tree_device = false;
+  // tree and block lists built on the device (hh_tree_gpu.hip, the default)
+  // or on host threads (hh_tree.cpp); both give the same result bit for bit
+  bool tree_device = true;
   DeviceTreeBuilder *tbuild = nullptr;
   DBuf xdev;  // coordinates (n,3) uploaded for a device build from host data
   uint64_t builds = 0, loads = 0;  // tree builds and weight loads (graph signatures)
@@ -114,14 +114,30 @@ struct hh_model {
   int64_t n_dense_tasks = 0;
   std::vector<FarBlock> far;  // is, m, js, n (offsets set per batch)
   std::vector<int32_t> far_rank;
-  // far workspace
-  DBuf blk, it, ipiv, jpiv, mu2, nu2, stop, active_c, active_r, ybase_c, ybase_r,
-      applylist, Y, Cpool, Qpool;
-  PBuf h_active, h_ybase_c, h_ybase_r, h_stop, h_list;
-  // chunked step / apply lists and per-chunk partials
-  DBuf tfirst_c, tfirst_r, tiles_c, tiles_r, cfirst_c, cfirst_r, part_c, part_r, x_c, x_r, invb;
-  DBuf a_first_i, a_first_j, w_i, w_j;
-  PBuf h_tfirst_c, h_tfirst_r, h_cfirst_c, h_cfirst_r, h_afirst_i, h_afirst_j;
+  // far workspace of the batched iterations: blocks with pool offsets (all
+  // batches, far-list order), per-block ACA state, per-slot lists of the
+  // current iteration (active list double-buffered for the device plan),
+  // pass tile lists per kind, chunk partials, apply lists, counts
+  DBuf fb_all, it, ipiv, jpiv, mu2, nu2, stop, act2[2], ybase_c, ybase_r, applylist, Y, Cpool, Qpool;
+  DBuf toff_c, toff_r, tiles_fc, tiles_sc, tiles_fr, tiles_sr, cfirst_c, cfirst_r, part_c, part_r,
+      x_c, x_r, invb;
+  DBuf a_first_i, a_first_j, w_i, w_j, cnt2[2], plan_part, rank_dev, ent_acc;
+  PBuf h_stop, h_cnt, h_cnt_apply, h_act, h_ybase_c, h_ybase_r, h_toff_c, h_toff_r, h_cfirst_c,
+      h_cfirst_r, h_list, h_afirst_i, h_afirst_j;
+  // batches of the batched far field (blocks whose factor pools fit the byte
+  // budget) and the bounds of their per-iteration counts (every block active)
+  struct FarBatch {
+    int64_t b0, b1;
+    int64_t ntc, ntr, ncc, ncr, ysum_c, ysum_r;
+  };
+  std::vector<FarBatch> batches;
+  uint64_t batch_key = 0;  // configuration the batches were laid out for
+  // 1 (default): each iteration's active list, prefix counts and stop flags
+  // stay on the device (aca_plan kernels), no host round trip per iteration
+  // (HH_ACA_CONTROL=host: the host compacts after reading the stop flags)
+  int aca_control = 1;
+  bool judge_pending = false;  // judge flags / errors / entry counts not yet read back
+  std::vector<hipEvent_t> evpool;  // events around the full-MLP ACA passes
   // persistent far-field workspace
   // far field: 1 batched iterations, 0 persistent kernel, -1 (default) the
   // persistent kernel when there are no more far blocks than compute units
@@ -179,19 +195,21 @@ struct hh_model {
                     &jd_s6, &jd_flag, &jd_err})
       b->release();
     for (auto *b : {&xp, &perm, &fp, &u, &fin, &uout, &near_blocks, &near_tasks, &dense_blocks,
-                    &dense_tasks, &blk, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &active_c,
-                    &active_r, &ybase_c, &ybase_r, &applylist, &Y, &Cpool,
-                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &tfirst_c, &tfirst_r, &tiles_c, &tiles_r,
-                    &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r, &invb, &a_first_i,
-                    &a_first_j, &w_i, &w_j})
+                    &dense_tasks, &fb_all, &it, &ipiv, &jpiv, &mu2, &nu2, &stop, &act2[0],
+                    &act2[1], &ybase_c, &ybase_r, &applylist, &Y, &Cpool,
+                    &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &toff_c, &toff_r, &tiles_fc,
+                    &tiles_sc, &tiles_fr, &tiles_sr, &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r,
+                    &invb, &a_first_i, &a_first_j, &w_i, &w_j, &cnt2[0], &cnt2[1], &plan_part,
+                    &rank_dev, &ent_acc})
       b->release();
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (auto &b : wf32) b.release();
     for (auto &b : wf64) b.release();
     for (auto &b : wh_f) b.release();
     for (auto &b : wh_bf16) b.release();
     for (auto &b : wh_f16) b.release();
-    for (auto *b : {&h_active, &h_ybase_c, &h_ybase_r, &h_stop, &h_list, &h_tfirst_c,
-                    &h_tfirst_r, &h_cfirst_c, &h_cfirst_r, &h_afirst_i, &h_afirst_j})
+    for (auto *b : {&h_stop, &h_cnt, &h_cnt_apply, &h_act, &h_ybase_c, &h_ybase_r, &h_toff_c,
+                    &h_toff_r, &h_cfirst_c, &h_cfirst_r, &h_list, &h_afirst_i, &h_afirst_j})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -422,6 +440,8 @@ struct hh_model {
                              (int32_t)tl.size(J), 0, 0});
     }
     far_rank.assign(far.size(), 0);
+    rank_pending = judge_pending = false;
+    batches.clear();
     p_ready = false;
     built = true;
   }
@@ -550,68 +570,21 @@ struct hh_model {
     L.wf = devw_f(false);
     L.wh = devw_h(prec);
     HIPCHK(launch_persistent_aca(L, st));
+    rank_src = p_rank.as<int32_t>();
     rank_pending = true;
   }
 
   bool rank_pending = false;
+  const int32_t *rank_src = nullptr;  // device ranks of the last far field (far-list order)
   void fetch_ranks() {
     if (!rank_pending) return;
-    HIPCHK(hipMemcpy(far_rank.data(), p_rank.p, far_rank.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(far_rank.data(), rank_src, far_rank.size() * 4, hipMemcpyDeviceToHost));
     rank_pending = false;
   }
 
   // ---- far field: batched ACA -------------------------------------------
   // Low-rank apply of the stopped blocks `done` (indices into hb), chunked
   // over I and J like the steps.
-  void apply_blocks(const std::vector<FarBlock> &hb, const std::vector<int32_t> &done, int MI,
-                    bool f64, hipStream_t st) {
-    const int nl = (int)done.size();
-    h_list.reserve((size_t)nl * 4);
-    applylist.reserve((size_t)nl * 4);
-    for (auto *b : {&h_afirst_i, &h_afirst_j}) b->reserve((size_t)(nl + 1) * 4);
-    for (auto *b : {&a_first_i, &a_first_j}) b->reserve((size_t)(nl + 1) * 4);
-    int32_t *hl = h_list.as<int32_t>(), *fi = h_afirst_i.as<int32_t>(),
-            *fj = h_afirst_j.as<int32_t>();
-    int64_t nci = 0, ncj = 0;
-    for (int a = 0; a < nl; a++) {
-      const FarBlock &f = hb[(size_t)done[(size_t)a]];
-      hl[a] = done[(size_t)a];
-      fi[a] = (int32_t)nci;
-      fj[a] = (int32_t)ncj;
-      nci += (f.m + aca_chunk - 1) / aca_chunk;
-      ncj += (f.n + aca_chunk - 1) / aca_chunk;
-    }
-    fi[nl] = (int32_t)nci;
-    fj[nl] = (int32_t)ncj;
-    w_i.reserve((size_t)std::max<int64_t>(nci, 1) * MI * 3 * 8);
-    w_j.reserve((size_t)std::max<int64_t>(ncj, 1) * MI * 3 * 8);
-    HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(a_first_i.p, fi, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(a_first_j.p, fj, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
-    AcaApplyLaunch al{};
-    al.blk = blk.as<FarBlock>();
-    al.list = applylist.as<int32_t>();
-    al.it = it.as<int32_t>();
-    al.fp = fp.as<float4>();
-    al.u = u.as<double>();
-    al.use_symmetry = use_symmetry;
-    al.nlist = nl;
-    al.pool_f64 = f64;
-    al.pool_c16 = pool16(true);
-    al.pool_q16 = pool16(false);
-    al.C = Cpool.p;
-    al.Q = Qpool.p;
-    al.first_i = a_first_i.as<int32_t>();
-    al.first_j = a_first_j.as<int32_t>();
-    al.nchunks_i = nci;
-    al.nchunks_j = ncj;
-    al.w_i = w_i.as<double>();
-    al.w_j = w_j.as<double>();
-    al.max_iter = MI;
-    al.chunk = aca_chunk;
-    HIPCHK(launch_aca_apply(al, st));
-  }
-
   size_t pool_budget() {
     if (far_pool_bytes == 0) {
       size_t fr = 0, tot = 0;
@@ -623,23 +596,19 @@ struct hh_model {
   }
 
   // The surrogate judge (hh_model_set_surrogate): per far block, the full
-  // FP32 MLP against the surrogate on sampled entries; sur_flag[b] = 1 for
-  // the blocks whose ACA takes surrogate entries.
+  // FP32 MLP against the surrogate on sampled entries; jd_flag[b] = 1 for the
+  // blocks whose ACA takes surrogate entries.  The flags stay on the device;
+  // fetch_judge() reads them (and the entry counts) when asked for.
   void judge_surrogate(hipStream_t st) {
     const int64_t nfar = (int64_t)far.size();
+    judge_pending = false;
     sur_flag.assign((size_t)nfar, 0);
     sur_err.assign((size_t)nfar, 0.0f);
     for (auto &v : sur_stats) v = 0;
     if (!sur || nfar == 0) return;
     const int S = sur_samples;
-    upload(jd_blk, far.data(), far.size() * sizeof(FarBlock));
-    jd_r.reserve((size_t)nfar * S * 12);
-    jd_m9.reserve((size_t)nfar * S * 36);
-    jd_s6.reserve((size_t)nfar * S * 24);
-    jd_flag.reserve((size_t)nfar * 4);
-    jd_err.reserve((size_t)nfar * 4);
     JudgeLaunch L{};
-    L.blk = jd_blk.as<FarBlock>();
+    L.blk = fb_all.as<FarBlock>();  // only is, m, js, n are read
     L.nblk = nfar;
     L.xp = xp.as<float4>();
     L.samples = S;
@@ -656,255 +625,474 @@ struct hh_model {
     L.wh = devw_h(prec);
     L.cus = g_cus;
     HIPCHK(launch_judge(L, st));
-    HIPCHK(hipMemcpyAsync(sur_flag.data(), jd_flag.p, (size_t)nfar * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(sur_err.data(), jd_err.p, (size_t)nfar * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    sur_stats[0] = nfar;
-    for (int32_t f : sur_flag) sur_stats[f ? 1 : 2]++;
+    judge_pending = true;
   }
 
-  void far_field(hipStream_t st) {
+  // Host copies of the last judged far field: flags, errors, entry counts.
+  void fetch_judge() {
+    if (!judge_pending) return;
+    const int64_t nfar = (int64_t)far.size();
+    HIPCHK(hipMemcpy(sur_flag.data(), jd_flag.p, (size_t)nfar * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sur_err.data(), jd_err.p, (size_t)nfar * 4, hipMemcpyDeviceToHost));
+    int64_t ent[2] = {0, 0};
+    HIPCHK(hipMemcpy(ent, ent_acc.p, 16, hipMemcpyDeviceToHost));
+    sur_stats[0] = nfar;
+    sur_stats[1] = sur_stats[2] = 0;
+    for (int32_t f : sur_flag) sur_stats[f ? 1 : 2]++;
+    sur_stats[3] = ent[1];
+    sur_stats[4] = ent[0];
+    judge_pending = false;
+  }
+
+  // Batches of far blocks whose C/Q pools fit the byte budget, their blocks
+  // with pool offsets (one device array, far-list order), and every buffer
+  // the iterations use sized for the largest batch.  Laid out once per tree
+  // build and configuration.
+  uint64_t far_key() const {
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t v : {(uint64_t)builds, (uint64_t)max_iter, (uint64_t)prec, (uint64_t)(sur != nullptr),
+                       (uint64_t)far_c16, (uint64_t)far_q16, (uint64_t)aca_chunk, (uint64_t)far.size(),
+                       (uint64_t)far_pool_bytes, (uint64_t)sur_samples}) {
+      h ^= v;
+      h *= 1099511628211ull;
+    }
+    return h;
+  }
+
+  void prepare_far_batches() {
+    const int64_t nfar = (int64_t)far.size();
+    (void)pool_budget();
+    if (!batches.empty() && batch_key == far_key()) return;
+    batches.clear();
+    const bool f64 = prec == PREC_FP32;
+    const bool c16 = pool16(true), q16 = pool16(false);
+    const size_t escc = f64 ? 72 : (c16 ? 18 : 36), escq = f64 ? 72 : (q16 ? 18 : 36);
+    const int MI = max_iter;
+    auto block_bytes = [&](const FarBlock &f) {
+      const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
+      return kmax * ((int64_t)f.m * (int64_t)escc + (int64_t)f.n * (int64_t)escq);
+    };
+    // BF16 pools keep items in groups of 64 (1,152 bytes, the kernels'
+    // aligned layout); FP32 / FP64 pools interleave 9 entries per item
+    auto pool_bytes = [](int64_t items, size_t esc, bool grouped) {
+      items = std::max<int64_t>(items, 1);
+      return grouped ? (size_t)((items + 63) / 64) * 1152 : (size_t)items * esc;
+    };
+    std::vector<FarBlock> hb((size_t)nfar);
+    size_t cmax = 1, qmax = 1;
+    int64_t nbmax = 1, tcmax = 1, trmax = 1, ccmax = 1, crmax = 1, ymax = 1;
+    const int CH = aca_chunk;
+    int64_t b0 = 0;
+    while (b0 < nfar) {
+      FarBatch B{b0, b0, 0, 0, 0, 0, 0, 0};
+      int64_t bytes = 0, co = 0, qo = 0;
+      while (B.b1 < nfar) {
+        FarBlock f = far[(size_t)B.b1];
+        if (B.b1 > b0 && bytes + block_bytes(f) > (int64_t)pool_budget()) break;
+        bytes += block_bytes(f);
+        const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
+        f.coff = co;
+        f.qoff = qo;
+        co += kmax * f.m;
+        qo += kmax * f.n;
+        hb[(size_t)B.b1] = f;
+        B.ntc += (f.m + 15) / 16;
+        B.ntr += (f.n + 15) / 16;
+        B.ncc += (f.m + CH - 1) / CH;
+        B.ncr += (f.n + CH - 1) / CH;
+        B.ysum_c += f.m;
+        B.ysum_r += f.n;
+        B.b1++;
+      }
+      if (B.ntc > INT32_MAX || B.ntr > INT32_MAX || B.ncc > INT32_MAX || B.ncr > INT32_MAX)
+        throw HHError(HH_ERR_ARG, "far-field batch too large for 32-bit tile indices");
+      cmax = std::max(cmax, pool_bytes(co, escc, c16));
+      qmax = std::max(qmax, pool_bytes(qo, escq, q16));
+      nbmax = std::max(nbmax, B.b1 - B.b0);
+      tcmax = std::max(tcmax, B.ntc), trmax = std::max(trmax, B.ntr);
+      ccmax = std::max(ccmax, B.ncc), crmax = std::max(crmax, B.ncr);
+      ymax = std::max(ymax, std::max(B.ysum_c, B.ysum_r));
+      batches.push_back(B);
+      b0 = B.b1;
+    }
+    upload(fb_all, hb.data(), hb.size() * sizeof(FarBlock));
+    Cpool.reserve(cmax);
+    Qpool.reserve(qmax);
+    Y.reserve((size_t)ymax * 9 * 4);
+    const size_t nb = (size_t)nbmax;
+    it.reserve(nb * 4);
+    ipiv.reserve(nb * MI * 4);
+    jpiv.reserve(nb * (MI + 1) * 4);
+    mu2.reserve(nb * 8);
+    nu2.reserve(nb * 8);
+    stop.reserve(nb * 4);
+    for (auto &a : act2) a.reserve(nb * 4);
+    ybase_c.reserve(nb * 8);
+    ybase_r.reserve(nb * 8);
+    toff_c.reserve(nb * 4);
+    toff_r.reserve(nb * 4);
+    for (auto *b : {&cfirst_c, &cfirst_r, &a_first_i, &a_first_j}) b->reserve((nb + 1) * 4);
+    applylist.reserve(nb * 4);
+    invb.reserve(nb * 9 * 8);
+    tiles_fc.reserve((size_t)tcmax * 24);
+    tiles_sc.reserve((size_t)tcmax * 24);
+    tiles_fr.reserve((size_t)trmax * 24);
+    tiles_sr.reserve((size_t)trmax * 24);
+    part_c.reserve((size_t)ccmax * kAcaPartBytes);
+    part_r.reserve((size_t)crmax * kAcaPartBytes);
+    x_c.reserve((size_t)ccmax * MI * 9 * 8);
+    x_r.reserve((size_t)crmax * MI * 9 * 8);
+    w_i.reserve((size_t)ccmax * MI * 3 * 8);
+    w_j.reserve((size_t)crmax * MI * 3 * 8);
+    for (auto &c : cnt2) c.reserve(sizeof(AcaCounts));
+    plan_part.reserve((size_t)kPlanGroups * kPlanFields * 8);
+    rank_dev.reserve((size_t)std::max<int64_t>(nfar, 1) * 4);
+    ent_acc.reserve(16);
+    const int S = sur_samples;
+    jd_r.reserve((size_t)std::max<int64_t>(nfar, 1) * S * 12);
+    jd_m9.reserve((size_t)std::max<int64_t>(nfar, 1) * S * 36);
+    jd_s6.reserve((size_t)std::max<int64_t>(nfar, 1) * S * 24);
+    jd_flag.reserve((size_t)std::max<int64_t>(nfar, 1) * 4);
+    jd_err.reserve((size_t)std::max<int64_t>(nfar, 1) * 4);
+    for (auto *b : {&h_stop, &h_act, &h_toff_c, &h_toff_r, &h_list}) b->reserve(nb * 4);
+    for (auto *b : {&h_ybase_c, &h_ybase_r}) b->reserve(nb * 8);
+    for (auto *b : {&h_cfirst_c, &h_cfirst_r, &h_afirst_i, &h_afirst_j}) b->reserve((nb + 1) * 4);
+    h_cnt.reserve(sizeof(AcaCounts));
+    h_cnt_apply.reserve(sizeof(AcaCounts));
+    batch_key = far_key();
+  }
+
+  // Launch descriptors of one batch (counts from cnt, grids from the bounds).
+  AcaStepLaunch step_launch_for(const FarBatch &B, const FarBlock *blk, const AcaCounts *cnt,
+                                const int32_t *active, int64_t grid_cap) {
+    AcaStepLaunch sl{};
+    sl.blk = blk;
+    sl.active = active;
+    sl.ybase_c = ybase_c.as<int64_t>();
+    sl.ybase_r = ybase_r.as<int64_t>();
+    sl.Y = Y.as<float>();
+    sl.it = it.as<int32_t>();
+    sl.ipiv = ipiv.as<int32_t>();
+    sl.jpiv = jpiv.as<int32_t>();
+    sl.mu2 = mu2.as<double>();
+    sl.nu2 = nu2.as<double>();
+    sl.stop = stop.as<int32_t>();
+    sl.max_iter = max_iter;
+    sl.eps2 = epsilon * epsilon;
+    sl.cfirst_c = cfirst_c.as<int32_t>();
+    sl.cfirst_r = cfirst_r.as<int32_t>();
+    sl.part_c = part_c.p;
+    sl.part_r = part_r.p;
+    sl.x_c = x_c.as<double>();
+    sl.x_r = x_r.as<double>();
+    sl.inv = invb.as<double>();
+    sl.chunk = aca_chunk;
+    sl.cnt = cnt;
+    sl.nactive = B.b1 - B.b0;
+    sl.nchunks_c = B.ncc;
+    sl.nchunks_r = B.ncr;
+    sl.grid_cap = grid_cap;
+    sl.pool_f64 = prec == PREC_FP32;
+    sl.pool_c16 = pool16(true);
+    sl.pool_q16 = pool16(false);
+    sl.C = Cpool.p;
+    sl.Q = Qpool.p;
+    return sl;
+  }
+
+  AcaApplyLaunch apply_launch_for(const FarBatch &B, const FarBlock *blk, const AcaCounts *cnt,
+                                  int64_t grid_cap) {
+    AcaApplyLaunch al{};
+    al.blk = blk;
+    al.list = applylist.as<int32_t>();
+    al.it = it.as<int32_t>();
+    al.fp = fp.as<float4>();
+    al.u = u.as<double>();
+    al.use_symmetry = use_symmetry;
+    al.cnt = cnt;
+    al.nlist = B.b1 - B.b0;
+    al.grid_cap = grid_cap;
+    al.pool_f64 = prec == PREC_FP32;
+    al.pool_c16 = pool16(true);
+    al.pool_q16 = pool16(false);
+    al.C = Cpool.p;
+    al.Q = Qpool.p;
+    al.first_i = a_first_i.as<int32_t>();
+    al.first_j = a_first_j.as<int32_t>();
+    al.nchunks_i = B.ncc;
+    al.nchunks_j = B.ncr;
+    al.w_i = w_i.as<double>();
+    al.w_j = w_j.as<double>();
+    al.max_iter = max_iter;
+    al.chunk = aca_chunk;
+    return al;
+  }
+
+  // One ACA iteration's passes over the active slots of cnt: per pass the
+  // tile lists, the full-MLP entries, the surrogate entries, the steps.
+  // hc: the host's copy of the counts (host control: exact grids), or NULL
+  // (device control: grids from the batch bounds, at most grid_cap).
+  void aca_passes(const FarBatch &B, const FarBlock *blk, const int32_t *kind, const AcaCounts *cnt,
+                  const AcaCounts *hc, const int32_t *active, int64_t grid_cap, hipStream_t st,
+                  bool timed) {
+    AcaStepLaunch sl = step_launch_for(B, blk, cnt, active, grid_cap);
+    if (hc) sl.nactive = hc->na, sl.nchunks_c = hc->ncc, sl.nchunks_r = hc->ncr;
+    for (int pass = 0; pass < 2; pass++) {
+      AcaExpandLaunch ex{};
+      ex.blk = blk;
+      ex.active = active;
+      ex.it = it.as<int32_t>();
+      ex.ipiv = ipiv.as<int32_t>();
+      ex.jpiv = jpiv.as<int32_t>();
+      ex.ybase = pass ? ybase_r.as<int64_t>() : ybase_c.as<int64_t>();
+      ex.toff = pass ? toff_r.as<int32_t>() : toff_c.as<int32_t>();
+      ex.kind = kind;
+      ex.cnt = cnt;
+      ex.grid = hc ? hc->na : B.b1 - B.b0;
+      ex.max_iter = max_iter;
+      ex.row_pass = pass;
+      ex.tiles_full = pass ? tiles_fr.p : tiles_fc.p;
+      ex.tiles_sur = pass ? tiles_sr.p : tiles_sc.p;
+      HIPCHK(launch_aca_expand(ex, st));
+      AcaMlpLaunch ml{};
+      ml.prec = prec;
+      ml.xp = xp.as<float4>();
+      ml.tiles = ex.tiles_full;
+      ml.dntiles = pass ? &cnt->ntr[0] : &cnt->ntc[0];
+      ml.ntiles = hc ? (pass ? hc->ntr[0] : hc->ntc[0]) : (pass ? B.ntr : B.ntc);
+      ml.row_pass = pass;
+      ml.Y = Y.as<float>();
+      ml.wf = devw_f(false);
+      ml.wh = devw_h(prec);
+      ml.max_blocks = 8 * g_cus;
+      ml.cus = g_cus;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (timed) {
+        while (evpool.size() < ev_used + 2) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          evpool.push_back(e);
+        }
+        e0 = evpool[ev_used], e1 = evpool[ev_used + 1];
+        ev_used += 2;
+        HIPCHK(hipEventRecord(e0, st));
+      }
+      HIPCHK(launch_aca_mlp(ml, st));
+      if (timed) HIPCHK(hipEventRecord(e1, st));
+      if (sur && kind) {
+        AcaSurLaunch su{};
+        su.xp = xp.as<float4>();
+        su.tiles = ex.tiles_sur;
+        su.dntiles = pass ? &cnt->ntr[1] : &cnt->ntc[1];
+        su.ntiles = hc ? (pass ? hc->ntr[1] : hc->ntc[1]) : (pass ? B.ntr : B.ntc);
+        su.row_pass = pass;
+        su.Y = Y.as<float>();
+        su.params = sur->params.as<float>();
+        su.inv_scale = 1.0f / sur->r_hi;
+        su.cus = g_cus;
+        HIPCHK(launch_aca_sur(su, st));
+      }
+      HIPCHK(launch_aca_steps(sl, pass, st));
+    }
+  }
+  size_t ev_used = 0;
+
+  // Host control of one batch: per iteration the host builds the active
+  // list and its prefix counts, uploads them, and reads the stop flags back.
+  void batch_host_control(const FarBatch &B, const FarBlock *blk, const std::vector<FarBlock> &hb,
+                          const int32_t *kind, const std::vector<int32_t> &hkind, hipStream_t st) {
+    const int nb = (int)(B.b1 - B.b0);
+    const int CH = aca_chunk;
+    AcaCounts *hc = h_cnt.as<AcaCounts>(), *ha = h_cnt_apply.as<AcaCounts>();
+    AcaCounts *dc = cnt2[0].as<AcaCounts>();
+    std::vector<int32_t> act((size_t)nb);
+    for (int b = 0; b < nb; b++) act[(size_t)b] = b;
+    int64_t ent[2] = {0, 0};
+    while (!act.empty()) {
+      const int na = (int)act.size();
+      int32_t *a_ = h_act.as<int32_t>(), *tc = h_toff_c.as<int32_t>(), *tr = h_toff_r.as<int32_t>();
+      int32_t *fc = h_cfirst_c.as<int32_t>(), *fr = h_cfirst_r.as<int32_t>();
+      int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
+      AcaCounts c{};
+      for (int a = 0; a < na; a++) {
+        const int b = act[(size_t)a];
+        const FarBlock &f = hb[(size_t)b];
+        const int kd = hkind.empty() ? 0 : hkind[(size_t)b];
+        a_[a] = b;
+        yc[a] = c.ysum_c;
+        yr[a] = c.ysum_r;
+        tc[a] = c.ntc[kd];
+        tr[a] = c.ntr[kd];
+        fc[a] = c.ncc;
+        fr[a] = c.ncr;
+        c.ysum_c += f.m;
+        c.ysum_r += f.n;
+        c.ntc[kd] += (f.m + 15) / 16;
+        c.ntr[kd] += (f.n + 15) / 16;
+        c.ncc += (f.m + CH - 1) / CH;
+        c.ncr += (f.n + CH - 1) / CH;
+        ent[kd] += f.m + f.n;
+      }
+      c.na = na;
+      fc[na] = c.ncc;
+      fr[na] = c.ncr;
+      *hc = c;
+      HIPCHK(hipMemcpyAsync(act2[0].p, a_, (size_t)na * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(ybase_c.p, yc, (size_t)na * 8, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(ybase_r.p, yr, (size_t)na * 8, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(toff_c.p, tc, (size_t)na * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(toff_r.p, tr, (size_t)na * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(cfirst_c.p, fc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(cfirst_r.p, fr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dc, hc, sizeof(AcaCounts), hipMemcpyHostToDevice, st));
+      aca_passes(B, blk, kind, dc, &c, act2[0].as<int32_t>(), INT64_MAX, st, true);
+      HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      const int32_t *hs = h_stop.as<int32_t>();
+      std::vector<int32_t> done, keep;
+      for (int a = 0; a < na; a++) (hs[a] ? done : keep).push_back(act[(size_t)a]);
+      if (!done.empty()) {
+        const int nl = (int)done.size();
+        int32_t *hl = h_list.as<int32_t>(), *fi = h_afirst_i.as<int32_t>(), *fj = h_afirst_j.as<int32_t>();
+        AcaCounts d = c;
+        d.nl = nl;
+        d.nci = d.ncj = 0;
+        for (int a = 0; a < nl; a++) {
+          const FarBlock &f = hb[(size_t)done[(size_t)a]];
+          hl[a] = done[(size_t)a];
+          fi[a] = d.nci;
+          fj[a] = d.ncj;
+          d.nci += (f.m + CH - 1) / CH;
+          d.ncj += (f.n + CH - 1) / CH;
+        }
+        fi[nl] = d.nci;
+        fj[nl] = d.ncj;
+        *ha = d;
+        HIPCHK(hipMemcpyAsync(applylist.p, hl, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(a_first_i.p, fi, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(a_first_j.p, fj, (size_t)(nl + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dc, ha, sizeof(AcaCounts), hipMemcpyHostToDevice, st));
+        AcaApplyLaunch al = apply_launch_for(B, blk, dc, INT64_MAX);
+        al.nlist = nl;
+        al.nchunks_i = d.nci;
+        al.nchunks_j = d.ncj;
+        HIPCHK(launch_aca_apply(al, st));
+        HIPCHK(hipStreamSynchronize(st));  // the pinned lists are rewritten next iteration
+      }
+      act.swap(keep);
+    }
+    // entry counts of this batch onto the device accumulator
+    int64_t acc[2] = {0, 0};
+    HIPCHK(hipMemcpy(acc, ent_acc.p, 16, hipMemcpyDeviceToHost));
+    acc[0] += ent[0];
+    acc[1] += ent[1];
+    HIPCHK(hipMemcpy(ent_acc.p, acc, 16, hipMemcpyHostToDevice));
+  }
+
+  // Device control of one batch: the first plan takes every block; each
+  // iteration's finalize step writes the stop flags, the next plan compacts
+  // the active list and lists the stopped blocks, which are applied at once.
+  // max_iter iterations stop every block.  Enqueue only: no host read.
+  void batch_device_control(const FarBatch &B, const FarBlock *blk, const int32_t *kind,
+                            hipStream_t st, bool timed) {
+    const int nb = (int)(B.b1 - B.b0);
+    const int64_t cap = 32 * (int64_t)g_cus;
+    AcaPlanLaunch pl{};
+    pl.blk = blk;
+    pl.kind = kind;
+    pl.nb = nb;
+    pl.chunk = aca_chunk;
+    pl.ybase_c = ybase_c.as<int64_t>();
+    pl.ybase_r = ybase_r.as<int64_t>();
+    pl.toff_c = toff_c.as<int32_t>();
+    pl.toff_r = toff_r.as<int32_t>();
+    pl.cfirst_c = cfirst_c.as<int32_t>();
+    pl.cfirst_r = cfirst_r.as<int32_t>();
+    pl.alist = applylist.as<int32_t>();
+    pl.first_i = a_first_i.as<int32_t>();
+    pl.first_j = a_first_j.as<int32_t>();
+    pl.part = plan_part.as<int64_t>();
+    pl.act_in = nullptr, pl.stop_in = nullptr, pl.cnt_in = nullptr;
+    pl.act_out = act2[0].as<int32_t>();
+    pl.cnt_out = cnt2[0].as<AcaCounts>();
+    HIPCHK(launch_aca_plan(pl, st));
+    int cur = 0;
+    for (int iter = 0; iter < max_iter; iter++) {
+      const AcaCounts *c = cnt2[cur].as<AcaCounts>();
+      aca_passes(B, blk, kind, c, nullptr, act2[cur].as<int32_t>(), cap, st, timed);
+      const int nxt = cur ^ 1;
+      pl.act_in = act2[cur].as<int32_t>();
+      pl.stop_in = stop.as<int32_t>();
+      pl.cnt_in = c;
+      pl.act_out = act2[nxt].as<int32_t>();
+      pl.cnt_out = cnt2[nxt].as<AcaCounts>();
+      HIPCHK(launch_aca_plan(pl, st));
+      HIPCHK(launch_aca_apply(apply_launch_for(B, blk, cnt2[nxt].as<AcaCounts>(), cap), st));
+      cur = nxt;
+    }
+    HIPCHK(launch_aca_entries(cnt2[cur].as<AcaCounts>(), ent_acc.as<int64_t>(), st));
+  }
+
+  // Batched far field: judge, then per batch the iterations under device or
+  // host control; ranks into rank_dev (read when asked for).
+  void far_field(hipStream_t st, bool timed = true) {
     if (resolved_far_mode() == 0) {
       far_field_persistent(st);
       return;
     }
     const int64_t nfar = (int64_t)far.size();
     if (nfar == 0) return;
+    prepare_far_batches();
     judge_surrogate(st);
-    const bool f64 = prec == PREC_FP32;
-    const bool c16 = pool16(true), q16 = pool16(false);
-    const size_t escc = f64 ? 72 : (c16 ? 18 : 36), escq = f64 ? 72 : (q16 ? 18 : 36);
-    const int MI = max_iter;
-    // batch budget: C and Q pools, kmax m and kmax n entries
-    auto block_bytes = [&](const FarBlock &f) {
-      const int64_t kmax = std::min<int64_t>(MI, std::min(f.m, f.n));
-      return kmax * ((int64_t)f.m * (int64_t)escc + (int64_t)f.n * (int64_t)escq);
-    };
-    int64_t b0 = 0;
-    while (b0 < nfar) {
-      // batch: blocks [b0, b1) whose reservations fit
-      int64_t b1 = b0, bytes = 0, ysum_c = 0, ysum_r = 0;
-      while (b1 < nfar) {
-        const FarBlock &f = far[(size_t)b1];
-        if (b1 > b0 && bytes + block_bytes(f) > (int64_t)pool_budget()) break;
-        bytes += block_bytes(f);
-        ysum_c += f.m;
-        ysum_r += f.n;
-        b1++;
+    HIPCHK(launch_zero_f64(ent_acc.as<double>(), 2, st));  // two int64 zeros
+    ev_used = 0;
+    const bool host_ctl = aca_control == 0;
+    std::vector<int32_t> hkind;
+    std::vector<FarBlock> hb;
+    if (host_ctl) {
+      if (sur) {
+        fetch_judge();
+        hkind = sur_flag;
+        judge_pending = true;  // entry counts still to come
       }
-      const int nb = (int)(b1 - b0);
-      std::vector<FarBlock> hb((size_t)nb);
-      int64_t co = 0, qo = 0;
-      for (int b = 0; b < nb; b++) {
-        hb[(size_t)b] = far[(size_t)(b0 + b)];
-        const int64_t kmax = std::min<int64_t>(MI, std::min(hb[(size_t)b].m, hb[(size_t)b].n));
-        hb[(size_t)b].coff = co;
-        hb[(size_t)b].qoff = qo;
-        co += kmax * hb[(size_t)b].m;
-        qo += kmax * hb[(size_t)b].n;
-      }
-      // BF16 pools keep items in groups of 64 (1,152 bytes, the kernels'
-      // aligned layout); FP32 / FP64 pools interleave 9 entries per item
-      auto pool_bytes = [](int64_t items, size_t esc, bool grouped) {
-        items = std::max<int64_t>(items, 1);
-        return grouped ? (size_t)((items + 63) / 64) * 1152 : (size_t)items * esc;
-      };
-      Cpool.reserve(pool_bytes(co, escc, c16));
-      Qpool.reserve(pool_bytes(qo, escq, q16));
-      Y.reserve((size_t)std::max(ysum_c, ysum_r) * 9 * 4);
-      upload(blk, hb.data(), hb.size() * sizeof(FarBlock));
-      std::vector<int32_t> zeros_it((size_t)nb, 0), jp((size_t)nb * (MI + 1), 0);
-      for (int b = 0; b < nb; b++) jp[(size_t)b * (MI + 1)] = hb[(size_t)b].n / 2;
-      std::vector<double> zd((size_t)nb, 0.0);
-      upload(it, zeros_it.data(), (size_t)nb * 4);
-      ipiv.reserve((size_t)nb * MI * 4);
-      upload(jpiv, jp.data(), jp.size() * 4);
-      upload(mu2, zd.data(), (size_t)nb * 8);
-      upload(nu2, zd.data(), (size_t)nb * 8);
-      stop.reserve((size_t)nb * 4);
-      active_c.reserve((size_t)nb * 4);
-      ybase_c.reserve((size_t)nb * 8);
-      ybase_r.reserve((size_t)nb * 8);
-      applylist.reserve((size_t)nb * 4);
-      h_active.reserve((size_t)nb * 4);
-      h_ybase_c.reserve((size_t)nb * 8);
-      h_ybase_r.reserve((size_t)nb * 8);
-      h_stop.reserve((size_t)nb * 4);
-      h_list.reserve((size_t)nb * 4);
-      // active list: blocks on full-MLP entries first, then the surrogate's
-      // (the order survives the filtering below), so that each pass's tiles
-      // split into one full-MLP range and one surrogate range
-      std::vector<int32_t> act;
-      act.reserve((size_t)nb);
-      for (int pass = 0; pass < 2; pass++)
-        for (int b = 0; b < nb; b++)
-          if ((sur_flag[(size_t)(b0 + b)] != 0) == (pass == 1)) act.push_back(b);
-      int iter = 0;
-      auto last = std::chrono::steady_clock::now();
-      while (!act.empty()) {
-        const int na = (int)act.size();
-        if (verbose) {
-          const auto now = std::chrono::steady_clock::now();
-          if (iter == 0 || std::chrono::duration<double>(now - last).count() > 10.0) {
-            std::fprintf(stderr, "[hh] far blocks %lld..%lld of %lld: iteration %d, %d active\n",
-                         (long long)b0, (long long)b1, (long long)nfar, iter, na);
-            std::fflush(stderr);
-            last = now;
-          }
-        }
-        iter++;
-        // per-slot prefix counts of the 16-item tiles (MLP passes) and the
-        // aca_chunk-item chunks (steps) on the column (i in I) and row
-        // (j in J) sides; kernels locate their tile or chunk from these
-        for (auto *b : {&h_tfirst_c, &h_tfirst_r, &h_cfirst_c, &h_cfirst_r})
-          b->reserve((size_t)(na + 1) * 4);
-        for (auto *b : {&tfirst_c, &tfirst_r, &cfirst_c, &cfirst_r})
-          b->reserve((size_t)(na + 1) * 4);
-        int32_t *tc = h_tfirst_c.as<int32_t>(), *tr = h_tfirst_r.as<int32_t>();
-        int32_t *fc = h_cfirst_c.as<int32_t>(), *fr = h_cfirst_r.as<int32_t>();
-        int64_t *yc = h_ybase_c.as<int64_t>(), *yr = h_ybase_r.as<int64_t>();
-        int32_t *ha = h_active.as<int32_t>();
-        int64_t sc = 0, sr = 0, ntc = 0, ntr = 0, ncc = 0, ncr = 0;
-        int nfull = 0;  // active blocks on full-MLP entries (the first nfull)
-        int64_t full_items = 0;
-        for (int a = 0; a < na; a++) {
-          const FarBlock &f = hb[(size_t)act[(size_t)a]];
-          if (!sur_flag[(size_t)(b0 + act[(size_t)a])]) nfull = a + 1, full_items += f.m + f.n;
-          ha[a] = act[(size_t)a];
-          yc[a] = sc;
-          yr[a] = sr;
-          tc[a] = (int32_t)ntc;
-          tr[a] = (int32_t)ntr;
-          fc[a] = (int32_t)ncc;
-          fr[a] = (int32_t)ncr;
-          sc += f.m;
-          sr += f.n;
-          ntc += (f.m + 15) / 16;
-          ntr += (f.n + 15) / 16;
-          ncc += (f.m + aca_chunk - 1) / aca_chunk;
-          ncr += (f.n + aca_chunk - 1) / aca_chunk;
-        }
-        if (ntc > INT32_MAX || ntr > INT32_MAX)
-          throw HHError(HH_ERR_ARG, "far-field batch too large for 32-bit tile indices");
-        tc[na] = (int32_t)ntc;
-        tr[na] = (int32_t)ntr;
-        fc[na] = (int32_t)ncc;
-        fr[na] = (int32_t)ncr;
-        tiles_c.reserve((size_t)ntc * 24);
-        tiles_r.reserve((size_t)ntr * 24);
-        part_c.reserve((size_t)ncc * kAcaPartBytes);
-        part_r.reserve((size_t)ncr * kAcaPartBytes);
-        x_c.reserve((size_t)ncc * MI * 9 * 8);
-        x_r.reserve((size_t)ncr * MI * 9 * 8);
-        invb.reserve((size_t)na * 9 * 8);
-        HIPCHK(hipMemcpyAsync(active_c.p, ha, (size_t)na * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(ybase_c.p, yc, (size_t)na * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(ybase_r.p, yr, (size_t)na * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(tfirst_c.p, tc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(tfirst_r.p, tr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(cfirst_c.p, fc, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(cfirst_r.p, fr, (size_t)(na + 1) * 4, hipMemcpyHostToDevice, st));
-        AcaMlpLaunch ml{};
-        ml.prec = prec;
-        ml.xp = xp.as<float4>();
-        ml.blk = blk.as<FarBlock>();
-        ml.active = active_c.as<int32_t>();
-        ml.it = it.as<int32_t>();
-        ml.ipiv = ipiv.as<int32_t>();
-        ml.jpiv = jpiv.as<int32_t>();
-        ml.max_iter = MI;
-        ml.Y = Y.as<float>();
-        ml.wf = devw_f(false);
-        ml.wh = devw_h(prec);
-        ml.max_blocks = 8 * g_cus;
-        ml.cus = g_cus;
-        AcaStepLaunch sl{};
-        sl.blk = blk.as<FarBlock>();
-        sl.active = active_c.as<int32_t>();
-        sl.ybase_c = ybase_c.as<int64_t>();
-        sl.ybase_r = ybase_r.as<int64_t>();
-        sl.Y = Y.as<float>();
-        sl.it = it.as<int32_t>();
-        sl.ipiv = ipiv.as<int32_t>();
-        sl.jpiv = jpiv.as<int32_t>();
-        sl.mu2 = mu2.as<double>();
-        sl.nu2 = nu2.as<double>();
-        sl.stop = stop.as<int32_t>();
-        sl.max_iter = MI;
-        sl.eps2 = epsilon * epsilon;
-        sl.cfirst_c = cfirst_c.as<int32_t>();
-        sl.cfirst_r = cfirst_r.as<int32_t>();
-        sl.part_c = part_c.p;
-        sl.part_r = part_r.p;
-        sl.x_c = x_c.as<double>();
-        sl.x_r = x_r.as<double>();
-        sl.inv = invb.as<double>();
-        sl.chunk = aca_chunk;
-        sl.nactive = na;
-        sl.nchunks_c = ncc;
-        sl.nchunks_r = ncr;
-        sl.pool_f64 = f64;
-        sl.pool_c16 = c16;
-        sl.pool_q16 = q16;
-        sl.C = Cpool.p;
-        sl.Q = Qpool.p;
-        // column pass, residual, row pivot, normalisation
-        // tiles of the full-MLP blocks come first in each pass's tile list
-        const int64_t ntc_full = tc[nfull], ntr_full = tr[nfull];
-        AcaSurLaunch su{};
-        su.xp = xp.as<float4>();
-        su.Y = Y.as<float>();
-        su.cus = g_cus;
-        if (sur) su.params = sur->params.as<float>(), su.inv_scale = 1.0f / sur->r_hi;
-        ml.row_pass = 0;
-        ml.ybase = ybase_c.as<int64_t>();
-        ml.tfirst = tfirst_c.as<int32_t>();
-        ml.nactive = na;
-        ml.tiles = tiles_c.p;
-        ml.ntiles = ntc_full;
-        HIPCHK(hipEventRecord(evm[0], st));
-        HIPCHK(launch_aca_mlp(ml, st));
-        HIPCHK(hipEventRecord(evm[1], st));
-        su.tiles = tiles_c.p, su.tile0 = ntc_full, su.ntiles = ntc - ntc_full, su.row_pass = 0;
-        HIPCHK(launch_aca_sur(su, st));
-        HIPCHK(launch_aca_steps(sl, 0, st));
-        // row pass, residual, column pivot, norms, stop test
-        ml.row_pass = 1;
-        ml.ybase = ybase_r.as<int64_t>();
-        ml.tfirst = tfirst_r.as<int32_t>();
-        ml.nactive = na;
-        ml.tiles = tiles_r.p;
-        ml.ntiles = ntr_full;
-        HIPCHK(hipEventRecord(evm[2], st));
-        HIPCHK(launch_aca_mlp(ml, st));
-        HIPCHK(hipEventRecord(evm[3], st));
-        su.tiles = tiles_r.p, su.tile0 = ntr_full, su.ntiles = ntr - ntr_full, su.row_pass = 1;
-        HIPCHK(launch_aca_sur(su, st));
-        HIPCHK(launch_aca_steps(sl, 1, st));
-        HIPCHK(hipMemcpyAsync(h_stop.p, stop.p, (size_t)na * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        {
-          float t0 = 0, t1 = 0;
-          HIPCHK(hipEventElapsedTime(&t0, evm[0], evm[1]));
-          HIPCHK(hipEventElapsedTime(&t1, evm[2], evm[3]));
-          prof_ms[1] += (double)t0 + (double)t1;
-          prof_pairs[1] += full_items;
-          prof_launches[1] += 2;
-          sur_stats[3] += sc + sr - full_items;
-          sur_stats[4] += full_items;
-        }
-        const int32_t *hs = h_stop.as<int32_t>();
-        std::vector<int32_t> done, keep;
-        for (int a = 0; a < na; a++) (hs[a] ? done : keep).push_back(act[(size_t)a]);
-        if (!done.empty()) apply_blocks(hb, done, MI, f64, st);
-        act.swap(keep);
-      }
-      std::vector<int32_t> ranks((size_t)nb);
-      HIPCHK(hipMemcpyAsync(ranks.data(), it.p, (size_t)nb * 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      for (int b = 0; b < nb; b++) far_rank[(size_t)(b0 + b)] = ranks[(size_t)b];
-      b0 = b1;
+      hb.resize((size_t)nfar);
+      HIPCHK(hipMemcpy(hb.data(), fb_all.p, (size_t)nfar * sizeof(FarBlock), hipMemcpyDeviceToHost));
     }
+    for (const FarBatch &B : batches) {
+      const int nb = (int)(B.b1 - B.b0);
+      const FarBlock *blk = fb_all.as<FarBlock>() + B.b0;
+      const int32_t *kind = sur ? jd_flag.as<int32_t>() + B.b0 : nullptr;
+      HIPCHK(launch_aca_init(blk, nb, max_iter, it.as<int32_t>(), jpiv.as<int32_t>(), mu2.as<double>(),
+                             nu2.as<double>(), st));
+      if (host_ctl) {
+        std::vector<FarBlock> hbb(hb.begin() + B.b0, hb.begin() + B.b1);
+        std::vector<int32_t> hk;
+        if (!hkind.empty()) hk.assign(hkind.begin() + B.b0, hkind.begin() + B.b1);
+        batch_host_control(B, blk, hbb, kind, hk, st);
+      } else {
+        batch_device_control(B, blk, kind, st, timed);
+      }
+      HIPCHK(launch_copy_ranks(it.as<int32_t>(), nb, rank_dev.as<int32_t>() + B.b0, st));
+    }
+    if (sur) judge_pending = true;
+    rank_src = rank_dev.as<int32_t>();
+    rank_pending = true;
+  }
+
+  // ACA pass times (full-MLP entries) of the last batched far field, after
+  // the stream has completed it.
+  void collect_aca_profile() {
+    prof_ms[1] = 0;
+    for (size_t i = 0; i + 1 < ev_used; i += 2) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, evpool[i], evpool[i + 1]));
+      prof_ms[1] += t;
+    }
+    prof_launches[1] = (int64_t)(ev_used / 2);
+    ev_used = 0;
+    int64_t ent[2] = {0, 0};
+    HIPCHK(hipMemcpy(ent, ent_acc.p, 16, hipMemcpyDeviceToHost));
+    prof_pairs[1] = ent[0];
   }
 
   void near_field(hipStream_t st, const DBuf &blocks, const DBuf &tasks, int64_t ntasks,
@@ -963,7 +1151,7 @@ struct hh_model {
     if (persistent)
       far_field_persistent(st);
     else
-      far_field(st);
+      far_field(st, timed);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
   }
@@ -977,9 +1165,13 @@ struct hh_model {
   // build), the far field on the persistent kernel (no host reads per ACA
   // iteration) and its buffers set up by an earlier dot.
   bool capturable() const {
-    if (!built || !tree_reuse || !has_weights || resolved_far_mode() != 0 || !ev[0]) return false;
+    if (!built || !tree_reuse || !has_weights || !ev[0]) return false;
     if (fp.bytes < 16 * (size_t)n || u.bytes < 24 * (size_t)n) return false;
-    return far.empty() || (p_ready && p_maxiter == max_iter && p_f64 == (int)(prec == PREC_FP32));
+    if (far.empty()) return true;
+    if (resolved_far_mode() == 0)
+      return p_ready && p_maxiter == max_iter && p_f64 == (int)(prec == PREC_FP32);
+    // batched iterations under device control, laid out by an earlier dot
+    return aca_control == 1 && !batches.empty() && batch_key == far_key();
   }
 
   // Changes whenever something a captured matvec reads or launches with does.
@@ -991,12 +1183,17 @@ struct hh_model {
     };
     uint64_t e;
     std::memcpy(&e, &epsilon, 8);
+    uint64_t tol;
+    std::memcpy(&tol, &sur_tol, 8);
     for (uint64_t v : {(uint64_t)prec, (uint64_t)max_iter, e, (uint64_t)use_symmetry,
                        (uint64_t)n_near_tasks, (uint64_t)p_nslots, (uint64_t)p_slotC,
-                       (uint64_t)p_slotQ, (uint64_t)p_slotY, (uint64_t)far.size(), builds, loads})
+                       (uint64_t)p_slotQ, (uint64_t)p_slotY, (uint64_t)far.size(), builds, loads,
+                       (uint64_t)resolved_far_mode(), (uint64_t)aca_control, batch_key,
+                       (uint64_t)(uintptr_t)sur, tol, (uint64_t)sur_samples, (uint64_t)near_overlap})
       mix(v);
     for (const DBuf *b : {&xp, &perm, &fp, &u, &near_blocks, &near_tasks, &p_blk, &p_order, &p_rank,
-                          &p_poolC, &p_poolQ, &p_Y})
+                          &p_poolC, &p_poolQ, &p_Y, &fb_all, &Cpool, &Qpool, &Y, &x_c, &x_r,
+                          &part_c, &part_r, &w_i, &w_j, &jd_flag, &rank_dev})
       mix((uint64_t)(uintptr_t)b->p);
     for (const DBuf *arr : {wf32, wf64}) for (int k = 0; k < 5; k++) mix((uint64_t)(uintptr_t)arr[k].p);
     for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)wh_f[k].p);
@@ -1044,10 +1241,13 @@ struct hh_model {
     if (!capturable()) throw HHError(HH_ERR_STATE, "matvec not capturable in the current state");
     HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
     HIPCHK(launch_zero_f64(u.as<double>(), 3 * n, st));
-    if (!far.empty())
-      near_beside_persistent_far(st, false);
-    else
+    const bool persistent = resolved_far_mode() == 0;
+    if (!far.empty() && (persistent || near_overlap)) {
+      near_beside_far(st, false, persistent);
+    } else {
       near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+      if (!far.empty()) far_field(st, false);
+    }
     HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
   }
 
@@ -1073,6 +1273,7 @@ struct hh_model {
     HIPCHK(launch_scatter_velocity(u.as<double>(), n, perm.as<int32_t>(), u_dev, st));
     HIPCHK(hipEventRecord(ev_all[1], st));
     HIPCHK(hipEventSynchronize(ev_all[1]));
+    if (!persistent && !far.empty()) collect_aca_profile();
     float a = 0, b = 0, c = 0;
     HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
     HIPCHK(hipEventElapsedTime(&b, far0_ev, ev[2]));
@@ -1272,11 +1473,12 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       throw HHError(HH_ERR_DEVICE, "hipStreamCreate failed");
     }
     if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
-    if (const char *tb = std::getenv("HH_TREE_BUILD")) m->tree_device = std::strcmp(tb, "device") == 0;
+    if (const char *tb = std::getenv("HH_TREE_BUILD")) m->tree_device = std::strcmp(tb, "host") != 0;
     if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
     if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
     if (const char *ov = std::getenv("HH_NEAR_OVERLAP")) m->near_overlap = std::atoi(ov) != 0;
+    if (const char *ac = std::getenv("HH_ACA_CONTROL")) m->aca_control = std::strcmp(ac, "host") != 0;
     m->aca_chunk = chunk;
     if (const char *gb = std::getenv("HH_FAR_POOL_GB"))
       m->far_pool_bytes = (size_t)(std::atof(gb) * (double)(1ull << 30));
@@ -1554,6 +1756,7 @@ int hh_model_set_surrogate(hh_model *m, hh_surrogate *s, double tol, int samples
 int hh_model_surrogate_stats(hh_model *m, int64_t stats[5]) {
   return guarded([&]() {
     if (!m || !stats) throw HHError(HH_ERR_ARG, "NULL argument");
+    m->fetch_judge();
     for (int k = 0; k < 5; k++) stats[k] = m->sur_stats[k];
   });
 }
@@ -1561,6 +1764,7 @@ int hh_model_surrogate_stats(hh_model *m, int64_t stats[5]) {
 int hh_model_surrogate_errors(hh_model *m, float *err) {
   return guarded([&]() {
     if (!m || !err) throw HHError(HH_ERR_ARG, "NULL argument");
+    m->fetch_judge();
     if (m->sur_err.size() != m->far.size())
       throw HHError(HH_ERR_STATE, "no judged matvec since the last tree build");
     std::copy(m->sur_err.begin(), m->sur_err.end(), err);

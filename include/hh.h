@@ -88,8 +88,8 @@ int hh_model_set_tree_reuse(hh_model *m, int flag);
 
 /* Where update_coord builds the tree and lists (the reference builds them on
  * the host, src/HignnModel.cpp:457-461): 0 on host threads, 1 on the device
- * (see hh_tree_build_device; the same result bit for bit).  The environment
- * variable HH_TREE_BUILD=device|host sets the default of new models. */
+ * (the default; see hh_tree_build_device: the same result bit for bit).
+ * HH_TREE_BUILD=host in the environment makes new models build on the host. */
 int hh_model_set_tree_build(hh_model *m, int device);
 
 /* dot(u, f) -- src/HignnModel.cpp:545-634: u = M(x) f, host arrays. */

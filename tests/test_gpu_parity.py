@@ -112,7 +112,7 @@ def test_dot_fp32_persistent_far_field_parity(model_small, xavier_w, monkeypatch
 
 def test_dot_enqueue_states_and_graph_replays(model_small, xavier_w, monkeypatch):
     """hh_model_dot_capturable / hh_model_dot_enqueue: refused without tree
-    reuse, in batched mode and before the persistent kernel is set up for
+    reuse, in batched mode under host control and before the persistent kernel is set up for
     the current iteration cap; otherwise the enqueued matvec, directly and
     as a HIP graph replayed with eager work in between, equals the eager
     one.  The signature follows the ACA settings."""
@@ -129,11 +129,13 @@ def test_dot_enqueue_states_and_graph_replays(model_small, xavier_w, monkeypatch
         m.update_coord()
         return m
 
-    monkeypatch.setenv("HH_FAR_MODE", "1")  # batched: stop flags read back per iteration
+    monkeypatch.setenv("HH_FAR_MODE", "1")
+    monkeypatch.setenv("HH_ACA_CONTROL", "host")  # batched, stop flags read back per iteration
     mb = make()
     mb.set(tree_reuse=True)
     mb.dot(f)
     assert not mb.dot_capturable()[0]
+    monkeypatch.delenv("HH_ACA_CONTROL")
 
     monkeypatch.setenv("HH_FAR_MODE", "0")
     m = make()
