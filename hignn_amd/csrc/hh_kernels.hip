@@ -608,7 +608,9 @@ template <class H>
 __global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, const int32_t *dntiles,
                                                                  WH W) {
   A.ntiles = *dntiles;
-  if (A.ntiles <= 0) return;  // uniform: no fragments to stage
+  // a workgroup whose first tile is past the count has no work: it leaves
+  // before staging the fragments (uniform)
+  if ((int64_t)blockIdx.x * (blockDim.x >> 6) * 4 >= A.ntiles) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);

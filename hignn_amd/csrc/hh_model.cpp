@@ -203,6 +203,9 @@ struct hh_model {
                     &rank_dev, &ent_acc})
       b->release();
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_na)
+      if (e) (void)hipEventDestroy(e);
+    h_na.release();
     for (auto &b : wf32) b.release();
     for (auto &b : wf64) b.release();
     for (auto &b : wh_f) b.release();
@@ -992,9 +995,19 @@ struct hh_model {
   // Device control of one batch: the first plan takes every block; each
   // iteration's finalize step writes the stop flags, the next plan compacts
   // the active list and lists the stopped blocks, which are applied at once.
-  // max_iter iterations stop every block.  Enqueue only: no host read.
+  // max_iter iterations stop every block.  With early_exit (not under stream
+  // capture) each plan's active count is copied to pinned memory; the host
+  // waits for the copy kLag iterations back -- the device still has kLag
+  // iterations queued -- and stops enqueuing once that count is zero.
+  static constexpr int kLag = 3, kRing = 8;
+  PBuf h_na;
+  hipEvent_t ev_na[kRing] = {};
   void batch_device_control(const FarBatch &B, const FarBlock *blk, const int32_t *kind,
-                            hipStream_t st, bool timed) {
+                            hipStream_t st, bool timed, bool early_exit) {
+    if (early_exit && !ev_na[0]) {
+      for (auto &e : ev_na) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      h_na.reserve(kRing * sizeof(int32_t));
+    }
     const int nb = (int)(B.b1 - B.b0);
     const int64_t cap = 32 * (int64_t)g_cus;
     AcaPlanLaunch pl{};
@@ -1029,6 +1042,16 @@ struct hh_model {
       HIPCHK(launch_aca_plan(pl, st));
       HIPCHK(launch_aca_apply(apply_launch_for(B, blk, cnt2[nxt].as<AcaCounts>(), cap), st));
       cur = nxt;
+      if (early_exit) {
+        int32_t *na = h_na.as<int32_t>();
+        HIPCHK(hipMemcpyAsync(na + iter % kRing, &cnt2[cur].as<AcaCounts>()->na, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(ev_na[iter % kRing], st));
+        if (iter >= kLag) {
+          const int back = (iter - kLag) % kRing;
+          HIPCHK(hipEventSynchronize(ev_na[back]));
+          if (na[back] == 0) break;  // every block stopped and was applied by then
+        }
+      }
     }
     HIPCHK(launch_aca_entries(cnt2[cur].as<AcaCounts>(), ent_acc.as<int64_t>(), st));
   }
@@ -1060,6 +1083,11 @@ struct hh_model {
     }
     for (const FarBatch &B : batches) {
       const int nb = (int)(B.b1 - B.b0);
+      if (verbose) {
+        std::fprintf(stderr, "[hh] far blocks %lld..%lld of %lld (%s control)\n", (long long)B.b0,
+                     (long long)B.b1, (long long)nfar, host_ctl ? "host" : "device");
+        std::fflush(stderr);
+      }
       const FarBlock *blk = fb_all.as<FarBlock>() + B.b0;
       const int32_t *kind = sur ? jd_flag.as<int32_t>() + B.b0 : nullptr;
       HIPCHK(launch_aca_init(blk, nb, max_iter, it.as<int32_t>(), jpiv.as<int32_t>(), mu2.as<double>(),
@@ -1070,7 +1098,7 @@ struct hh_model {
         if (!hkind.empty()) hk.assign(hkind.begin() + B.b0, hkind.begin() + B.b1);
         batch_host_control(B, blk, hbb, kind, hk, st);
       } else {
-        batch_device_control(B, blk, kind, st, timed);
+        batch_device_control(B, blk, kind, st, timed, timed);
       }
       HIPCHK(launch_copy_ranks(it.as<int32_t>(), nb, rank_dev.as<int32_t>() + B.b0, st));
     }

@@ -90,6 +90,10 @@ class Model {
   void set_max_relative_coord(size_t v) { check(hh_model_set_max_relative_coord(m_, (int64_t)v)); }
   void set_post_check_flag(bool v) { check(hh_model_set_post_check_flag(m_, v)); }
   void set_tree_reuse(bool v) { check(hh_model_set_tree_reuse(m_, v)); }
+  void set_tree_build(const std::string &where) {
+    if (where != "device" && where != "host") throw std::invalid_argument("tree build: 'device' or 'host'");
+    check(hh_model_set_tree_build(m_, where == "device" ? 1 : 0));
+  }
   void set_use_symmetry_flag(bool v) { check(hh_model_set_use_symmetry_flag(m_, v)); }
   void set_max_far_field_distance(double v) { check(hh_model_set_max_far_field_distance(m_, v)); }
 
@@ -715,6 +719,7 @@ PYBIND11_MODULE(hignn, m) {
       .def(py::init<py::array_t<float, py::array::forcecast>, int>())
       .def("load_two_body_model", &Model::load_two_body_model)
       .def("set_tree_reuse", &Model::set_tree_reuse)
+      .def("set_tree_build", &Model::set_tree_build)
       .def("load_two_body_weights", &Model::load_two_body_weights)
       .def("load_three_body_model", &Model::load_three_body_model)
       .def("set_epsilon", &Model::set_epsilon)
