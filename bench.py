@@ -319,6 +319,10 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size before the timed region "
                          "(known-size reference for HBM byte counters)")
+    ap.add_argument("--no-tune-near", action="store_true",
+                    help="matvec bench: keep the default tile shape of the 16-bit near kernel")
+    ap.add_argument("--tune-far", action="store_true",
+                    help="matvec bench: also time the far field's launch choices (6 matvecs)")
     ap.add_argument("--no-tune-far", action="store_true",
                     help="stepper: keep the automatic far-field choice instead of timing the "
                          "candidates on the first forces")
@@ -551,6 +555,15 @@ def matvec_bench(args):
         del src, dst
 
     use_sur = sur is not None
+    # tile shape of the 16-bit near kernel (and, with --tune-far, the far
+    # field's launch choices) tuned on these forces, untimed
+    model.set(precision=args.prec)
+    model.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
+    near_tuning = model.tune_near(f_dev.ptr) if not args.no_tune_near else []
+    log(f"[bench] near-kernel tile shapes (shape, tiles, threads, ms): {near_tuning}")
+    far_tuning = model.tune_far_field(f_dev.ptr) if args.tune_far else []
+    if far_tuning:
+        log(f"[bench] far-field launch choices (mode, chunk, matvec ms): {far_tuning}")
     # chosen form: warmup, then K timed steps
     for i in range(args.warmup):
         dt = run(args.prec, use_sur)
@@ -704,6 +717,9 @@ def matvec_bench(args):
             "mean_far_rank": float(ranks.mean()) if ranks.size else 0.0,
             "update_coord_s": round(t_update, 3),
             "update_coord_host_build_s": round(t_update_host, 3),
+            "near_shape_tuning": [{"shape": a, "tiles": b, "threads": c, "ms": round(d, 1)}
+                                  for a, b, c, d in near_tuning],
+            "far_launch_tuning": [{"mode": a, "chunk": b, "ms": round(c, 1)} for a, b, c in far_tuning],
             "close_ms": round(float(np.median(tcl)), 2), "far_ms": round(float(np.median(tfar)), 2),
             "far_mlp_ms": round(far_mlp_ms, 2),
         },

@@ -18,8 +18,17 @@ struct DevWeightsH {  // 16-bit fragments + FP32 layer 1 and biases
   const void *w2f, *w3f;
 };
 
+// Tile shapes of the 16-bit near kernel: 16-pair tiles per MLP call and
+// threads per persistent workgroup (index NearLaunch::hcfg).
+struct NearShape {
+  int tiles, threads;
+};
+constexpr NearShape kNearShapes[] = {{4, 512}, {2, 512}, {2, 1024}, {4, 256}};
+constexpr int kNearShapeCount = 4;
+
 struct NearLaunch {
   int prec;
+  int hcfg;        // 16-bit kernel tile shape (kNearShapes)
   const float4 *xp;
   const float4 *fp;
   const int32_t *tlist;

@@ -456,13 +456,16 @@ __device__ __forceinline__ void stage_fragments(const WH &W, H *lw2, H *lw3) {
 constexpr int kNearHThreads = 512;
 constexpr int kNearHTiles = 4;
 
-template <class H>
-__global__ __launch_bounds__(kNearHThreads) void near_h_kernel(NearSrc<float> S, WH W) {
+// Tile shape of the near kernel: NT 16-pair tiles per MLP call (NT
+// targets of a task at a time), THREADS per persistent workgroup (one per
+// compute unit: THREADS/256 waves per SIMD).  The launch picks one of the
+// shapes below (NearLaunch::hcfg), tuned by hh_model_tune.
+template <class H, int NT, int THREADS>
+__global__ __launch_bounds__(THREADS) void near_h_kernel(NearSrc<float> S, WH W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   H *lw2 = (H *)lds;
   H *lw3 = (H *)(lds + kW2FragBytes);
   stage_fragments<H>(W, lw2, lw3);
-  constexpr int NT = kNearHTiles;
   near_body<float, NT>(S, [&](const float(&rx)[NT], const float(&ry)[NT], const float(&rz)[NT],
                               f32x4(&y)[NT]) { mlp_tiles_h<H, NT>(W, lw2, lw3, rx, ry, rz, y); });
 }
@@ -2317,10 +2320,11 @@ static void allow_big_lds() {
   static bool done = false;
   if (done) return;
   const int lds = kW2FragBytes + kW3FragBytes;
-  (void)hipFuncSetAttribute((const void *)near_h_kernel<bf16x8>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  (void)hipFuncSetAttribute((const void *)near_h_kernel<f16x8>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  for (const void *k : {(const void *)near_h_kernel<bf16x8, 4, 512>, (const void *)near_h_kernel<f16x8, 4, 512>,
+                        (const void *)near_h_kernel<bf16x8, 2, 512>, (const void *)near_h_kernel<f16x8, 2, 512>,
+                        (const void *)near_h_kernel<bf16x8, 2, 1024>, (const void *)near_h_kernel<f16x8, 2, 1024>,
+                        (const void *)near_h_kernel<bf16x8, 4, 256>, (const void *)near_h_kernel<f16x8, 4, 256>})
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipFuncSetAttribute((const void *)aca_mlp_h_kernel<bf16x8>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipFuncSetAttribute((const void *)aca_mlp_h_kernel<f16x8>,
@@ -2357,12 +2361,24 @@ hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
     allow_big_lds();
     NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
     WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
-    const int grid = grid_for(L.ntasks, kNearHThreads / 64, L.cus);
     const size_t lds = kW2FragBytes + kW3FragBytes;
-    if (L.prec == PREC_BF16)
-      hipLaunchKernelGGL(near_h_kernel<bf16x8>, dim3(grid), dim3(kNearHThreads), lds, st, S, W);
-    else
-      hipLaunchKernelGGL(near_h_kernel<f16x8>, dim3(grid), dim3(kNearHThreads), lds, st, S, W);
+    const bool b16 = L.prec == PREC_BF16;
+    const int threads = kNearShapes[L.hcfg].threads;
+    const int grid = grid_for(L.ntasks, threads / 64, L.cus);
+#define HH_NEAR_H(NT, TH)                                                                         \
+  do {                                                                                            \
+    if (b16)                                                                                      \
+      hipLaunchKernelGGL((near_h_kernel<bf16x8, NT, TH>), dim3(grid), dim3(TH), lds, st, S, W);  \
+    else                                                                                          \
+      hipLaunchKernelGGL((near_h_kernel<f16x8, NT, TH>), dim3(grid), dim3(TH), lds, st, S, W);   \
+  } while (0)
+    switch (L.hcfg) {
+      case 1: HH_NEAR_H(2, 512); break;
+      case 2: HH_NEAR_H(2, 1024); break;
+      case 3: HH_NEAR_H(4, 256); break;
+      default: HH_NEAR_H(4, 512);
+    }
+#undef HH_NEAR_H
   }
   return hipGetLastError();
 }

@@ -157,6 +157,7 @@ struct hh_model {
   bool pool16(bool c) const { return prec != PREC_FP32 && !sur && (c ? far_c16 : far_q16); }
   bool verbose = false;  // HH_VERBOSE=1: far-field progress lines on stderr
   int aca_chunk = kAcaChunk;  // HH_ACA_CHUNK: items per ACA step chunk (1..kAcaChunk)
+  int near_hcfg = 0;          // tile shape of the 16-bit near kernel (kNearShapes)
   DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
   bool p_ready = false;
   int64_t p_slotC = 0, p_slotQ = 0, p_slotY = 0;
@@ -1127,6 +1128,7 @@ struct hh_model {
                   int p, double *uacc, const int32_t *tlist) {
     NearLaunch L{};
     L.prec = p;
+    L.hcfg = near_hcfg;
     L.xp = xp.as<float4>();
     L.fp = fp.as<float4>();
     L.tlist = tlist;
@@ -1217,7 +1219,8 @@ struct hh_model {
                        (uint64_t)n_near_tasks, (uint64_t)p_nslots, (uint64_t)p_slotC,
                        (uint64_t)p_slotQ, (uint64_t)p_slotY, (uint64_t)far.size(), builds, loads,
                        (uint64_t)resolved_far_mode(), (uint64_t)aca_control, batch_key,
-                       (uint64_t)(uintptr_t)sur, tol, (uint64_t)sur_samples, (uint64_t)near_overlap})
+                       (uint64_t)(uintptr_t)sur, tol, (uint64_t)sur_samples, (uint64_t)near_overlap,
+                       (uint64_t)near_hcfg})
       mix(v);
     for (const DBuf *b : {&xp, &perm, &fp, &u, &near_blocks, &near_tasks, &p_blk, &p_order, &p_rank,
                           &p_poolC, &p_poolQ, &p_Y, &fb_all, &Cpool, &Qpool, &Y, &x_c, &x_r,
@@ -1262,6 +1265,39 @@ struct hh_model {
     }
     far_mode = pick[0];
     aca_chunk = pick[1];
+  }
+
+  // Tile-shape tuning of the 16-bit near kernel on the caller's forces: each
+  // shape runs the near field once untimed and once timed (HIP events on
+  // st); the fastest is kept.  Rows of `out`: {shape, tiles, threads, ms}.
+  void tune_near(const float *f_dev, hipStream_t st, std::vector<std::array<double, 4>> &out) {
+    check_ready();
+    out.clear();
+    if (prec != PREC_BF16 && prec != PREC_FP16) return;  // the FP32 kernel has one shape
+    HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
+    const int h0 = near_hcfg;
+    int pick = h0;
+    double best = 0;
+    try {
+      for (int c = 0; c < kNearShapeCount; c++) {
+        near_hcfg = c;
+        float ms = 0;
+        for (int rep = 0; rep < 2; rep++) {
+          HIPCHK(launch_zero_f64(u.as<double>(), 3 * n, st));
+          HIPCHK(hipEventRecord(ev[0], st));
+          near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
+          HIPCHK(hipEventRecord(ev[1], st));
+          HIPCHK(hipEventSynchronize(ev[1]));
+          HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        }
+        out.push_back({(double)c, (double)kNearShapes[c].tiles, (double)kNearShapes[c].threads, (double)ms});
+        if (out.size() == 1 || ms < best) best = ms, pick = c;
+      }
+    } catch (...) {
+      near_hcfg = h0;
+      throw;
+    }
+    near_hcfg = pick;
   }
 
   // u = M(x) f without host synchronisation or timing (capturable state only).
@@ -1502,6 +1538,11 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
     }
     if (const char *fm = std::getenv("HH_FAR_MODE")) m->far_mode = std::atoi(fm);
     if (const char *tb = std::getenv("HH_TREE_BUILD")) m->tree_device = std::strcmp(tb, "host") != 0;
+    if (const char *ns = std::getenv("HH_NEAR_SHAPE")) {
+      const int v = std::atoi(ns);
+      if (v < 0 || v >= kNearShapeCount) throw HHError(HH_ERR_ARG, "HH_NEAR_SHAPE out of range");
+      m->near_hcfg = v;
+    }
     if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
     if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
@@ -1635,6 +1676,28 @@ int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *st
     const int nr = (int)std::min<size_t>(out.size(), (size_t)std::max(max_rows, 0));
     for (int r = 0; r < nr && table; r++)
       for (int c = 0; c < 3; c++) table[3 * r + c] = out[(size_t)r][(size_t)c];
+    *rows = nr;
+  });
+}
+
+int hh_model_set_near_shape(hh_model *m, int shape) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (shape < 0 || shape >= kNearShapeCount) throw HHError(HH_ERR_ARG, "near shape out of range");
+    m->near_hcfg = shape;
+  });
+}
+
+int hh_model_tune_near(hh_model *m, const float *f_dev, int64_t n, void *stream, double *table,
+                       int max_rows, int *rows) {
+  return guarded([&]() {
+    if (!m || !f_dev || !rows) throw HHError(HH_ERR_ARG, "NULL argument");
+    if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
+    std::vector<std::array<double, 4>> out;
+    m->tune_near(f_dev, stream ? (hipStream_t)stream : m->stream, out);
+    const int nr = (int)std::min<size_t>(out.size(), (size_t)std::max(max_rows, 0));
+    for (int r = 0; r < nr && table; r++)
+      for (int c = 0; c < 4; c++) table[4 * r + c] = out[(size_t)r][(size_t)c];
     *rows = nr;
   });
 }

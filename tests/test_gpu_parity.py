@@ -195,6 +195,36 @@ def test_tune_far_field(model_small, xavier_w):
     assert O.rel_l2(m.dot(f), om.dot(xavier_w, f)) <= 1e-5
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_near_shapes_and_tuning(model_small, xavier_w, prec):
+    """Every tile shape of the 16-bit near kernel (16-pair tiles per MLP call
+    x threads per workgroup) computes the same matvec up to the FP64
+    summation order; hh_model_tune_near times all of them and keeps the
+    fastest; the FP32 mode has nothing to tune."""
+    x, f = model_small["x"], model_small["f"]
+    m = abi.Model(x, 64)
+    m.load_weights(xavier_w)
+    m.set(epsilon=1e-5, max_iter=50, precision=prec)
+    m.update_coord()
+    us = []
+    for shape in range(4):
+        m.set_near_shape(shape)
+        us.append(m.dot(f))
+    for shape in range(1, 4):
+        assert O.rel_l2(us[shape], us[0]) <= 1e-12, shape
+    with pytest.raises(RuntimeError):
+        m.set_near_shape(4)
+    fd = abi.DeviceArray.of(f)
+    rows = m.tune_near(fd.ptr)
+    print(f"near shapes ({prec}):", rows)
+    assert [r[0] for r in rows] == [0, 1, 2, 3]
+    assert [(r[1], r[2]) for r in rows] == [(4, 512), (2, 512), (2, 1024), (4, 256)]
+    assert all(r[3] > 0 for r in rows)
+    assert O.rel_l2(m.dot(f), us[0]) <= 1e-12
+    m.set(precision="fp32")
+    assert m.tune_near(fd.ptr) == []
+
+
 def test_dense_dot_parity(model_small, xavier_w):
     m, om, f = model_small["m"], model_small["om"], model_small["f"]
     err = O.rel_l2(m.dense_dot(f), om.dense_dot(xavier_w, f))
