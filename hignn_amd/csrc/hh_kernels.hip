@@ -624,24 +624,37 @@ __global__ __launch_bounds__(kNearHThreads) void aca_mlp_h_kernel(AcaMlpArgs A, 
 
 // ---- surrogate-generated ACA entries -------------------------------------
 // The tiles of blocks that passed the surrogate judge.  One wave per 16-item
-// tile; the surrogate's three layers run on v_mfma_f32_16x16x4f32 (FP32
-// products and sums: the ACA needs entries without 16-bit noise), with the
-// tile's items on the column axis and the hidden features on the row axis,
-// so each layer's output registers are the next layer's B operand:
-//   layer 1: A = [W1 | b1] rows 16t..16t+15 (lane g, p: W1[16t+p][g], b1 for
-//            g = 3), B = (z, 1) of item p; D lane (g, p) register r =
-//            feature 16t + 4g + r, t = 0, 1;
-//   layer 2: K-step (t, r) takes B = SiLU of layer-1 register r of tile t,
-//            i.e. feature 16t + 4g + r, so its A operand is W2[16mt + p][16t
-//            + 4g + r]; the bias enters as C;
-//   layer 3: the same with W3 (rows 6..15 zero); output 4g + r of item p
-//            lands in lane (g, p), register r.
-// Each lane gathers its 26 weight fragments and 12 biases from the
-// parameters once.  Y receives the symmetric block s(r)/|r| in the full
-// MLP's row-major layout (the step kernels' symmetrisation leaves it
-// unchanged).  Same displacement convention as aca_mlp_body: x_j - x_i.
+// tile, with the tile's items on the column axis of the MFMAs and the hidden
+// features on the row axis, so each layer's output registers are the next
+// layer's B operand:
+//   layer 1 (K = 3 inputs + bias) on v_mfma_f32_16x16x4f32: A = [W1 | b1]
+//            rows 16t..16t+15 (lane g, p: W1[16t+p][g], b1 for g = 3), B =
+//            (z, 1) of item p; D lane (g, p) register r = feature 16t + 4g +
+//            r, t = 0, 1;
+//   layers 2 and 3 (K = 32) on v_mfma_f32_16x16x32_f16: lane (g, p) holds
+//            K-slots 8g + j, j = 0..7, which are the features f(g, j) = 4g + j
+//            (j < 4) and 16 + 4g + j - 4 (j >= 4) -- exactly the eight
+//            activations the lane holds after the layer before -- so A =
+//            W[16mt + p][f(g, j)] (W3 rows 6..15 zero) and the bias enters as C.
+// The ACA needs entries without 16-bit noise, so the 16-bit operands are
+// split: w = w_hi + w_lo and a = a_hi + a_lo with w_hi = fp16(w), w_lo =
+// fp16(w - w_hi), and each product is w_hi a_hi + w_hi a_lo + w_lo a_hi in
+// FP32 accumulation (three MFMAs): ~2^-22 of each term (FP16 subnormals are
+// kept), about a quarter of the MFMA cycles of the FP32 instruction.  Each
+// lane gathers and splits its weight fragments and biases once.  Y receives
+// the symmetric block s(r)/|r| in the full MLP's row-major layout (the step
+// kernels' symmetrisation leaves it unchanged).  Same displacement
+// convention as aca_mlp_body: x_j - x_i.
+__device__ __forceinline__ int sur_feature(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + j - 4; }
+
+__device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo) {
+  hi = (_Float16)v;
+  lo = (_Float16)(v - (float)hi);
+}
+
 struct SurMfma {
-  float w1[2], w2[2][2][4], w3[2][4];
+  float w1[2];
+  f16x8 w2h[2], w2l[2], w3h, w3l;
   f32x4 b2v[2], b3v;
 
   // The lane's weight fragments and biases, gathered from the parameters.
@@ -653,19 +666,37 @@ struct SurMfma {
 #pragma unroll
     for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-      for (int t = 0; t < 2; t++)
+      for (int j = 0; j < 8; j++) {
+        _Float16 hi, lo;
+        split16(params[kOW2 + (16 * mt + p) * kH + sur_feature(g, j)], hi, lo);
+        w2h[mt][j] = hi;
+        w2l[mt][j] = lo;
+      }
 #pragma unroll
-        for (int r = 0; r < 4; r++) w2[mt][t][r] = params[kOW2 + (16 * mt + p) * kH + 16 * t + 4 * g + r];
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) w3[t][r] = p < kOut ? params[kOW3 + p * kH + 16 * t + 4 * g + r] : 0.0f;
+    for (int j = 0; j < 8; j++) {
+      _Float16 hi, lo;
+      split16(p < kOut ? params[kOW3 + p * kH + sur_feature(g, j)] : 0.0f, hi, lo);
+      w3h[j] = hi;
+      w3l[j] = lo;
+    }
 #pragma unroll
     for (int mt = 0; mt < 2; mt++)
       b2v[mt] = f32x4{params[kOB2 + 16 * mt + 4 * g], params[kOB2 + 16 * mt + 4 * g + 1],
                       params[kOB2 + 16 * mt + 4 * g + 2], params[kOB2 + 16 * mt + 4 * g + 3]};
 #pragma unroll
     for (int r = 0; r < 4; r++) b3v[r] = 4 * g + r < kOut ? params[kOB3 + 4 * g + r] : 0.0f;
+  }
+
+  // The lane's eight activations (two output registers of four of the layer
+  // before) as split 16-bit B operands: K-slot 8g + j <- a[j / 4][j % 4].
+  __device__ __forceinline__ static void operands(const f32x4 (&a)[2], f16x8 &hi, f16x8 &lo) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      _Float16 h, l;
+      split16(a[j / 4][j % 4], h, l);
+      hi[j] = h;
+      lo[j] = l;
+    }
   }
 
   // Outputs 4g + r (r = 0..3) of the tile's item p; bz is z_g of item p
@@ -678,23 +709,23 @@ struct SurMfma {
 #pragma unroll
       for (int r = 0; r < 4; r++) a1[t][r] = sur::silu(a1[t][r]);
     }
+    f16x8 h1, l1;
+    operands(a1, h1, l1);
     f32x4 a2[2];
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) {
-      f32x4 c = b2v[mt];
-#pragma unroll
-      for (int t = 0; t < 2; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) c = mma(w2[mt][t][r], a1[t][r], c);
+      f32x4 c = mma(w2h[mt], h1, b2v[mt]);
+      c = mma(w2h[mt], l1, c);
+      c = mma(w2l[mt], h1, c);
 #pragma unroll
       for (int r = 0; r < 4; r++) c[r] = sur::silu(c[r]);
       a2[mt] = c;
     }
-    f32x4 y = b3v;
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) y = mma(w3[t][r], a2[t][r], y);
+    f16x8 h2, l2;
+    operands(a2, h2, l2);
+    f32x4 y = mma(w3h, h2, b3v);
+    y = mma(w3h, l2, y);
+    y = mma(w3l, h2, y);
     return y;
   }
 };
