@@ -1587,32 +1587,48 @@ __device__ __forceinline__ void apply_reduce_chunk(const AcaApplyArgs &A, const 
   // ten rank indices (30 sums, padded to 32) per wave reduction; each rank
   // index's three partial sums go to LDS as they are formed
   const int lane = threadIdx.x & 63;
+  typedef typename std::conditional<sizeof(S) == 8, double, float>::type F;  // exact
   for (int l0 = 0; l0 < K; l0 += 10) {
     const int nl = min(10, K - l0);
+    // two rank indices at a time: both factors' loads are in flight before
+    // either is used (an item past the chunk, or a rank past nl, reads the
+    // chunk's first item of factor 0 and adds nothing)
 #pragma unroll 1
-    for (int c = 0; c < 10; c++) {
-      double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-      if (c < nl) {  // uniform
+    for (int c0 = 0; c0 < 10; c0 += 2) {
+      F qv[2][kAcaIpt][9];
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++)
 #pragma unroll
         for (int s = 0; s < kAcaIpt; s++) {
           const int it = ch.off + threadIdx.x + 64 * s;
-          if (it >= send) continue;
-          typename std::conditional<sizeof(S) == 8, double, float>::type q[9];  // exact
-          PoolAt<S>::load(Src, item0 + (int64_t)(l0 + c) * ns + it, q);
-          if (SIDE == 0) {
-            x0 += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
-            x1 += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
-            x2 += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
-          } else {
-            x0 += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
-            x1 += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
-            x2 += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+          const bool ok = it < send && c0 + cc < nl;
+          PoolAt<S>::load(Src, item0 + (ok ? (int64_t)(l0 + c0 + cc) * ns + it : (int64_t)ch.off), qv[cc][s]);
+        }
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        const int c = c0 + cc;
+        double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+        if (c < nl) {  // uniform
+#pragma unroll
+          for (int s = 0; s < kAcaIpt; s++) {
+            const int it = ch.off + threadIdx.x + 64 * s;
+            if (it >= send) continue;
+            const F(&q)[9] = qv[cc][s];
+            if (SIDE == 0) {
+              x0 += (double)q[0] * f[s].x + (double)q[1] * f[s].y + (double)q[2] * f[s].z;
+              x1 += (double)q[3] * f[s].x + (double)q[4] * f[s].y + (double)q[5] * f[s].z;
+              x2 += (double)q[6] * f[s].x + (double)q[7] * f[s].y + (double)q[8] * f[s].z;
+            } else {
+              x0 += (double)q[0] * f[s].x + (double)q[3] * f[s].y + (double)q[6] * f[s].z;
+              x1 += (double)q[1] * f[s].x + (double)q[4] * f[s].y + (double)q[7] * f[s].z;
+              x2 += (double)q[2] * f[s].x + (double)q[5] * f[s].y + (double)q[8] * f[s].z;
+            }
           }
         }
+        sh[(3 * c) * 65 + lane] = x0;
+        sh[(3 * c + 1) * 65 + lane] = x1;
+        sh[(3 * c + 2) * 65 + lane] = x2;
       }
-      sh[(3 * c) * 65 + lane] = x0;
-      sh[(3 * c + 1) * 65 + lane] = x1;
-      sh[(3 * c + 2) * 65 + lane] = x2;
     }
     sh[30 * 65 + lane] = 0.0;
     sh[31 * 65 + lane] = 0.0;
@@ -1655,21 +1671,35 @@ __device__ __forceinline__ void apply_update_chunk(const AcaApplyArgs &A, const 
   else
     Dst = Qp, item0 = fb.qoff;
   const int dend = min(ch.off + A.chunk, nd);
+  typedef typename std::conditional<sizeof(S) == 8, double, float>::type F;  // exact
+  auto add = [&](const F (&c)[9], int l, double &o0, double &o1, double &o2) {
+    const double x0 = wsh[3 * l], x1 = wsh[3 * l + 1], x2 = wsh[3 * l + 2];
+    if (SIDE == 0) {
+      o0 += (double)c[0] * x0 + (double)c[1] * x1 + (double)c[2] * x2;
+      o1 += (double)c[3] * x0 + (double)c[4] * x1 + (double)c[5] * x2;
+      o2 += (double)c[6] * x0 + (double)c[7] * x1 + (double)c[8] * x2;
+    } else {
+      o0 += (double)c[0] * x0 + (double)c[3] * x1 + (double)c[6] * x2;
+      o1 += (double)c[1] * x0 + (double)c[4] * x1 + (double)c[7] * x2;
+      o2 += (double)c[2] * x0 + (double)c[5] * x1 + (double)c[8] * x2;
+    }
+  };
   for (int d = ch.off + threadIdx.x; d < dend; d += blockDim.x) {
     double o0 = 0, o1 = 0, o2 = 0;
-    for (int l = 0; l < K; l++) {
-      typename std::conditional<sizeof(S) == 8, double, float>::type c[9];  // exact
+    int l = 0;
+    // four factors' loads in flight before the first is used; the sums keep
+    // the order l = 0, 1, 2, ...
+    for (; l + 4 <= K; l += 4) {
+      F c[4][9];
+#pragma unroll
+      for (int q = 0; q < 4; q++) PoolAt<S>::load(Dst, item0 + (int64_t)(l + q) * nd + d, c[q]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) add(c[q], l + q, o0, o1, o2);
+    }
+    for (; l < K; l++) {
+      F c[9];
       PoolAt<S>::load(Dst, item0 + (int64_t)l * nd + d, c);
-      const double x0 = wsh[3 * l], x1 = wsh[3 * l + 1], x2 = wsh[3 * l + 2];
-      if (SIDE == 0) {
-        o0 += (double)c[0] * x0 + (double)c[1] * x1 + (double)c[2] * x2;
-        o1 += (double)c[3] * x0 + (double)c[4] * x1 + (double)c[5] * x2;
-        o2 += (double)c[6] * x0 + (double)c[7] * x1 + (double)c[8] * x2;
-      } else {
-        o0 += (double)c[0] * x0 + (double)c[3] * x1 + (double)c[6] * x2;
-        o1 += (double)c[1] * x0 + (double)c[4] * x1 + (double)c[7] * x2;
-        o2 += (double)c[2] * x0 + (double)c[5] * x1 + (double)c[8] * x2;
-      }
+      add(c, l, o0, o1, o2);
     }
     double *u = A.u + 3 * (size_t)(d0 + d);
     unsafeAtomicAdd(u + 0, o0);
