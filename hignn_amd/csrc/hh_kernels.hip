@@ -1175,8 +1175,15 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++)
 #pragma unroll
-      for (int c = 0; c < kCrossLC; c++)
-        if (c < nl) PoolAt<S>::load(pool, (int64_t)(l0 + c) * stride + off[s], vv[s][c]);
+      for (int c = 0; c < kCrossLC; c++) {
+        if (c >= nl) continue;
+        if (ncross > 0 && l0 + c == k - 1) {  // uniform: factor k-1 is already in a
+#pragma unroll
+          for (int e = 0; e < 9; e++) vv[s][c][e] = a[s][e];
+        } else {
+          PoolAt<S>::load(pool, (int64_t)(l0 + c) * stride + off[s], vv[s][c]);
+        }
+      }
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++) {
 #pragma unroll
