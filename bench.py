@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size before the timed region "
                          "(known-size reference for HBM byte counters)")
+    ap.add_argument("--surrogate-file", default="",
+                    help="matvec bench: load the surrogate's parameters from this .npy if it "
+                         "exists, else train and save them there")
     ap.add_argument("--no-tune-near", action="store_true",
                     help="matvec bench: keep the default tile shape of the 16-bit near kernel")
     ap.add_argument("--tune-far", action="store_true",
@@ -517,7 +520,16 @@ def matvec_bench(args):
     # by the library's own kernels on samples labelled by the full MLP (setup,
     # once per pair function; not part of a matvec)
     sur, sur_info = None, None
-    if not args.no_surrogate:
+    if not args.no_surrogate and args.surrogate_file and os.path.exists(args.surrogate_file):
+        # parameters of an earlier run of this benchmark (profiling runs skip
+        # the training's dispatches)
+        sur = abi.Surrogate(args.sur_rlo, args.sur_rhi)
+        sur.set_params(np.load(args.surrogate_file, allow_pickle=False))
+        sur_info = {"net": "3-32-32-6 SiLU", "loaded_from": args.surrogate_file,
+                    "r_range": [args.sur_rlo, args.sur_rhi], "judge_tol": args.sur_tol,
+                    "judge_samples": 32}
+        log(f"[bench] surrogate: {sur_info}")
+    elif not args.no_surrogate:
         t0 = time.time()
         r_s, t_s = model.surrogate_samples(args.sur_samples, args.sur_rlo, args.sur_rhi, seed=1)
         sur = abi.Surrogate(args.sur_rlo, args.sur_rhi)
@@ -533,6 +545,8 @@ def matvec_bench(args):
                     "train_s": round(time.time() - t0, 2), "judge_tol": args.sur_tol,
                     "judge_samples": 32}
         log(f"[bench] surrogate: {sur_info}")
+        if args.surrogate_file:
+            np.save(args.surrogate_file, sur.params())
 
     def run(prec, use_sur):
         model.set(precision=prec)
@@ -560,7 +574,7 @@ def matvec_bench(args):
     model.set(precision=args.prec)
     model.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
     near_tuning = model.tune_near(f_dev.ptr) if not args.no_tune_near else []
-    log(f"[bench] near-kernel tile shapes (shape, tiles, threads, ms): {near_tuning}")
+    log(f"[bench] near-kernel tile shapes (shape, tiles, threads, matvec ms): {near_tuning}")
     far_tuning = model.tune_far_field(f_dev.ptr) if args.tune_far else []
     if far_tuning:
         log(f"[bench] far-field launch choices (mode, chunk, matvec ms): {far_tuning}")

@@ -240,7 +240,7 @@ class Model:
         return [(int(a), int(b), float(c)) for a, b, c in t[:rows.value]]
 
     def tune_near(self, f_ptr, stream=None):
-        """Rows (shape, tiles, threads, near-field ms) of hh_model_tune_near;
+        """Rows (shape, tiles, threads, matvec ms) of hh_model_tune_near;
         the model keeps the fastest 16-bit near-kernel tile shape."""
         t = np.zeros((8, 4), np.float64)
         rows = ctypes.c_int(0)

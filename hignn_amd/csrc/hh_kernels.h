@@ -228,6 +228,7 @@ struct AcaPlanLaunch {
   const int32_t *act_in;   // previous active list; NULL: blocks 0..nb-1
   const int32_t *stop_in;  // per previous slot; NULL: none stopped
   const AcaCounts *cnt_in; // previous counts; NULL: first plan (nb blocks, zero totals)
+  int select_kind;         // first plan: only blocks of this kind (-1: all blocks)
   int nb;
   int chunk;
   int32_t *act_out;
@@ -244,8 +245,10 @@ hipError_t launch_aca_init(const FarBlock *blk, int nb, int max_iter, int32_t *i
                            double *mu2, double *nu2, hipStream_t st);
 // acc[0..1] += cnt->entries[0..1] (entries per kind of a finished batch).
 hipError_t launch_aca_entries(const AcaCounts *cnt, int64_t *acc, hipStream_t st);
-// Final ranks of a batch into the per-far-block rank array.
-hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, hipStream_t st);
+// Final ranks of a batch into the per-far-block rank array; with kind, only
+// the blocks of kind select_kind.
+hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, const int32_t *kind,
+                             int select_kind, hipStream_t st);
 hipError_t launch_aca_expand(const AcaExpandLaunch &L, hipStream_t st);
 hipError_t launch_aca_mlp(const AcaMlpLaunch &L, hipStream_t st);
 // which = 0: column residual, row pivot, column normalisation;

@@ -1487,7 +1487,7 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 // the hard stop tests of iteration k (src/FarDot.cpp:1002-1108).  mu2[b]
 // holds mu2 of iteration k-1 without its cross terms, nu2[b] its nu2; it[b]
 // ends as the block's rank when it stops.
-__device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, double *cs, double *qs,
+__device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, double *pr,
                                               ArgMax *sh_am) {
   const int b = A.active[slot];
   const FarBlock fb = A.blk[b];
@@ -1503,15 +1503,14 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
     double sc = 0.0, sq = 0.0;
     for (int c = c0; c < c1; c++) sc += A.x_c[(size_t)c * A.max_iter * 9 + t];
     for (int c = r0; c < r1; c++) sq += A.x_r[(size_t)c * A.max_iter * 9 + t];
-    cs[t] = sc;
-    qs[t] = sq;
+    pr[t] = 2.0 * fabs(sc) * fabs(sq);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double mu2 = A.mu2[b];
     bool soft = false;
     if (k >= 1) {
-      for (int t = 0; t < ncross; t++) mu2 += 2.0 * fabs(cs[t]) * fabs(qs[t]);
+      for (int t = 0; t < ncross; t++) mu2 += pr[t];  // the reference's order (l, d1, d2)
       soft = A.nu2[b] < mu2 * A.eps2;
     }
     if (soft) {  // iteration k-1 met the tolerance: rank k
@@ -1534,14 +1533,14 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
       A.stop[slot] = (itn >= A.max_iter || itn >= mn) ? 1 : 0;
     }
   }
-  __syncthreads();  // cs / qs / sh_am are reused by the next slot
+  __syncthreads();  // pr / sh_am are reused by the next slot
 }
 
 __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
-  __shared__ double cs[126 * 9], qs[126 * 9];
+  __shared__ double pr[126 * 9];
   __shared__ ArgMax sh_am[4];
   const int na = A.cnt->na;
-  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, cs, qs, sh_am);
+  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, pr, sh_am);
 }
 
 // ---- chunked apply of stopped blocks (src/FarDot.cpp:1200-1388) -----------
@@ -1591,17 +1590,17 @@ __device__ __forceinline__ void apply_reduce_chunk(const AcaApplyArgs &A, const 
     const int it = ch.off + threadIdx.x + 64 * s;
     f[s] = it < send ? A.fp[s0 + it] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  // ten rank indices (30 sums, padded to 32) per wave reduction; each rank
-  // index's three partial sums go to LDS as they are formed
+  // eight rank indices (24 sums) per wave reduction; each rank index's three
+  // partial sums go to LDS as they are formed
   const int lane = threadIdx.x & 63;
   typedef typename std::conditional<sizeof(S) == 8, double, float>::type F;  // exact
-  for (int l0 = 0; l0 < K; l0 += 10) {
-    const int nl = min(10, K - l0);
+  for (int l0 = 0; l0 < K; l0 += 8) {
+    const int nl = min(8, K - l0);
     // two rank indices at a time: both factors' loads are in flight before
     // either is used (an item past the chunk, or a rank past nl, reads the
     // chunk's first item of factor 0 and adds nothing)
 #pragma unroll 1
-    for (int c0 = 0; c0 < 10; c0 += 2) {
+    for (int c0 = 0; c0 < 8; c0 += 2) {
       F qv[2][kAcaIpt][9];
 #pragma unroll
       for (int cc = 0; cc < 2; cc++)
@@ -1637,16 +1636,14 @@ __device__ __forceinline__ void apply_reduce_chunk(const AcaApplyArgs &A, const 
         sh[(3 * c + 2) * 65 + lane] = x2;
       }
     }
-    sh[30 * 65 + lane] = 0.0;
-    sh[31 * 65 + lane] = 0.0;
-    wave_sum_lds<double, 32>(sh, W + 3 * l0, 3 * nl);
+    wave_sum_lds<double, 24>(sh, W + 3 * l0, 3 * nl);
   }
 }
 
 template <class SC, class SQ, int SIDE>
 __global__ __launch_bounds__(64) void aca_apply_reduce_kernel(AcaApplyArgs A, const SC *Cp,
                                                               const SQ *Qp) {
-  __shared__ double sh[32 * 65];
+  __shared__ double sh[24 * 65];
   const int nl = A.cnt->nl, nc = SIDE == 0 ? A.cnt->ncj : A.cnt->nci;
   for (int q = blockIdx.x; q < nc; q += gridDim.x) apply_reduce_chunk<SC, SQ, SIDE>(A, Cp, Qp, q, nl, sh);
 }
@@ -1735,11 +1732,14 @@ __device__ __forceinline__ void plan_values(const AcaPlanLaunch &L, int e, int64
   b = L.act_in ? L.act_in[e] : e;
   keep = !(L.stop_in && L.stop_in[e]);
   full = !(L.kind && L.kind[b]);
+  // first plan of a loop over one kind: the other blocks are neither kept nor
+  // applied
+  const bool skip = !L.act_in && L.select_kind >= 0 && (full ? 0 : 1) != L.select_kind;
   const FarBlock fb = L.blk[b];
   const int64_t m = fb.m, n = fb.n;
   const int64_t tm = (m + 15) / 16, tn = (n + 15) / 16;
   const int64_t cm = (m + L.chunk - 1) / L.chunk, cn = (n + L.chunk - 1) / L.chunk;
-  const int64_t k = keep ? 1 : 0, d = keep ? 0 : 1, f = full ? 1 : 0;
+  const int64_t k = (keep && !skip) ? 1 : 0, d = (keep || skip) ? 0 : 1, f = full ? 1 : 0;
   v[0] = k;
   v[1] = k * m;
   v[2] = k * n;
@@ -1839,14 +1839,39 @@ __global__ __launch_bounds__(kPlanGroups) void aca_plan_scan_kernel(AcaPlanLaunc
 
 // The group's range in order, 256 elements per round: block-wide exclusive
 // scan of the placing values, plus the group's offset and the rounds before.
+// The two Y-row sums are 64-bit; the other placing fields (slots, tiles,
+// chunks; all below 2^31, checked when the batches are laid out) 32-bit.
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t t, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = __shfl_up(t, o, 64);
+    if (lane >= o) t += y;
+  }
+  return t;
+}
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t t, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(t, o, 64);
+    if (lane >= o) t += y;
+  }
+  return t;
+}
+
 __global__ __launch_bounds__(256) void aca_plan_write_kernel(AcaPlanLaunch L) {
-  constexpr int F = 12;  // fields that place an element
-  __shared__ int64_t wsum[4][F];
+  constexpr int F = 12;  // fields that place an element: 1 and 2 64-bit, the rest 32-bit
+  __shared__ int64_t wsum64[4][2];
+  __shared__ int32_t wsum32[4][F];
   int e0, e1;
   plan_range(L, e0, e1);
-  int64_t carry[F];
+  int64_t carry64[2];
+  int32_t carry32[F];
 #pragma unroll
-  for (int f = 0; f < F; f++) carry[f] = L.part[(size_t)blockIdx.x * kPlanFields + f];
+  for (int f = 0; f < F; f++) {
+    const int64_t c = L.part[(size_t)blockIdx.x * kPlanFields + f];
+    if (f == 1 || f == 2) carry64[f - 1] = c;
+    else carry32[f] = (int32_t)c;
+  }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int r0 = e0; r0 < e1; r0 += blockDim.x) {  // uniform
     const int e = r0 + (int)threadIdx.x;
@@ -1859,43 +1884,52 @@ __global__ __launch_bounds__(256) void aca_plan_write_kernel(AcaPlanLaunch L) {
 #pragma unroll
       for (int f = 0; f < kPlanFields; f++) v[f] = 0;
     }
-    int64_t incl[F];
+    int64_t pre64[2];
+    int32_t pre32[F];
 #pragma unroll
     for (int f = 0; f < F; f++) {
-      int64_t t = v[f];
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int64_t y = __shfl_up(t, o, 64);
-        if (lane >= o) t += y;
+      if (f == 1 || f == 2) {
+        const int64_t t = wave_incl_scan(v[f], lane);
+        pre64[f - 1] = t - v[f];
+        if (lane == 63) wsum64[wv][f - 1] = t;
+      } else {
+        const int32_t x = (int32_t)v[f];
+        const int32_t t = wave_incl_scan(x, lane);
+        pre32[f] = t - x;
+        if (lane == 63) wsum32[wv][f] = t;
       }
-      incl[f] = t;
-      if (lane == 63) wsum[wv][f] = t;
     }
     __syncthreads();
-    int64_t pre[F];
 #pragma unroll
     for (int f = 0; f < F; f++) {
-      int64_t w = 0;
-      for (int q = 0; q < wv; q++) w += wsum[q][f];
-      pre[f] = carry[f] + w + incl[f] - v[f];
-      carry[f] += wsum[0][f] + wsum[1][f] + wsum[2][f] + wsum[3][f];
+      if (f == 1 || f == 2) {
+        int64_t w = 0;
+        for (int q = 0; q < wv; q++) w += wsum64[q][f - 1];
+        pre64[f - 1] += carry64[f - 1] + w;
+        carry64[f - 1] += wsum64[0][f - 1] + wsum64[1][f - 1] + wsum64[2][f - 1] + wsum64[3][f - 1];
+      } else {
+        int32_t w = 0;
+        for (int q = 0; q < wv; q++) w += wsum32[q][f];
+        pre32[f] += carry32[f] + w;
+        carry32[f] += wsum32[0][f] + wsum32[1][f] + wsum32[2][f] + wsum32[3][f];
+      }
     }
     __syncthreads();  // wsum is rewritten by the next round
-    if (e < e1) {
+    if (e < e1 && (v[0] | v[9])) {  // kept or stopped (not skipped)
       if (keep) {
-        const int s = (int)pre[0];
+        const int s = pre32[0];
         L.act_out[s] = b;
-        L.ybase_c[s] = pre[1];
-        L.ybase_r[s] = pre[2];
-        L.toff_c[s] = (int32_t)(full ? pre[3] : pre[4]);
-        L.toff_r[s] = (int32_t)(full ? pre[5] : pre[6]);
-        L.cfirst_c[s] = (int32_t)pre[7];
-        L.cfirst_r[s] = (int32_t)pre[8];
+        L.ybase_c[s] = pre64[0];
+        L.ybase_r[s] = pre64[1];
+        L.toff_c[s] = full ? pre32[3] : pre32[4];
+        L.toff_r[s] = full ? pre32[5] : pre32[6];
+        L.cfirst_c[s] = pre32[7];
+        L.cfirst_r[s] = pre32[8];
       } else {
-        const int a = (int)pre[9];
+        const int a = pre32[9];
         L.alist[a] = b;
-        L.first_i[a] = (int32_t)pre[10];
-        L.first_j[a] = (int32_t)pre[11];
+        L.first_i[a] = pre32[10];
+        L.first_j[a] = pre32[11];
       }
     }
   }
@@ -1915,9 +1949,10 @@ __global__ void aca_entries_kernel(const AcaCounts *cnt, int64_t *acc) {
   if (threadIdx.x < 2) acc[threadIdx.x] += cnt->entries[threadIdx.x];
 }
 
-__global__ void copy_ranks_kernel(const int32_t *it, int nb, int32_t *rank_out) {
+__global__ void copy_ranks_kernel(const int32_t *it, int nb, int32_t *rank_out, const int32_t *kind,
+                                  int select_kind) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) rank_out[b] = it[b];
+  if (b < nb && (!kind || (kind[b] ? 1 : 0) == select_kind)) rank_out[b] = it[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -2471,10 +2506,11 @@ hipError_t launch_aca_entries(const AcaCounts *cnt, int64_t *acc, hipStream_t st
   return hipGetLastError();
 }
 
-hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, hipStream_t st) {
+hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, const int32_t *kind,
+                             int select_kind, hipStream_t st) {
   if (nb <= 0) return hipSuccess;
   hipLaunchKernelGGL(copy_ranks_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, it, nb,
-                     rank_out);
+                     rank_out, kind, select_kind);
   return hipGetLastError();
 }
 

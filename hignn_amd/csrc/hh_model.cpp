@@ -836,9 +836,11 @@ struct hh_model {
   // tile lists, the full-MLP entries, the surrogate entries, the steps.
   // hc: the host's copy of the counts (host control: exact grids), or NULL
   // (device control: grids from the batch bounds, at most grid_cap).
+  // select_kind: 1 (only blocks on surrogate entries: no full-MLP launch), 0
+  // (only full-MLP blocks: no surrogate launch), -1 (both kinds).
   void aca_passes(const FarBatch &B, const FarBlock *blk, const int32_t *kind, const AcaCounts *cnt,
                   const AcaCounts *hc, const int32_t *active, int64_t grid_cap, hipStream_t st,
-                  bool timed) {
+                  bool timed, int select_kind = -1) {
     AcaStepLaunch sl = step_launch_for(B, blk, cnt, active, grid_cap);
     if (hc) sl.nactive = hc->na, sl.nchunks_c = hc->ncc, sl.nchunks_r = hc->ncr;
     for (int pass = 0; pass < 2; pass++) {
@@ -871,7 +873,8 @@ struct hh_model {
       ml.max_blocks = 8 * g_cus;
       ml.cus = g_cus;
       hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (timed) {
+      const bool mlp = select_kind != 1;
+      if (timed && mlp) {
         while (evpool.size() < ev_used + 2) {
           hipEvent_t e;
           HIPCHK(hipEventCreate(&e));
@@ -881,9 +884,9 @@ struct hh_model {
         ev_used += 2;
         HIPCHK(hipEventRecord(e0, st));
       }
-      HIPCHK(launch_aca_mlp(ml, st));
-      if (timed) HIPCHK(hipEventRecord(e1, st));
-      if (sur && kind) {
+      if (mlp) HIPCHK(launch_aca_mlp(ml, st));
+      if (timed && mlp) HIPCHK(hipEventRecord(e1, st));
+      if (sur && kind && select_kind != 0) {
         AcaSurLaunch su{};
         su.xp = xp.as<float4>();
         su.tiles = ex.tiles_sur;
@@ -1004,7 +1007,7 @@ struct hh_model {
   PBuf h_na;
   hipEvent_t ev_na[kRing] = {};
   void batch_device_control(const FarBatch &B, const FarBlock *blk, const int32_t *kind,
-                            hipStream_t st, bool timed, bool early_exit) {
+                            hipStream_t st, bool timed, bool early_exit, int select_kind) {
     if (early_exit && !ev_na[0]) {
       for (auto &e : ev_na) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       h_na.reserve(kRing * sizeof(int32_t));
@@ -1026,6 +1029,7 @@ struct hh_model {
     pl.first_i = a_first_i.as<int32_t>();
     pl.first_j = a_first_j.as<int32_t>();
     pl.part = plan_part.as<int64_t>();
+    pl.select_kind = kind ? select_kind : -1;
     pl.act_in = nullptr, pl.stop_in = nullptr, pl.cnt_in = nullptr;
     pl.act_out = act2[0].as<int32_t>();
     pl.cnt_out = cnt2[0].as<AcaCounts>();
@@ -1033,7 +1037,8 @@ struct hh_model {
     int cur = 0;
     for (int iter = 0; iter < max_iter; iter++) {
       const AcaCounts *c = cnt2[cur].as<AcaCounts>();
-      aca_passes(B, blk, kind, c, nullptr, act2[cur].as<int32_t>(), cap, st, timed);
+      aca_passes(B, blk, kind, c, nullptr, act2[cur].as<int32_t>(), cap, st, timed,
+                 kind ? select_kind : -1);
       const int nxt = cur ^ 1;
       pl.act_in = act2[cur].as<int32_t>();
       pl.stop_in = stop.as<int32_t>();
@@ -1058,18 +1063,22 @@ struct hh_model {
   }
 
   // Batched far field: judge, then per batch the iterations under device or
-  // host control; ranks into rank_dev (read when asked for).
-  void far_field(hipStream_t st, bool timed = true) {
-    if (resolved_far_mode() == 0) {
-      far_field_persistent(st);
-      return;
-    }
-    const int64_t nfar = (int64_t)far.size();
-    if (nfar == 0) return;
+  // host control; ranks into rank_dev (read when asked for).  With a
+  // surrogate under device control the iterations can run as two loops per
+  // batch (far_loops with select_kind 1, then 0): blocks on surrogate entries
+  // need no LDS-resident MLP kernel and run beside the near field; the
+  // judge-rejected blocks follow once the near field is done.
+  bool split_far() const { return sur && aca_control == 1; }
+
+  void far_setup(hipStream_t st) {
     prepare_far_batches();
     judge_surrogate(st);
     HIPCHK(launch_zero_f64(ent_acc.as<double>(), 2, st));  // two int64 zeros
     ev_used = 0;
+  }
+
+  void far_loops(hipStream_t st, bool timed, int select_kind) {
+    const int64_t nfar = (int64_t)far.size();
     const bool host_ctl = aca_control == 0;
     std::vector<int32_t> hkind;
     std::vector<FarBlock> hb;
@@ -1085,8 +1094,9 @@ struct hh_model {
     for (const FarBatch &B : batches) {
       const int nb = (int)(B.b1 - B.b0);
       if (verbose) {
-        std::fprintf(stderr, "[hh] far blocks %lld..%lld of %lld (%s control)\n", (long long)B.b0,
-                     (long long)B.b1, (long long)nfar, host_ctl ? "host" : "device");
+        std::fprintf(stderr, "[hh] far blocks %lld..%lld of %lld (%s control, kind %d)\n",
+                     (long long)B.b0, (long long)B.b1, (long long)nfar, host_ctl ? "host" : "device",
+                     select_kind);
         std::fflush(stderr);
       }
       const FarBlock *blk = fb_all.as<FarBlock>() + B.b0;
@@ -1099,13 +1109,33 @@ struct hh_model {
         if (!hkind.empty()) hk.assign(hkind.begin() + B.b0, hkind.begin() + B.b1);
         batch_host_control(B, blk, hbb, kind, hk, st);
       } else {
-        batch_device_control(B, blk, kind, st, timed, timed);
+        batch_device_control(B, blk, kind, st, timed, timed, select_kind);
       }
-      HIPCHK(launch_copy_ranks(it.as<int32_t>(), nb, rank_dev.as<int32_t>() + B.b0, st));
+      HIPCHK(launch_copy_ranks(it.as<int32_t>(), nb, rank_dev.as<int32_t>() + B.b0,
+                               kind && select_kind >= 0 ? kind : nullptr, select_kind, st));
     }
+  }
+
+  void far_finish() {
     if (sur) judge_pending = true;
     rank_src = rank_dev.as<int32_t>();
     rank_pending = true;
+  }
+
+  void far_field(hipStream_t st, bool timed = true) {
+    if (resolved_far_mode() == 0) {
+      far_field_persistent(st);
+      return;
+    }
+    if (far.empty()) return;
+    far_setup(st);
+    if (split_far()) {
+      far_loops(st, timed, 1);
+      far_loops(st, timed, 0);
+    } else {
+      far_loops(st, timed, -1);
+    }
+    far_finish();
   }
 
   // ACA pass times (full-MLP entries) of the last batched far field, after
@@ -1171,19 +1201,30 @@ struct hh_model {
   // batched far field's per-iteration host waits are on st only, so the
   // near kernel keeps running through them.
   void near_beside_far(hipStream_t st, bool timed, bool persistent) {
+    // the judge first (its FP32 MLP would only compete with the near field)
+    const bool split = !persistent && split_far() && !far.empty();
+    if (timed) HIPCHK(hipEventRecord(far0_ev, st));
+    if (split) far_setup(st);
     HIPCHK(hipEventRecord(fork_ev, st));
     HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
     if (timed) HIPCHK(hipEventRecord(ev[0], side));
     near_field(side, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
     if (timed) HIPCHK(hipEventRecord(ev[1], side));
     HIPCHK(hipEventRecord(join_ev, side));
-    if (timed) HIPCHK(hipEventRecord(far0_ev, st));
-    if (persistent)
+    if (persistent) {
       far_field_persistent(st);
-    else
+    } else if (split) {
+      far_loops(st, timed, 1);  // beside the near field
+    } else {
       far_field(st, timed);
-    if (timed) HIPCHK(hipEventRecord(ev[2], st));
+    }
+    if (timed && !split) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
+    if (split) {
+      far_loops(st, timed, 0);  // judge-rejected blocks, full MLP, after the near field
+      far_finish();
+      if (timed) HIPCHK(hipEventRecord(ev[2], st));
+    }
   }
 
   int resolved_far_mode() const {
@@ -1268,30 +1309,24 @@ struct hh_model {
   }
 
   // Tile-shape tuning of the 16-bit near kernel on the caller's forces: each
-  // shape runs the near field once untimed and once timed (HIP events on
-  // st); the fastest is kept.  Rows of `out`: {shape, tiles, threads, ms}.
+  // shape runs one untimed and one timed matvec (HIP events around the whole
+  // matvec: the near field runs beside the far field, so the best shape
+  // depends on both); the fastest is kept.  Rows of `out`: {shape, tiles,
+  // threads, matvec ms}.
   void tune_near(const float *f_dev, hipStream_t st, std::vector<std::array<double, 4>> &out) {
     check_ready();
     out.clear();
     if (prec != PREC_BF16 && prec != PREC_FP16) return;  // the FP32 kernel has one shape
-    HIPCHK(launch_gather_forces(f_dev, n, perm.as<int32_t>(), fp.as<float4>(), st));
     const int h0 = near_hcfg;
     int pick = h0;
     double best = 0;
     try {
       for (int c = 0; c < kNearShapeCount; c++) {
         near_hcfg = c;
-        float ms = 0;
-        for (int rep = 0; rep < 2; rep++) {
-          HIPCHK(launch_zero_f64(u.as<double>(), 3 * n, st));
-          HIPCHK(hipEventRecord(ev[0], st));
-          near_field(st, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
-          HIPCHK(hipEventRecord(ev[1], st));
-          HIPCHK(hipEventSynchronize(ev[1]));
-          HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        }
-        out.push_back({(double)c, (double)kNearShapes[c].tiles, (double)kNearShapes[c].threads, (double)ms});
-        if (out.size() == 1 || ms < best) best = ms, pick = c;
+        for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
+        HIPCHK(hipStreamSynchronize(st));
+        out.push_back({(double)c, (double)kNearShapes[c].tiles, (double)kNearShapes[c].threads, t_total});
+        if (out.size() == 1 || t_total < best) best = t_total, pick = c;
       }
     } catch (...) {
       near_hcfg = h0;

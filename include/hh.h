@@ -182,10 +182,10 @@ int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *st
  * {4, 512}, {2, 512}, {2, 1024}, {4, 256}; default 0 (HH_NEAR_SHAPE sets the
  * default of new models).  hh_model_tune_near times every shape on the
  * caller's forces (device array f_dev, (n,3) float32; one untimed and one
- * timed near field each) and keeps the fastest; table receives up to
- * max_rows rows of {shape, tiles, threads, near-field ms}, *rows their
- * number (0 in the FP32 mode, whose kernel has one shape).  Every shape
- * computes the same sums up to rounding.  No reference counterpart: the
+ * timed matvec each, since the near field shares the GPU with the far field)
+ * and keeps the fastest; table receives up to max_rows rows of {shape,
+ * tiles, threads, matvec ms}, *rows their number (0 in the FP32 mode, whose
+ * kernel has one shape).  Every shape computes the same sums up to rounding.  No reference counterpart: the
  * reference's close-field launch is fixed (src/CloseDot.cpp:25-337). */
 int hh_model_set_near_shape(hh_model *m, int shape);
 int hh_model_tune_near(hh_model *m, const float *f_dev, int64_t n, void *stream, double *table,
