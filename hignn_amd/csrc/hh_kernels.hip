@@ -1080,13 +1080,19 @@ __device__ __forceinline__ void wave_sum_lds(T *sh, double *dst, int nvalid) {
   __syncthreads();
 }
 
+// Sums each of the N values x over the 64 lanes (FP64 across lanes) and
+// stores the sums to dst[0..nvalid).  For workgroups of one wave only: the
+// wave's LDS instructions execute in order, so the exchange through sh needs
+// wave-scope ordering, not a workgroup barrier (whose release fence would
+// also wait for the wave's outstanding global loads).
 template <class T, int N>
 __device__ __forceinline__ void wave_sum_store(const T (&x)[N], T *sh, double *dst, int nvalid) {
   static_assert(N <= 32, "at most 32 values");
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int e = 0; e < N; e++) sh[e * 65 + lane] = x[e];
-  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   const int e = lane & 31, h = lane >> 5;
   double t = 0.0;
   if (e < N) {
@@ -1095,7 +1101,8 @@ __device__ __forceinline__ void wave_sum_store(const T (&x)[N], T *sh, double *d
   }
   t += __shfl_xor(t, 32, 64);
   if (lane < nvalid) dst[lane] = t;
-  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // 3x3 Gram sums of one item's block a (row-major): column side GC = a^T a,
@@ -1130,6 +1137,8 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 // whenever nu2 < (mu2_{k-1} + nu2) eps^2 already holds without the cross
 // terms (they only add to mu2).
 
+constexpr int kCrossN = 9;  // cross sums per wave reduction (one factor)
+
 // Residual sums of a lane's kAcaIpt items, sum = sum_{l<k} L_l R_l with the
 // 3x3 products of the reference's dot loops (src/FarDot.cpp:487-511, 812-836;
 // the caller subtracts them from the new column or row),
@@ -1140,12 +1149,9 @@ __device__ __forceinline__ void gram_row(const double (&q)[9], double (&g)[6]) {
 //                x[9l + 3d1 + d2] = sum_i sum_row C_{k-1}(i)[row][d1] C_l(i)[row][d2]
 //   row side:    L_l = C_l(ik) from LDS, R_l = Q_l(j) from the pool,
 //                x[9l + 3d1 + d2] = sum_j sum_col Q_{k-1}(j)[d1][col] Q_l(j)[d2][col]
-// kCrossLC iterations per wave reduction.  item0 is the lane's first item of
-// C_0 (Q_0) in the pool, factor l starts stride items later; the lane's items
-// are 64 apart.
-constexpr int kCrossLC = 1;
-constexpr int kCrossN = 9 * kCrossLC;  // cross sums per wave reduction
-
+// One wave per chunk (the kernels launch workgroups of one wave).  item0 is
+// the lane's first item of C_0 (Q_0) in the pool, factor l starts stride
+// items later; the lane's items are 64 apart.
 template <class P, class S, int SIDE>
 __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool (&valid)[kAcaIpt],
                                                const S *pool, int64_t item0, int64_t stride,
@@ -1168,67 +1174,73 @@ __device__ __forceinline__ void residual_cross(P (&sum)[kAcaIpt][9], const bool 
       a[s][e] = (valid[s] && ncross > 0) ? a[s][e] : (P)0;
     }
   }
-  for (int l0 = 0; l0 < k; l0 += kCrossLC) {
-    const int nl = min(kCrossLC, k - l0);
-    const bool cross = l0 < ncross;  // uniform
-    P x[kCrossN];
-#pragma unroll
-    for (int e = 0; e < kCrossN; e++) x[e] = (P)0;
-    P vv[kAcaIpt][kCrossLC][9];
-#pragma unroll
-    for (int s = 0; s < kAcaIpt; s++)
-#pragma unroll
-      for (int c = 0; c < kCrossLC; c++) {
-        if (c >= nl) continue;
-        if (ncross > 0 && l0 + c == k - 1) {  // uniform: factor k-1 is already in a
-#pragma unroll
-          for (int e = 0; e < 9; e++) vv[s][c][e] = a[s][e];
-        } else {
-          PoolAt<S>::load(pool, (int64_t)(l0 + c) * stride + off[s], vv[s][c]);
-        }
-      }
+  // sum += L_l R_l for the lane's items from factor w = factor l; with the
+  // cross terms of factor l against a when cross
+  auto accum = [&](const P (&w)[kAcaIpt][9], int l, bool cross, P (&x)[9]) {
 #pragma unroll
     for (int s = 0; s < kAcaIpt; s++) {
+      if (!valid[s]) continue;
+      const P *cl = SIDE == 0 ? w[s] : piv[l];
+      const P *ql = SIDE == 0 ? piv[l] : w[s];
 #pragma unroll
-      for (int c = 0; c < kCrossLC; c++) {
-        if (!valid[s] || c >= nl) continue;
-        const P(&v)[9] = vv[s][c];
-        const P *cl = SIDE == 0 ? v : piv[l0 + c];
-        const P *ql = SIDE == 0 ? piv[l0 + c] : v;
+      for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-          for (int cc = 0; cc < 3; cc++) {
-            P &t = sum[s][3 * r + cc];
-            if (sizeof(P) == 8) {  // FP64 pools: the oracle's rounding, no contraction
-              t += cl[3 * r] * ql[cc];
-              t += cl[3 * r + 1] * ql[3 + cc];
-              t += cl[3 * r + 2] * ql[6 + cc];
-            } else {
-              t = fma(cl[3 * r], ql[cc], t);
-              t = fma(cl[3 * r + 1], ql[3 + cc], t);
-              t = fma(cl[3 * r + 2], ql[6 + cc], t);
-            }
+        for (int cc = 0; cc < 3; cc++) {
+          P &t = sum[s][3 * r + cc];
+          if (sizeof(P) == 8) {  // FP64 pools: the oracle's rounding, no contraction
+            t += cl[3 * r] * ql[cc];
+            t += cl[3 * r + 1] * ql[3 + cc];
+            t += cl[3 * r + 2] * ql[6 + cc];
+          } else {
+            t = fma(cl[3 * r], ql[cc], t);
+            t = fma(cl[3 * r + 1], ql[3 + cc], t);
+            t = fma(cl[3 * r + 2], ql[6 + cc], t);
           }
-        if (!cross) continue;
+        }
+      if (!cross) continue;
 #pragma unroll
-        for (int d1 = 0; d1 < 3; d1++)
+      for (int d1 = 0; d1 < 3; d1++)
 #pragma unroll
-          for (int d2 = 0; d2 < 3; d2++) {
-            P &t = x[9 * c + 3 * d1 + d2];
-            if (SIDE == 0) {
-              t = fma(a[s][d1], v[d2], t);
-              t = fma(a[s][d1 + 3], v[d2 + 3], t);
-              t = fma(a[s][d1 + 6], v[d2 + 6], t);
-            } else {
-              t = fma(a[s][3 * d1], v[3 * d2], t);
-              t = fma(a[s][3 * d1 + 1], v[3 * d2 + 1], t);
-              t = fma(a[s][3 * d1 + 2], v[3 * d2 + 2], t);
-            }
+        for (int d2 = 0; d2 < 3; d2++) {
+          P &t = x[3 * d1 + d2];
+          if (SIDE == 0) {
+            t = fma(a[s][d1], w[s][d2], t);
+            t = fma(a[s][d1 + 3], w[s][d2 + 3], t);
+            t = fma(a[s][d1 + 6], w[s][d2 + 6], t);
+          } else {
+            t = fma(a[s][3 * d1], w[s][3 * d2], t);
+            t = fma(a[s][3 * d1 + 1], w[s][3 * d2 + 1], t);
+            t = fma(a[s][3 * d1 + 2], w[s][3 * d2 + 2], t);
           }
-      }
+        }
     }
-    if (cross) wave_sum_store<P, kCrossN>(x, sh, X + 9 * l0, 9 * min(nl, ncross - l0));
+  };
+  // Factors 0 .. kp-1 come from the pool; with cross terms factor k-1 is
+  // already in a (and has no cross terms: k-1 >= ncross).  The loads of
+  // factor l+1 go out as soon as factor l has been used, into the same
+  // registers, so that they are in flight during the wave reduction of
+  // factor l's cross terms.
+  const int kp = ncross > 0 ? k - 1 : k;
+  P v[kAcaIpt][9];
+  if (kp > 0) {
+#pragma unroll
+    for (int s = 0; s < kAcaIpt; s++) PoolAt<S>::load(pool, off[s], v[s]);
+  }
+  for (int l = 0; l < kp; l++) {
+    const bool cross = l < ncross;  // uniform
+    P x[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) x[e] = (P)0;
+    accum(v, l, cross, x);
+    if (l + 1 < kp) {
+#pragma unroll
+      for (int s = 0; s < kAcaIpt; s++) PoolAt<S>::load(pool, (int64_t)(l + 1) * stride + off[s], v[s]);
+    }
+    if (cross) wave_sum_store<P, 9>(x, sh, X + 9 * l, 9);
+  }
+  if (ncross > 0) {
+    P x[9];
+    accum(a, k - 1, false, x);
   }
 }
 
