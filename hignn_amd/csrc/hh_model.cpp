@@ -96,7 +96,8 @@ struct hh_model {
   std::vector<int64_t> inst_counts;
   int prec = PREC_FP32;
   // byte budget of one far-field batch (factor pools + Gram histories); 0:
-  // half the device memory free at the first far field, at most 192 GiB
+  // 70% of the device memory free at the first far field, leaving at least
+  // 32 GiB (fewer, larger batches: each batch runs its own iteration loop)
   size_t far_pool_bytes = 0;
   hipStream_t stream = nullptr;
   // weights
@@ -593,7 +594,8 @@ struct hh_model {
     if (far_pool_bytes == 0) {
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      far_pool_bytes = std::min<size_t>(fr / 2, (size_t)192 << 30);
+      const size_t keep = (size_t)32 << 30;
+      far_pool_bytes = std::min<size_t>((size_t)(0.7 * (double)fr), fr > keep ? fr - keep : 0);
       far_pool_bytes = std::max<size_t>(far_pool_bytes, (size_t)1 << 30);
     }
     return far_pool_bytes;
