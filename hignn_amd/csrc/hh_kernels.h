@@ -40,6 +40,8 @@ struct NearLaunch {
   DevWeightsH wh;
   int max_blocks;  // grid cap for the FP32/FP64 kernels
   int cus;         // compute units (grid of the LDS-resident 16-bit kernel)
+  int32_t *next;   // work-queue counter of the 16-bit kernel (zeroed by the
+                   // launch, on its stream), or null: round-robin tasks
 };
 
 // Counts of one batched ACA iteration.  They live in device memory, so the

@@ -332,6 +332,8 @@ struct NearSrc {
   const NearTask *tasks;
   int64_t ntasks;
   double *u;           // (.,3) accumulators
+  int32_t *next;       // work queue of the persistent kernel (zero before the
+                       // launch), or null: tasks dealt round-robin
 };
 
 // Epilogue of one MLP call: lane (g < 3, p) holds row g of the block of pair
@@ -375,7 +377,16 @@ __device__ __forceinline__ void near_body(const NearSrc<R> &S, Core core) {
   const int g = lane >> 4, p = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t task = wave; task < S.ntasks; task += nwaves) {
+  // First task: the wave's index; then, with a work queue, the next task not
+  // yet taken (tasks differ in cost with their blocks' source counts), else
+  // round-robin.
+  auto next_task = [&](int64_t task) -> int64_t {
+    if (!S.next) return task + nwaves;
+    int32_t t = 0;
+    if (lane == 0) t = atomicAdd(S.next, 1);
+    return nwaves + (int64_t)__builtin_amdgcn_readfirstlane(t);
+  };
+  for (int64_t task = wave; task < S.ntasks; task = next_task(task)) {
     const NearTask tk = S.tasks[task];
     const RowBlock B = S.blocks[tk.block];
     for (int tg = 0; tg < 4; tg += NT) {
@@ -652,17 +663,30 @@ __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo) {
   lo = (_Float16)(v - (float)hi);
 }
 
+// The hidden layers carry scaled pre-activations z' = -z log2 e (W1, b1 and
+// b2 scaled on loading; W2 unchanged, as log2 e ln 2 = 1) and the scaled
+// activation s' = z' sigma(z) = z' / (1 + 2^z'), so that silu z = -ln 2 s'
+// costs one exponential, one add, one reciprocal and one multiply; the
+// factor -ln 2 is folded into W3.
+constexpr float kSurScale = -1.4426950408889634f;  // -log2 e
+constexpr float kSurUnscale = -0.6931471805599453f;  // -ln 2
+__device__ __forceinline__ float silu_scaled(float zs) {
+  return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
+}
+
 struct SurMfma {
   float w1[2];
   f16x8 w2h[2], w2l[2], w3h, w3l;
   f32x4 b2v[2], b3v;
+  float one;  // 1.0 the compiler does not fold: lo = fma(-hi, one, v) stays one v_fma_mix_f32
 
   // The lane's weight fragments and biases, gathered from the parameters.
   __device__ __forceinline__ void init(const float *params, int g, int p) {
     using namespace sur;
+    asm volatile("v_mov_b32 %0, 1.0" : "=v"(one));
 #pragma unroll
     for (int t = 0; t < 2; t++)
-      w1[t] = g < 3 ? params[kOW1 + (16 * t + p) * kIn + g] : params[kOB1 + 16 * t + p];
+      w1[t] = kSurScale * (g < 3 ? params[kOW1 + (16 * t + p) * kIn + g] : params[kOB1 + 16 * t + p]);
 #pragma unroll
     for (int mt = 0; mt < 2; mt++)
 #pragma unroll
@@ -675,27 +699,31 @@ struct SurMfma {
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       _Float16 hi, lo;
-      split16(p < kOut ? params[kOW3 + p * kH + sur_feature(g, j)] : 0.0f, hi, lo);
+      split16(p < kOut ? kSurUnscale * params[kOW3 + p * kH + sur_feature(g, j)] : 0.0f, hi, lo);
       w3h[j] = hi;
       w3l[j] = lo;
     }
 #pragma unroll
     for (int mt = 0; mt < 2; mt++)
-      b2v[mt] = f32x4{params[kOB2 + 16 * mt + 4 * g], params[kOB2 + 16 * mt + 4 * g + 1],
-                      params[kOB2 + 16 * mt + 4 * g + 2], params[kOB2 + 16 * mt + 4 * g + 3]};
+#pragma unroll
+      for (int r = 0; r < 4; r++) b2v[mt][r] = kSurScale * params[kOB2 + 16 * mt + 4 * g + r];
 #pragma unroll
     for (int r = 0; r < 4; r++) b3v[r] = 4 * g + r < kOut ? params[kOB3 + 4 * g + r] : 0.0f;
   }
 
   // The lane's eight activations (two output registers of four of the layer
   // before) as split 16-bit B operands: K-slot 8g + j <- a[j / 4][j % 4].
-  __device__ __forceinline__ static void operands(const f32x4 (&a)[2], f16x8 &hi, f16x8 &lo) {
+  // hi = fp16(v) in pairs; lo = v - hi is exact in FP32 (one mixed-precision
+  // FMA reading hi's half), then rounded to FP16.
+  __device__ __forceinline__ void operands(const f32x4 (&a)[2], f16x8 &hi, f16x8 &lo) const {
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      _Float16 h, l;
-      split16(a[j / 4][j % 4], h, l);
-      hi[j] = h;
-      lo[j] = l;
+    for (int j = 0; j < 8; j += 2) {
+      const float v0 = a[j / 4][j % 4], v1 = a[j / 4][j % 4 + 1];
+      const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+      hi[j] = h0;
+      hi[j + 1] = h1;
+      lo[j] = (_Float16)__builtin_fmaf(-(float)h0, one, v0);
+      lo[j + 1] = (_Float16)__builtin_fmaf(-(float)h1, one, v1);
     }
   }
 
@@ -707,7 +735,7 @@ struct SurMfma {
     for (int t = 0; t < 2; t++) {
       a1[t] = mma(w1[t], bz, f32x4{0, 0, 0, 0});
 #pragma unroll
-      for (int r = 0; r < 4; r++) a1[t][r] = sur::silu(a1[t][r]);
+      for (int r = 0; r < 4; r++) a1[t][r] = silu_scaled(a1[t][r]);
     }
     f16x8 h1, l1;
     operands(a1, h1, l1);
@@ -718,7 +746,7 @@ struct SurMfma {
       c = mma(w2h[mt], l1, c);
       c = mma(w2l[mt], h1, c);
 #pragma unroll
-      for (int r = 0; r < 4; r++) c[r] = sur::silu(c[r]);
+      for (int r = 0; r < 4; r++) c[r] = silu_scaled(c[r]);
       a2[mt] = c;
     }
     f16x8 h2, l2;
@@ -2468,23 +2496,28 @@ static int grid_for(int64_t waves_needed, int waves_per_block, int max_blocks) {
   return (int)blocks;
 }
 
+__global__ void zero_i32_kernel(int32_t *p, int64_t n);
+
 hipError_t launch_near(const NearLaunch &L, hipStream_t st) {
   if (L.ntasks <= 0) return hipSuccess;
   if (L.prec == PREC_FP32) {
-    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u, nullptr};
     WF<float> W{(const float *)L.wf.w1b, (const float *)L.wf.w2f, (const float *)L.wf.b2,
                 (const float *)L.wf.w3f, (const float *)L.wf.b3};
     const int grid = grid_for(L.ntasks, 4, L.max_blocks);
     hipLaunchKernelGGL(near_fp32_kernel, dim3(grid), dim3(256), 0, st, S, W);
   } else if (L.prec == PREC_FP64) {
-    NearSrc<double> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    NearSrc<double> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u, nullptr};
     WF<double> W{(const double *)L.wf.w1b, (const double *)L.wf.w2f, (const double *)L.wf.b2,
                  (const double *)L.wf.w3f, (const double *)L.wf.b3};
     const int grid = grid_for(L.ntasks, 4, L.max_blocks);
     hipLaunchKernelGGL(near_fp64_kernel, dim3(grid), dim3(256), 0, st, S, W);
   } else {
     allow_big_lds();
-    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u};
+    if (L.next) {
+      hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
+    }
+    NearSrc<float> S{L.xp, L.fp, L.tlist, L.blocks, L.tasks, L.ntasks, L.u, L.next};
     WH W{L.wh.w1b, L.wh.b2, L.wh.b3, L.wh.w2f, L.wh.w3f};
     const size_t lds = kW2FragBytes + kW3FragBytes;
     const bool b16 = L.prec == PREC_BF16;
@@ -2748,6 +2781,11 @@ hipError_t launch_copy16(void *dst, const void *src, int64_t n16, hipStream_t st
   hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint4 *)dst,
                      (const uint4 *)src, n16);
   return hipGetLastError();
+}
+
+__global__ void zero_i32_kernel(int32_t *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 __global__ void zero_f64_kernel(double *p, int64_t n) {

@@ -108,7 +108,7 @@ struct hh_model {
   TreeLists tl;
   bool built = false;
   DBuf xp, perm, fp, u, fin, uout;
-  DBuf near_blocks, near_tasks;
+  DBuf near_blocks, near_tasks, near_next;
   int64_t n_near_tasks = 0;
   int64_t near_pairs = 0;  // pair-function evaluations of the near field
   DBuf dense_blocks, dense_tasks;
@@ -202,7 +202,7 @@ struct hh_model {
                     &Qpool, &chk_blocks, &chk_tasks, &chk_tlist, &chk_u, &toff_c, &toff_r, &tiles_fc,
                     &tiles_sc, &tiles_fr, &tiles_sr, &cfirst_c, &cfirst_r, &part_c, &part_r, &x_c, &x_r,
                     &invb, &a_first_i, &a_first_j, &w_i, &w_j, &cnt2[0], &cnt2[1], &plan_part,
-                    &rank_dev, &ent_acc})
+                    &rank_dev, &ent_acc, &near_next})
       b->release();
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_na)
@@ -511,6 +511,7 @@ struct hh_model {
     u.reserve(24 * (size_t)n);
     fin.reserve(12 * (size_t)n);
     uout.reserve(12 * (size_t)n);
+    near_next.reserve(256);
   }
 
   // ---- far field: persistent kernel -------------------------------------
@@ -1172,6 +1173,9 @@ struct hh_model {
     L.wh = devw_h(p);
     L.max_blocks = 8 * g_cus;
     L.cus = g_cus;
+    // one near launch of a matvec at a time per model: the persistent 16-bit
+    // kernel's work queue
+    L.next = (p == PREC_BF16 || p == PREC_FP16) ? near_next.as<int32_t>() : nullptr;
     HIPCHK(launch_near(L, st));
   }
 
@@ -1267,7 +1271,7 @@ struct hh_model {
       mix(v);
     for (const DBuf *b : {&xp, &perm, &fp, &u, &near_blocks, &near_tasks, &p_blk, &p_order, &p_rank,
                           &p_poolC, &p_poolQ, &p_Y, &fb_all, &Cpool, &Qpool, &Y, &x_c, &x_r,
-                          &part_c, &part_r, &w_i, &w_j, &jd_flag, &rank_dev})
+                          &part_c, &part_r, &w_i, &w_j, &jd_flag, &rank_dev, &near_next})
       mix((uint64_t)(uintptr_t)b->p);
     for (const DBuf *arr : {wf32, wf64}) for (int k = 0; k < 5; k++) mix((uint64_t)(uintptr_t)arr[k].p);
     for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)wh_f[k].p);
