@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size before the timed region "
                          "(known-size reference for HBM byte counters)")
+    ap.add_argument("--near-alone-steps", type=int, default=2,
+                    help="matvecs with the near field alone (not beside the far field), after "
+                         "the timed steps, for the near kernel's own rate")
     ap.add_argument("--surrogate-file", default="",
                     help="matvec bench: load the surrogate's parameters from this .npy if it "
                          "exists, else train and save them there")
@@ -600,6 +603,24 @@ def matvec_bench(args):
     e_near, e_far = work_counts(exp, ranks)
     stats = model.surrogate_stats() if use_sur else None
 
+    # the near kernel by itself: matvecs with the near field first and alone
+    # (the far field after it), outside the timed steps, in which the far
+    # iterations share the near kernel's compute units
+    near_alone = None
+    if args.near_alone_steps > 0:
+        model.set(near_overlap=False)
+        run(args.prec, use_sur)
+        pa = []
+        for _ in range(args.near_alone_steps):
+            run(args.prec, use_sur)
+            pa.append(model.profile())
+        model.set(near_overlap=True)
+        ms_a = sum(float(p[0][0]) for p in pa) / len(pa)
+        ev_a = sum(int(p[1][0]) for p in pa) / len(pa)
+        if ev_a > 0:
+            near_alone = (ms_a, MLP_FLOP * ev_a / (ms_a / 1000.0) / 1e12)
+            log(f"[bench] near kernel alone: {ms_a:.1f} ms, {near_alone[1]:.1f} TFLOP/s")
+
     # the same 16-bit MLP without the surrogate (every far entry from the full
     # MLP), one step: what the learned part gains
     u_nosur = t_nosur = ranks_nosur = None
@@ -666,6 +687,10 @@ def matvec_bench(args):
             "avg_launch_ms": round(ms_k / max(1, launches_k), 3),
             "launches_per_step": int(launches_k / args.steps),
             "near_kernel_tflops": rate("near"), "far_mlp_tflops": rate("far_mlp")}
+    if near_alone is not None:
+        roof.update({"near_alone_ms": round(near_alone[0], 3),
+                     "near_alone_tflops": round(near_alone[1], 2),
+                     "near_alone_frac": round(near_alone[1] / PEAK[args.prec], 4)})
 
     cpu = None
     if not args.no_cpu:

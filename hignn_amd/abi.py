@@ -55,6 +55,7 @@ def lib():
             "hh_model_tune_far_field": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
             "hh_model_tune_near": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
             "hh_model_set_near_shape": [c_vp, c_int],
+            "hh_model_set_near_overlap": [c_vp, c_int],
             "hh_surrogate_create": [ctypes.c_float, ctypes.c_float, c_vp],
             "hh_surrogate_range": [c_vp, c_vp, c_vp],
             "hh_surrogate_set_params": [c_vp, c_vp],
@@ -187,8 +188,10 @@ class Model:
         _chk(lib().hh_model_load_two_body(self.h, _p(w), w.size))
 
     def set(self, epsilon=None, max_iter=None, use_symmetry=None, max_relative_coord=None,
-            precision=None, tree_reuse=None, tree_build=None):
+            precision=None, tree_reuse=None, tree_build=None, near_overlap=None):
         L = lib()
+        if near_overlap is not None:
+            _chk(L.hh_model_set_near_overlap(self.h, int(bool(near_overlap))))
         if tree_build is not None:
             _chk(L.hh_model_set_tree_build(self.h, {"host": 0, "device": 1}[tree_build]))
         if tree_reuse is not None:

@@ -1699,6 +1699,14 @@ int hh_model_update_coord_device(hh_model *m, const float *coord_dev, int64_t n,
   });
 }
 
+int hh_model_set_near_overlap(hh_model *m, int on) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (on != 0 && on != 1) throw HHError(HH_ERR_ARG, "near overlap: 0 or 1");
+    m->near_overlap = on != 0;
+  });
+}
+
 int hh_model_set_tree_build(hh_model *m, int device) {
   return guarded([&]() {
     if (!m) throw HHError(HH_ERR_ARG, "NULL model");
