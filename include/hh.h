@@ -189,11 +189,12 @@ int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *st
  * reference's close-field launch is fixed (src/CloseDot.cpp:25-337). */
 int hh_model_set_near_shape(hh_model *m, int shape);
 
-/* Near field beside the far field (1, the default; HH_NEAR_OVERLAP=0 in the
- * environment makes new models start with 0): the near kernel runs on a
- * second stream beside the ACA iterations of the blocks on surrogate entries
- * (or beside the persistent far kernel); 0 runs the near field first and
- * alone, e.g. to time the near kernel by itself.  Same result up to the
+/* Near field beside the batched far field (1, the default; HH_NEAR_OVERLAP=0
+ * in the environment makes new models start with 0): the near kernel runs on
+ * a second stream beside the ACA iterations (with a surrogate: beside those
+ * of the blocks on surrogate entries); 0 runs the near field first and alone,
+ * then the far field, e.g. to time the near kernel by itself.  The
+ * persistent far kernel always runs beside the near field.  Same result up to the
  * order of the velocity sums.  No reference counterpart: the reference runs
  * CloseDot and FarDot one after the other (src/HignnModel.cpp:545-634). */
 int hh_model_set_near_overlap(hh_model *m, int on);
