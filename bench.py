@@ -39,11 +39,17 @@ sys.path.insert(0, ROOT)
 MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
 # HBM bytes per pair-function evaluation (read + written), measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this benchmark at N=1M, each
-# scaled by the counter's reading for a 1 GiB device copy (x2.0 and x1.0):
-# the near kernel reads 0.63 B and writes 7.49 B per evaluation (its FP64
-# velocity atomics, as the L2 issues them); the ACA passes of the full MLP
-# (leaf 256, FP16, earlier setting) wrote 36 B of entries and read 9.5 B.
-TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": 8.12}
+# scaled by the counter's reading for a 1 GiB device copy (x2.0 and x1.0).
+# The near kernel's writes are its FP64 velocity atomics as the L2 issues
+# them; the source-side ones come once per MLP call of T targets, so they
+# depend on the tile shape: with 4 tiles per call it reads 0.63 B and writes
+# 7.49 B per evaluation, with 2 tiles 0.46 B and 13.81 B.  The ACA passes of
+# the full MLP (leaf 256, FP16, earlier setting) wrote 36 B of entries and
+# read 9.5 B.
+NEAR_TRAFFIC_PER_PAIR = {4: 8.12, 2: 14.27}  # by tiles per MLP call
+NEAR_SHAPE_TILES = {0: 4, 1: 2, 2: 2, 3: 4}  # near kernel shapes (tiles, threads): 0 (4, 512),
+                                             # 1 (2, 512), 2 (2, 1024), 3 (4, 256)
+TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": NEAR_TRAFFIC_PER_PAIR[4]}
 PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # dense MFMA TFLOP/s
 
 
@@ -578,6 +584,9 @@ def matvec_bench(args):
     model.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
     near_tuning = model.tune_near(f_dev.ptr) if not args.no_tune_near else []
     log(f"[bench] near-kernel tile shapes (shape, tiles, threads, matvec ms): {near_tuning}")
+    near_shape = (int(min(near_tuning, key=lambda r: r[3])[0]) if near_tuning
+                  else int(os.environ.get("HH_NEAR_SHAPE", "0")))
+    TRAFFIC_PER_PAIR["near"] = NEAR_TRAFFIC_PER_PAIR[NEAR_SHAPE_TILES[near_shape]]
     far_tuning = model.tune_far_field(f_dev.ptr) if args.tune_far else []
     if far_tuning:
         log(f"[bench] far-field launch choices (mode, chunk, matvec ms): {far_tuning}")
@@ -682,7 +691,8 @@ def matvec_bench(args):
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[args.prec],
             "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4),
             "traffic": int(TRAFFIC_PER_PAIR[dom] * pairs_k / max(1, launches_k)),
-            "kernel": f"{kern[dom][0]}<{args.prec}>",
+            "kernel": f"{kern[dom][0]}<{args.prec}>"
+                      + (f" shape {near_shape} ({NEAR_SHAPE_TILES[near_shape]} tiles)" if dom == "near" else ""),
             "flop_per_launch": int(MLP_FLOP * pairs_k / max(1, launches_k)),
             "avg_launch_ms": round(ms_k / max(1, launches_k), 3),
             "launches_per_step": int(launches_k / args.steps),
