@@ -20,13 +20,10 @@ ARCH = os.environ.get("HH_ARCH", "gfx950")
 
 # -ffp-contract=off: the tree build must round every product on its own
 # (bit-exact tree vs. the oracle); fused multiply-adds in kernels are written
-# explicitly.  Packed FP32 ops are disabled for all device code.  MFMA
-# results go to ordinary VGPRs (gfx950 allows it), not to accumulation
-# registers that then need one v_accvgpr_read per result register.
+# explicitly.  Packed FP32 ops are disabled for all device code.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function",
 ]
 
