@@ -348,6 +348,12 @@ def main():
     ap.add_argument("--three-body", action="store_true",
                     help="time the three-body and self corrections on the gravity_field lattice "
                          "instead")
+    ap.add_argument("--workload", choices=["c2", "c4"], default=None,
+                    help="the full-size C2 (FP32 path + DOPRI steps) or C4 (filaments + surrogate) "
+                         "configuration instead of the headline matvec")
+    ap.add_argument("--c2-steps", type=int, default=8,
+                    help="accepted DOPRI steps of the C2 workload (toward its final time 0.2; "
+                         "~45 s each at N=100k under the reference's step control)")
     ap.add_argument("--instances", type=int, default=0,
                     help="time the many-instance workload with this many suspensions (64) "
                          "instead")
@@ -364,8 +370,178 @@ def main():
         return three_body_bench(args)
     if args.instances:
         return instances_bench(args)
+    if args.workload:
+        return workload_bench(args)
 
     return matvec_bench(args)
+
+
+def workload_bench(args):
+    """The full-size C2 and C4 configurations (hignn_amd.workloads.CONFIGS).
+
+    c2: 100,000 spheres uniform in a cube at 10% volume fraction (seed 1),
+    gravity, plain FP32 path (leaf 256, eps 1e-4, max rank 32, smooth seeded
+    weights): one matvec timed after a warm-up and checked against the FP64
+    evaluation at --check-targets targets, then --c2-steps steps of the
+    reference's "Rk4" (Dormand-Prince 5(4), dt 0.002, error threshold 1e-3)
+    with the device-resident integrator (forcing, tree rebuild and matvec per
+    stage on the device); value = accepted steps per second.
+
+    c4: 2,000 filaments of 50 beads (bond 2.2, 8%, seed 3), bonded Morse
+    (D 1, alpha 1, r0 2.2) plus gravity evaluated on the device, leaf 256,
+    eps 1e-4, max_iter 32; the surrogate trained on 10M displacements with
+    |r| log-uniform in [4, 200]; value = particle-updates/s of the BF16 +
+    surrogate matvec, with BF16 alone and the FP32 path timed beside it and
+    every form checked against the FP64 evaluation.
+
+    The reference's step control (include/TimeIntegrator.hpp:612-840) starts
+    every step at dt and compares the unnormalised L2 norm of the embedded
+    error over all particles with 1e-3.  At N = 100,000 the accepted steps
+    are ~1e-5 long, each after several attempts (~45 s per accepted step), so
+    reaching the final time 0.2 would take ~2e4 steps: the run stops after
+    --c2-steps accepted steps."""
+    from hignn_amd import abi, hignn, weights, workloads
+    from oracle import surrogate as S
+
+    name = {"c2": "c2_uniform_100k", "c4": "c4_filaments_100k"}[args.workload]
+    cfg = workloads.CONFIGS[name]
+    t0 = time.time()
+    x = workloads.positions(name)
+    n = x.shape[0]
+    log(f"[{args.workload}] positions N={n} in {time.time() - t0:.1f}s")
+    w = weights.smooth(0)
+    hignn.Init()
+    if args.workload == "c4":
+        fc = hignn.DeviceForcing()
+        fc.set_body_force([0.0, 0.0, -1.0])
+        fc.set_bonds(cfg["nfil"], cfg["nbead"], 1.0, 1.0, 2.2)
+        f = fc.eval(x)
+    else:
+        f = workloads.gravity(n)
+    m = abi.Model(x, cfg["leaf"])
+    m.load_weights(w)
+    m.set(epsilon=cfg["epsilon"], max_iter=cfg["max_iter"])
+    t_update, _ = time_update_coord(m)
+    counts = m.counts()
+    f_dev = abi.DeviceArray.of(f)
+    u_dev = abi.DeviceArray((n, 3))
+    sur = None
+
+    def run(prec, use_sur):
+        m.set(precision=prec)
+        m.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
+        abi.synchronize()
+        t = time.perf_counter()
+        m.dot_device(u_dev.ptr, f_dev.ptr)
+        abi.synchronize()
+        return time.perf_counter() - t
+
+    g = np.random.default_rng(7)
+    tg = np.sort(g.choice(n, size=min(args.check_targets, n), replace=False))
+    t0 = time.time()
+    ref = m.check_fp64(f, tg, 0)
+    t_chk = time.time() - t0
+
+    def rel(u):
+        return float(np.sqrt(np.sum((u[tg] - ref) ** 2) / np.sum(ref ** 2)))
+
+    cfg_line = {"workload": "", "n": n, "leaf": cfg["leaf"], "epsilon": cfg["epsilon"],
+                "max_iter": cfg["max_iter"], "tree_rows": counts[0], "close_pairs": counts[1],
+                "far_blocks": counts[2], "update_coord_s": round(t_update, 3),
+                "check_targets": int(tg.size), "check_s": round(t_chk, 1)}
+    if args.workload == "c2":
+        run("fp32", False)
+        t32 = min(run("fp32", False) for _ in range(max(1, args.fp32_steps)))
+        e32 = rel(u_dev.numpy())
+        log(f"[c2] fp32 matvec {t32:.3f}s, error vs FP64 {e32:.2e}")
+        hm = hignn.HignnModel(x, cfg["leaf"])
+        hm.load_two_body_weights(w)
+        hm.set_epsilon(cfg["epsilon"])
+        hm.set_max_iter(cfg["max_iter"])
+        fg = hignn.DeviceForcing()
+        fg.set_body_force([0.0, 0.0, -1.0])
+        it = hignn.ExplicitRk4()
+        it.set_time_step(0.002)
+        it.set_final_time(0.2)  # 100 steps of dt 0.002
+        it.set_max_steps(args.c2_steps)
+        it.set_num_rigid_body(n)
+        it.set_device_velocity(hm, fg)
+        state = np.zeros((n, 6), np.float32)
+        state[:, :3] = x
+        it.initialize(state)
+        t_start = [0.0]
+        accepted = [0]
+
+        def progress(pos, vel):  # one line per accepted step
+            accepted[0] += 1
+            log(f"[c2] accepted step {accepted[0]}: {time.perf_counter() - t_start[0]:.1f}s")
+
+        it.set_output_step(1)
+        it.set_output_func(progress)
+        abi.synchronize()
+        t0 = t_start[0] = time.perf_counter()
+        it.run()
+        abi.synchronize()
+        t_run = time.perf_counter() - t0
+        st = it.device_stats()
+        moved = float(np.abs(it.positions() - x).max())
+        log(f"[c2] {it.func_count()} DOPRI steps in {t_run:.1f}s: {st}, moved up to {moved:.3f}")
+        cfg_line.update({
+            "workload": f"uniform cube N={n}, 10% volume fraction, seed 1, gravity; FP32 path; "
+                        f"the first {args.c2_steps} accepted steps of Dormand-Prince 5(4) (the "
+                        "reference's Rk4: dt 0.002 per step, final time 0.2, threshold 1e-3), "
+                        "device-resident integrator with a tree rebuild per stage",
+            "fp32_matvec_s": round(t32, 4), "rel_err_fp64_fp32_path": e32,
+            "dopri_steps": it.func_count(), "velocity_evaluations": st["velocity_evaluations"],
+            "rejected": st["rejected"], "integration_s": round(t_run, 2),
+            "max_displacement": round(moved, 4)})
+        line = {"metric": "C2: FP32-path DOPRI steps/s at N=100k (device-resident integrator)",
+                "value": round(it.func_count() / t_run, 4), "unit": "steps/s", "n_gpus": 1,
+                "steps": it.func_count(), "warmup": 0,
+                "ms_per_step": round(1000.0 * t_run / max(1, it.func_count()), 2),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                "data": "synthetic", "config": cfg_line}
+    else:
+        t0 = time.time()
+        r_s, t_s = m.surrogate_samples(10_000_000, 4.0, 200.0, seed=1)
+        sur = abi.Surrogate(4.0, 200.0)
+        sur.set_params(S.init(0))
+        loss = sur.train(r_s.ptr, t_s.ptr, 10_000_000, args.sur_steps, 3e-3, batch=16384,
+                         lr_final=3e-5)
+        abi.synchronize()
+        del r_s, t_s
+        t_train = time.time() - t0
+        log(f"[c4] surrogate trained in {t_train:.1f}s (loss {loss:.3e})")
+        times = {}
+        errs = {}
+        for form, prec, use_sur, k in (("bf16+surrogate", args.prec, True, args.steps),
+                                       ("bf16", args.prec, False, 1),
+                                       ("fp32", "fp32", False, max(1, args.fp32_steps))):
+            run(prec, use_sur)
+            times[form] = min(run(prec, use_sur) for _ in range(k))
+            errs[form] = rel(u_dev.numpy())
+            if use_sur:
+                stats = m.surrogate_stats()
+                ranks = m.far_ranks()
+            log(f"[c4] {form}: {times[form]:.3f}s, error vs FP64 {errs[form]:.2e}")
+        cfg_line.update({
+            "workload": f"{cfg['nfil']} filaments x {cfg['nbead']} beads (N={n}), bond 2.2, 8%, "
+                        "seed 3; bonded Morse D 1, alpha 1, r0 2.2 plus gravity (device forcing); "
+                        "surrogate 3-32-32-6 SiLU trained on 10M samples, |r| in [4, 200]",
+            "surrogate_train_s": round(t_train, 2), "surrogate_loss": loss,
+            "surrogate_stats": dict(zip(["far_blocks_judged", "on_surrogate", "fell_back",
+                                         "surrogate_entries", "full_mlp_far_entries"], stats)),
+            "mean_far_rank": float(ranks.mean()) if ranks.size else None,
+            "matvec_s": {k: round(v, 4) for k, v in times.items()},
+            "rel_err_fp64": errs,
+            "speedup_vs_fp32": round(times["fp32"] / times["bf16+surrogate"], 3)})
+        line = {"metric": "C4: BF16+surrogate matvec particle-updates/s at N=100k filaments",
+                "value": round(n / times["bf16+surrogate"], 2), "unit": "particle-updates/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                "ms_per_step": round(1000.0 * times["bf16+surrogate"], 2),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": args.prec, "data": "synthetic", "config": cfg_line}
+    print(json.dumps(line), flush=True)
 
 
 def instances_bench(args):

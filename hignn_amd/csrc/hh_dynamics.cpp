@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "../../include/hh_dynamics.h"
@@ -262,6 +264,9 @@ struct hh_stepper {
   // The rebuild uses the first stage's input positions, as the first build
   // of the run does.
   int64_t rebuild_every = 0, rebuilds = 0;
+  // HH_STEPPER_LOG=1: one line on stderr per adaptive attempt (the reference
+  // prints each step's time and dt, include/TimeIntegrator.hpp:612-615)
+  const bool log_steps = std::getenv("HH_STEPPER_LOG") && std::atoi(std::getenv("HH_STEPPER_LOG")) != 0;
   bool pending_rebuild = false;
   void maybe_rebuild() {
     if (rebuild_every > 0 && count > 0 && count % rebuild_every == 0) pending_rebuild = true;
@@ -344,6 +349,11 @@ struct hh_stepper {
         HIPCHK(hipMemcpyAsync(herr.p, err.p, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         e = std::sqrt(*herr.as<double>());
+        if (log_steps) {
+          std::fprintf(stderr, "[stepper] t %.6f h %.4e err %.4e %s (%lld evaluations)\n", t, h, e,
+                       e > thr ? "rejected" : "accepted", (long long)evals);
+          std::fflush(stderr);
+        }
         if (e > thr) {
           h = h * std::max(0.8 * std::pow(e / thr, -0.2), 0.1);
           h = std::max(h, 1e-6);

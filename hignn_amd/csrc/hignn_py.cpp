@@ -462,6 +462,12 @@ class Integrator {
   void set_y_lim(py::list l) { set_lim(1, l); }
   void set_z_lim(py::list l) { set_lim(2, l); }
   void set_output_step(int s) { out_step_ = s; }
+  // device-resident runs: stop after this many accepted steps (0: run to the
+  // final time; hh_stepper_run's max_steps)
+  void set_max_steps(int64_t s) {
+    if (s < 0) throw std::runtime_error("max_steps must be >= 0");
+    max_steps_ = s;
+  }
   void initialize(py::array_t<float, py::array::forcecast> init) {
     if (init.ndim() != 2) throw std::runtime_error("Number of dimensions of input positions must be two");
     if ((size_t)init.shape(0) != n_) throw std::runtime_error("Inconsistent number of rigid bodies");
@@ -561,6 +567,7 @@ class Integrator {
   double dt_ = 1.0, tf_ = 10.0;
   int out_step_ = 1, count_ = 0;
   bool periodic_ = false, custom_out_ = false;
+  int64_t max_steps_ = 0;
   float lim_[2][3] = {{-1, -1, -1}, {1, 1, 1}};
   std::vector<V3> pos_, pos0_, vel_, off_;
   std::vector<float> ang_;
@@ -602,7 +609,7 @@ class Integrator {
       for (int j = 0; j < 3; j++) x[3 * i + j] = pos_[i].v[j];
     check(hh_stepper_initialize(stepper_, x.data(), (int64_t)n_));
     check(hh_stepper_set_output(stepper_, out_step_, custom_out_ ? &device_output : nullptr, this));
-    check(hh_stepper_run(stepper_, 0));
+    check(hh_stepper_run(stepper_, max_steps_));
     std::vector<float> v(3 * n_);
     check(hh_stepper_state(stepper_, x.data(), v.data(), nullptr, stats_));
     for (size_t i = 0; i < n_; i++)
@@ -774,6 +781,7 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_y_lim", &Euler::set_y_lim)
       .def("set_z_lim", &Euler::set_z_lim)
       .def("set_output_step", &Euler::set_output_step)
+      .def("set_max_steps", &Euler::set_max_steps)
       .def("initialize", &Euler::initialize)
       .def("run", &Euler::run)
       .def("func_count", &Euler::func_count)
@@ -797,6 +805,7 @@ PYBIND11_MODULE(hignn, m) {
       .def("set_y_lim", &Rk4::set_y_lim)
       .def("set_z_lim", &Rk4::set_z_lim)
       .def("set_output_step", &Rk4::set_output_step)
+      .def("set_max_steps", &Rk4::set_max_steps)
       .def("initialize", &Rk4::initialize)
       .def("run", &Rk4::run)
       .def("func_count", &Rk4::func_count)
