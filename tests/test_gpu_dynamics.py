@@ -107,7 +107,7 @@ def model(h, x, w, reuse=False):
 
 
 def integrate(h, cls, x, w, device, dt=0.02, tf=0.1, thr=None, lims=None, reuse=False,
-              graphs=True, morse=5.0, stats=None):
+              graphs=True, morse=5.0, stats=None, max_steps=None):
     m = model(h, x, w, reuse)
     fc = forcing(h, morse=morse)
     it = cls()
@@ -125,6 +125,8 @@ def integrate(h, cls, x, w, device, dt=0.02, tf=0.1, thr=None, lims=None, reuse=
     if device:
         it.set_device_velocity(m, fc)
         it.set_graphs(graphs)
+        if max_steps is not None:
+            it.set_max_steps(max_steps)
     else:
         def vel(t, pos):
             xx = np.ascontiguousarray(pos[:, :3])
@@ -190,6 +192,25 @@ def test_device_dopri_matches_host(xavier_w):
     assert ch == cd and len(oh) == len(od)
     print("dopri steps:", ch, "max |x_dev - x_host|:", np.abs(xd - xh).max())
     assert close(xd, xh)
+
+
+@pytest.mark.parametrize("cls", ["ExplicitEuler", "ExplicitRk4"])
+def test_device_max_steps_stops_at_the_full_runs_step(xavier_w, cls):
+    """set_max_steps(k): the device-resident run stops after k accepted steps,
+    at the positions the uncapped run reported after its k-th step."""
+    h = hignn()
+    x = suspension(6)
+    kw = dict(dt=0.002, tf=0.0006, thr=1e-2) if cls == "ExplicitRk4" else dict(dt=0.02, tf=0.1)
+    c = getattr(h, cls)
+    xf, cf, of = integrate(h, c, x, xavier_w, device=True, **kw)
+    assert cf >= 2 and len(of) == cf
+    k = cf - 1
+    xk, ck, ok = integrate(h, c, x, xavier_w, device=True, max_steps=k, **kw)
+    print(f"{cls}: {cf} steps uncapped, stopped after {ck}")
+    assert ck == k and len(ok) == k
+    assert close(xk, of[k - 1], 1e-6)
+    with pytest.raises(RuntimeError):
+        c().set_max_steps(-1)
 
 
 def test_device_euler_with_three_body_matches_host(xavier_w):
