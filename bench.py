@@ -3,8 +3,9 @@
 Workload (default): 1,000,000 spheres of radius 1 in 200 Gaussian blobs
 (sigma = 15), blob centres uniform in the cube sized for a 10% mean volume
 fraction, minimum centre distance 2.05, seed 2; gravity forcing (0,0,-1);
-leaf 128 (the workload does not fix it; measured at N=1M, 128 is fastest for
-both the chosen form and the FP32 path among 64, 128 and 256), ACA epsilon
+leaf 80 (the workload does not fix it; the chosen form measured at N=1M, s
+per matvec: leaf 48 1.453, 64 1.407, 80 1.383, 96 1.405, 128 1.491, 160 1.598,
+192 1.716; the FP32 path runs on the same tree), ACA epsilon
 1e-4, at most 60 ACA iterations, max_relative_coord 500000 (the reference's
 default); pair-function weights: the smooth seeded set of the 3-128-512-9
 MLP (hignn_amd.weights.smooth(0)).
@@ -15,8 +16,7 @@ the chosen form: BF16 MLP on MFMA with FP32 accumulation for the near field,
 far-field ACA entries from the learned surrogate (trained on the GPU on 10M
 samples labelled by the full MLP before the timed region) for every far
 block that passes the full MLP's per-block judge and from the BF16 MLP for
-the rest, BF16-stored ACA factors with FP32 arithmetic, FP64 norms and
-velocity sums.  The same form without the surrogate and the plain FP32 path
+the rest, FP32 ACA factors and arithmetic, FP64 norms and velocity sums.  The same form without the surrogate and the plain FP32 path
 are timed in the same run, and all are checked against an FP64 evaluation
 of the same block structure (dense far blocks) on a random subsample of
 targets.
@@ -41,12 +41,13 @@ MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this benchmark at N=1M, each
 # scaled by the counter's reading for a 1 GiB device copy (x2.0 and x1.0).
 # The near kernel's writes are its FP64 velocity atomics as the L2 issues
-# them; the source-side ones come once per MLP call of T targets, so they
-# depend on the tile shape: with 4 tiles per call it reads 0.63 B and writes
-# 7.49 B per evaluation, with 2 tiles 0.46 B and 13.81 B.  The ACA passes of
-# the full MLP (leaf 256, FP16, earlier setting) wrote 36 B of entries and
-# read 9.5 B.
-NEAR_TRAFFIC_PER_PAIR = {4: 8.12, 2: 14.27}  # by tiles per MLP call
+# them; the source-side ones come once per MLP call of T targets and the
+# target sums once per task and block, so they depend on the tile shape and
+# the leaf size: with 2 tiles per call it reads 0.76 B and writes 14.53 B per
+# evaluation at leaf 80 (0.46 B and 13.81 B at leaf 128), with 4 tiles 0.63 B
+# and 7.49 B at leaf 128.  The ACA passes of the full MLP (leaf 256, FP16,
+# earlier setting) wrote 36 B of entries and read 9.5 B.
+NEAR_TRAFFIC_PER_PAIR = {4: 8.12, 2: 15.29}  # by tiles per MLP call
 NEAR_SHAPE_TILES = {0: 4, 1: 2, 2: 2, 3: 4}  # near kernel shapes (tiles, threads): 0 (4, 512),
                                              # 1 (2, 512), 2 (2, 1024), 3 (4, 256)
 TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": NEAR_TRAFFIC_PER_PAIR[4]}
@@ -318,7 +319,7 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--max-iter", type=int, default=60)
-    ap.add_argument("--leaf", type=int, default=128)
+    ap.add_argument("--leaf", type=int, default=80)
     ap.add_argument("--max-relative-coord", type=int, default=500000)
     ap.add_argument("--epsilon", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
