@@ -1340,11 +1340,29 @@ __device__ __forceinline__ void col_resid_chunk(const AcaStepArgs &A, SC *Cp, co
 }
 
 // One wave per chunk, grid-stride over the iteration's chunks.
+// LDS of a residual-pass workgroup, sized by max_iter at launch (so that
+// more of these one-wave workgroups fit beside the near kernel's LDS): the
+// pivot factors piv[max_iter][9], the used pivots used[max_iter + 1] and the
+// cross-term exchange sh[kCrossN * 65].
+template <class P>
+__host__ __device__ constexpr size_t resid_lds_used_off(int max_iter) {
+  return sizeof(P) * 9 * (size_t)max_iter;
+}
+template <class P>
+__host__ __device__ constexpr size_t resid_lds_sh_off(int max_iter) {
+  return (resid_lds_used_off<P>(max_iter) + 4 * ((size_t)max_iter + 1) + 15) / 16 * 16;
+}
+template <class P>
+__host__ __device__ constexpr size_t resid_lds_bytes(int max_iter) {
+  return resid_lds_sh_off<P>(max_iter) + sizeof(P) * kCrossN * 65;
+}
+
 template <class P, class SC, class SQ>
 __global__ __launch_bounds__(64) void aca_col_resid_kernel(AcaStepArgs A, SC *Cp, const SQ *Qp) {
-  __shared__ P piv[128][9];
-  __shared__ int32_t used[128];
-  __shared__ P sh[kCrossN * 65];
+  extern __shared__ __attribute__((aligned(16))) unsigned char resid_lds[];
+  P(*piv)[9] = (P(*)[9])resid_lds;
+  int32_t *used = (int32_t *)(resid_lds + resid_lds_used_off<P>(A.max_iter));
+  P *sh = (P *)(resid_lds + resid_lds_sh_off<P>(A.max_iter));
   const int na = A.cnt->na, nc = A.cnt->ncc;
   for (int q = blockIdx.x; q < nc; q += gridDim.x) {
     col_resid_chunk<P, SC, SQ>(A, Cp, Qp, q, na, piv, used, sh);
@@ -1520,9 +1538,10 @@ __device__ __forceinline__ void row_resid_chunk(const AcaStepArgs &A, const SC *
 
 template <class P, class SC, class SQ>
 __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const SC *Cp, SQ *Qp) {
-  __shared__ P piv[128][9];
-  __shared__ int32_t used[129];
-  __shared__ P sh[kCrossN * 65];
+  extern __shared__ __attribute__((aligned(16))) unsigned char resid_lds[];
+  P(*piv)[9] = (P(*)[9])resid_lds;
+  int32_t *used = (int32_t *)(resid_lds + resid_lds_used_off<P>(A.max_iter));
+  P *sh = (P *)(resid_lds + resid_lds_sh_off<P>(A.max_iter));
   const int na = A.cnt->na, nc = A.cnt->ncr;
   for (int q = blockIdx.x; q < nc; q += gridDim.x) {
     row_resid_chunk<P, SC, SQ>(A, Cp, Qp, q, na, piv, used, sh);
@@ -2653,13 +2672,13 @@ template <class P, class SC, class SQ>
 static void step_launch(const AcaStepArgs &A, const AcaStepLaunch &L, int which, dim3 gc, dim3 gr,
                         dim3 ga, hipStream_t st) {
   if (which == 0) {
-    hipLaunchKernelGGL((aca_col_resid_kernel<P, SC, SQ>), gc, dim3(64), 0, st, A, (SC *)L.C,
-                       (const SQ *)L.Q);
+    hipLaunchKernelGGL((aca_col_resid_kernel<P, SC, SQ>), gc, dim3(64), resid_lds_bytes<P>(L.max_iter), st,
+                       A, (SC *)L.C, (const SQ *)L.Q);
     hipLaunchKernelGGL(aca_col_pivot_kernel<SC>, ga, dim3(64), 0, st, A, (const SC *)L.C);
     hipLaunchKernelGGL((aca_col_norm_kernel<P, SC>), gc, dim3(64), 0, st, A, (SC *)L.C);
   } else {
-    hipLaunchKernelGGL((aca_row_resid_kernel<P, SC, SQ>), gr, dim3(64), 0, st, A,
-                       (const SC *)L.C, (SQ *)L.Q);
+    hipLaunchKernelGGL((aca_row_resid_kernel<P, SC, SQ>), gr, dim3(64), resid_lds_bytes<P>(L.max_iter), st,
+                       A, (const SC *)L.C, (SQ *)L.Q);
   }
 }
 
