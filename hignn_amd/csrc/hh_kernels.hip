@@ -1611,8 +1611,10 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
   __syncthreads();  // pr / sh_am are reused by the next slot
 }
 
+// pr: 9 (max_iter - 2) doubles, sized at launch (fewer bytes of LDS than
+// max_iter = 128 needs, so that more workgroups fit beside the near kernel)
 __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
-  __shared__ double pr[126 * 9];
+  extern __shared__ double pr[];
   __shared__ ArgMax sh_am[4];
   const int na = A.cnt->na;
   for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, pr, sh_am);
@@ -2703,7 +2705,8 @@ hipError_t launch_aca_steps(const AcaStepLaunch &L, int which, hipStream_t st) {
   else
     step_launch<float, float, float>(A, L, which, gc, gr, ga, st);
   if (which == 1) {
-    hipLaunchKernelGGL(aca_finalize_kernel, ga, dim3(256), 0, st, A);
+    const size_t pr_bytes = sizeof(double) * 9 * (size_t)std::max(1, L.max_iter - 2);
+    hipLaunchKernelGGL(aca_finalize_kernel, ga, dim3(256), pr_bytes, st, A);
   }
   return hipGetLastError();
 }
