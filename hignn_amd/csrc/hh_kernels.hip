@@ -534,9 +534,13 @@ struct AcaMlpArgs {
 // writes its ceil(items / 16) tiles at toff[s] of the list of its kind; tile
 // u of a slot starts at item 16u.  Column pass: pivot x[js + jpiv[k]], items
 // x[is + i]; row pass: pivot x[is + ipiv[k]], items x[js + j].
-__global__ void expand_tiles_kernel(AcaExpandLaunch A) {
+// One wave per active slot: the lanes write the slot's consecutive tiles
+// (coalesced 24-byte records), grid-stride over the slots.
+__global__ __launch_bounds__(256) void expand_tiles_kernel(AcaExpandLaunch A) {
   const int na = A.cnt->na;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < na; s += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < na; s += nwaves) {
     const int b = A.active[s];
     const FarBlock fb = A.blk[b];
     const int k = A.it[b];
@@ -547,7 +551,7 @@ __global__ void expand_tiles_kernel(AcaExpandLaunch A) {
     const int64_t y0 = A.ybase[s];
     AcaPassTile *tl = (AcaPassTile *)((A.kind && A.kind[b]) ? A.tiles_sur : A.tiles_full);
     const int t0 = A.toff[s], nt = (lim + 15) / 16;
-    for (int t = 0; t < nt; t++) {
+    for (int t = lane; t < nt; t += 64) {
       const int off = 16 * t;
       tl[t0 + t] = AcaPassTile{piv, base + off, min(16, lim - off), 0, y0 + off};
     }
@@ -2598,7 +2602,7 @@ hipError_t launch_copy_ranks(const int32_t *it, int nb, int32_t *rank_out, const
 
 hipError_t launch_aca_expand(const AcaExpandLaunch &L, hipStream_t st) {
   if (L.grid <= 0) return hipSuccess;
-  const int64_t blocks = std::min<int64_t>((L.grid + 255) / 256, 4096);
+  const int64_t blocks = std::min<int64_t>((L.grid + 3) / 4, 8192);  // a wave per slot
   hipLaunchKernelGGL(expand_tiles_kernel, dim3((unsigned)blocks), dim3(256), 0, st, L);
   return hipGetLastError();
 }
