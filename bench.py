@@ -333,7 +333,8 @@ def main():
                     help="matvec bench: load the surrogate's parameters from this .npy if it "
                          "exists, else train and save them there")
     ap.add_argument("--no-tune-near", action="store_true",
-                    help="matvec bench: keep the default tile shape of the 16-bit near kernel")
+                    help="matvec bench: keep the default tile shape and workgroup count of the "
+                         "16-bit near kernel")
     ap.add_argument("--tune-far", action="store_true",
                     help="matvec bench: also time the far field's launch choices (6 matvecs)")
     ap.add_argument("--no-tune-far", action="store_true",
@@ -755,14 +756,19 @@ def matvec_bench(args):
         del src, dst
 
     use_sur = sur is not None
-    # tile shape of the 16-bit near kernel (and, with --tune-far, the far
+    # tile shape and workgroups of the 16-bit near kernel (and, with --tune-far, the far
     # field's launch choices) tuned on these forces, untimed
     model.set(precision=args.prec)
     model.set_surrogate(sur if use_sur else None, tol=args.sur_tol)
     near_tuning = model.tune_near(f_dev.ptr) if not args.no_tune_near else []
-    log(f"[bench] near-kernel tile shapes (shape, tiles, threads, matvec ms): {near_tuning}")
-    near_shape = (int(min(near_tuning, key=lambda r: r[3])[0]) if near_tuning
-                  else int(os.environ.get("HH_NEAR_SHAPE", "0")))
+    log(f"[bench] near-kernel launches (shape, tiles, threads, workgroups, matvec ms): {near_tuning}")
+    near_best = min(near_tuning, key=lambda r: r[4]) if near_tuning else None
+    near_shape = (int(near_best[0]) if near_tuning else int(os.environ.get("HH_NEAR_SHAPE", "0")))
+    # workgroups of the persistent near kernel (0: one per compute unit) and
+    # the compute units that the tuner's first candidate covered
+    near_wg = (int(near_best[3]) if near_tuning
+               else int(os.environ.get("HH_NEAR_WORKGROUPS", "0")))
+    n_cus = int(near_tuning[0][3]) if near_tuning else None
     TRAFFIC_PER_PAIR["near"] = NEAR_TRAFFIC_PER_PAIR[NEAR_SHAPE_TILES[near_shape]]
     far_tuning = model.tune_far_field(f_dev.ptr) if args.tune_far else []
     if far_tuning:
@@ -795,12 +801,14 @@ def matvec_bench(args):
     near_alone = None
     if args.near_alone_steps > 0:
         model.set(near_overlap=False)
+        model.set_near_workgroups(0)  # alone, the kernel gets every compute unit
         run(args.prec, use_sur)
         pa = []
         for _ in range(args.near_alone_steps):
             run(args.prec, use_sur)
             pa.append(model.profile())
         model.set(near_overlap=True)
+        model.set_near_workgroups(near_wg)
         ms_a = sum(float(p[0][0]) for p in pa) / len(pa)
         ev_a = sum(int(p[1][0]) for p in pa) / len(pa)
         if ev_a > 0:
@@ -869,11 +877,17 @@ def matvec_bench(args):
             "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4),
             "traffic": int(TRAFFIC_PER_PAIR[dom] * pairs_k / max(1, launches_k)),
             "kernel": f"{kern[dom][0]}<{args.prec}>"
-                      + (f" shape {near_shape} ({NEAR_SHAPE_TILES[near_shape]} tiles)" if dom == "near" else ""),
+                      + (f" shape {near_shape} ({NEAR_SHAPE_TILES[near_shape]} tiles)"
+                         + (f" on {near_wg} workgroups" if near_wg else "") if dom == "near" else ""),
             "flop_per_launch": int(MLP_FLOP * pairs_k / max(1, launches_k)),
             "avg_launch_ms": round(ms_k / max(1, launches_k), 3),
             "launches_per_step": int(launches_k / args.steps),
             "near_kernel_tflops": rate("near"), "far_mlp_tflops": rate("far_mlp")}
+    if dom == "near" and near_wg and n_cus:
+        # the near kernel ran on near_wg of the n_cus compute units (one
+        # workgroup per compute unit); the far kernels held the others
+        roof.update({"near_workgroups": near_wg, "compute_units": n_cus,
+                     "frac_of_its_compute_units": round(achieved / (PEAK[args.prec] * min(1.0, near_wg / n_cus)), 4)})
     if near_alone is not None:
         roof.update({"near_alone_ms": round(near_alone[0], 3),
                      "near_alone_tflops": round(near_alone[1], 2),
@@ -943,8 +957,8 @@ def matvec_bench(args):
             "mean_far_rank": float(ranks.mean()) if ranks.size else 0.0,
             "update_coord_s": round(t_update, 3),
             "update_coord_host_build_s": round(t_update_host, 3),
-            "near_shape_tuning": [{"shape": a, "tiles": b, "threads": c, "ms": round(d, 1)}
-                                  for a, b, c, d in near_tuning],
+            "near_launch_tuning": [{"shape": a, "tiles": b, "threads": c, "workgroups": d,
+                                    "ms": round(e, 1)} for a, b, c, d, e in near_tuning],
             "far_launch_tuning": [{"mode": a, "chunk": b, "ms": round(c, 1)} for a, b, c in far_tuning],
             "close_ms": round(float(np.median(tcl)), 2), "far_ms": round(float(np.median(tfar)), 2),
             "far_mlp_ms": round(far_mlp_ms, 2),

@@ -55,6 +55,7 @@ def lib():
             "hh_model_tune_far_field": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
             "hh_model_tune_near": [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp],
             "hh_model_set_near_shape": [c_vp, c_int],
+            "hh_model_set_near_workgroups": [c_vp, c_int],
             "hh_model_set_near_overlap": [c_vp, c_int],
             "hh_surrogate_create": [ctypes.c_float, ctypes.c_float, c_vp],
             "hh_surrogate_range": [c_vp, c_vp, c_vp],
@@ -243,16 +244,22 @@ class Model:
         return [(int(a), int(b), float(c)) for a, b, c in t[:rows.value]]
 
     def tune_near(self, f_ptr, stream=None):
-        """Rows (shape, tiles, threads, matvec ms) of hh_model_tune_near;
-        the model keeps the fastest 16-bit near-kernel tile shape."""
-        t = np.zeros((8, 4), np.float64)
+        """Rows (shape, tiles, threads, workgroups, matvec ms) of
+        hh_model_tune_near; the model keeps the fastest 16-bit near-kernel
+        tile shape and workgroup count."""
+        t = np.zeros((16, 5), np.float64)
         rows = ctypes.c_int(0)
         _chk(lib().hh_model_tune_near(self.h, c_vp(f_ptr), self.n, c_vp(stream) if stream else None,
-                                      _p(t), 8, ctypes.byref(rows)))
-        return [(int(a), int(b), int(c), float(d)) for a, b, c, d in t[:rows.value]]
+                                      _p(t), 16, ctypes.byref(rows)))
+        return [(int(a), int(b), int(c), int(d), float(e)) for a, b, c, d, e in t[:rows.value]]
 
     def set_near_shape(self, shape):
         _chk(lib().hh_model_set_near_shape(self.h, int(shape)))
+
+    def set_near_workgroups(self, workgroups):
+        """Workgroups of the persistent 16-bit near kernel (0: one per
+        compute unit)."""
+        _chk(lib().hh_model_set_near_workgroups(self.h, int(workgroups)))
 
     def dot_capturable(self):
         """(capturable, signature) of hh_model_dot_capturable."""

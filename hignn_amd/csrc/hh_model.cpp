@@ -159,6 +159,7 @@ struct hh_model {
   bool verbose = false;  // HH_VERBOSE=1: far-field progress lines on stderr
   int aca_chunk = kAcaChunk;  // HH_ACA_CHUNK: items per ACA step chunk (1..kAcaChunk)
   int near_hcfg = 0;          // tile shape of the 16-bit near kernel (kNearShapes)
+  int near_cus = 0;           // workgroups of the persistent 16-bit near kernel (0: one per CU)
   DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
   bool p_ready = false;
   int64_t p_slotC = 0, p_slotQ = 0, p_slotY = 0;
@@ -1172,7 +1173,7 @@ struct hh_model {
     L.wf = devw_f(p == PREC_FP64);
     L.wh = devw_h(p);
     L.max_blocks = 8 * g_cus;
-    L.cus = g_cus;
+    L.cus = near_workgroups();
     // one near launch of a matvec at a time per model: the persistent 16-bit
     // kernel's work queue
     L.next = (p == PREC_BF16 || p == PREC_FP16) ? near_next.as<int32_t>() : nullptr;
@@ -1267,7 +1268,7 @@ struct hh_model {
                        (uint64_t)p_slotQ, (uint64_t)p_slotY, (uint64_t)far.size(), builds, loads,
                        (uint64_t)resolved_far_mode(), (uint64_t)aca_control, batch_key,
                        (uint64_t)(uintptr_t)sur, tol, (uint64_t)sur_samples, (uint64_t)near_overlap,
-                       (uint64_t)near_hcfg})
+                       (uint64_t)near_hcfg, (uint64_t)near_cus})
       mix(v);
     for (const DBuf *b : {&xp, &perm, &fp, &u, &near_blocks, &near_tasks, &p_blk, &p_order, &p_rank,
                           &p_poolC, &p_poolQ, &p_Y, &fb_all, &Cpool, &Qpool, &Y, &x_c, &x_r,
@@ -1314,31 +1315,46 @@ struct hh_model {
     aca_chunk = pick[1];
   }
 
-  // Tile-shape tuning of the 16-bit near kernel on the caller's forces: each
-  // shape runs one untimed and one timed matvec (HIP events around the whole
-  // matvec: the near field runs beside the far field, so the best shape
-  // depends on both); the fastest is kept.  Rows of `out`: {shape, tiles,
-  // threads, matvec ms}.
-  void tune_near(const float *f_dev, hipStream_t st, std::vector<std::array<double, 4>> &out) {
+  int near_workgroups() const { return near_cus > 0 && near_cus < g_cus ? near_cus : g_cus; }
+
+  // Launch tuning of the 16-bit near kernel on the caller's forces: each
+  // candidate runs one untimed and one timed matvec (HIP events around the
+  // whole matvec: the near field runs beside the far field, so the best
+  // choice depends on both); the fastest is kept.  Candidates: every tile
+  // shape on one workgroup per compute unit, and, when the far field runs
+  // beside the near field, the first three shapes on 5/8 and 1/2 of the
+  // compute units (the far kernels then hold the other compute units at full
+  // occupancy instead of the one wave per SIMD that fits beside a near
+  // workgroup).  Rows of `out`: {shape, tiles, threads, workgroups, matvec ms}.
+  void tune_near(const float *f_dev, hipStream_t st, std::vector<std::array<double, 5>> &out) {
     check_ready();
     out.clear();
     if (prec != PREC_BF16 && prec != PREC_FP16) return;  // the FP32 kernel has one shape
-    const int h0 = near_hcfg;
-    int pick = h0;
+    std::vector<std::array<int, 2>> cand;
+    for (int c = 0; c < kNearShapeCount; c++) cand.push_back({c, 0});
+    if (!far.empty() && (resolved_far_mode() == 0 || near_overlap))
+      for (int wg : {g_cus * 5 / 8, g_cus / 2})
+        for (int c = 0; c < 3; c++) cand.push_back({c, wg});
+    const int h0 = near_hcfg, w0 = near_cus;
+    std::array<int, 2> pick{h0, w0};
     double best = 0;
     try {
-      for (int c = 0; c < kNearShapeCount; c++) {
-        near_hcfg = c;
+      for (const auto &c : cand) {
+        near_hcfg = c[0];
+        near_cus = c[1];
         for (int rep = 0; rep < 2; rep++) dot_device(uout.as<float>(), f_dev, st);
         HIPCHK(hipStreamSynchronize(st));
-        out.push_back({(double)c, (double)kNearShapes[c].tiles, (double)kNearShapes[c].threads, t_total});
+        out.push_back({(double)c[0], (double)kNearShapes[c[0]].tiles,
+                       (double)kNearShapes[c[0]].threads, (double)near_workgroups(), t_total});
         if (out.size() == 1 || t_total < best) best = t_total, pick = c;
       }
     } catch (...) {
       near_hcfg = h0;
+      near_cus = w0;
       throw;
     }
-    near_hcfg = pick;
+    near_hcfg = pick[0];
+    near_cus = pick[1];
   }
 
   // u = M(x) f without host synchronisation or timing (capturable state only).
@@ -1584,6 +1600,11 @@ int hh_model_create(const float *coord, int64_t n, int block_size, hh_model **ou
       if (v < 0 || v >= kNearShapeCount) throw HHError(HH_ERR_ARG, "HH_NEAR_SHAPE out of range");
       m->near_hcfg = v;
     }
+    if (const char *nw = std::getenv("HH_NEAR_WORKGROUPS")) {
+      const int v = std::atoi(nw);
+      if (v < 0) throw HHError(HH_ERR_ARG, "HH_NEAR_WORKGROUPS is negative");
+      m->near_cus = v;
+    }
     if (const char *c16 = std::getenv("HH_FAR_C16")) m->far_c16 = std::atoi(c16) != 0;
     if (const char *q16 = std::getenv("HH_FAR_Q16")) m->far_q16 = std::atoi(q16) != 0;
     if (const char *vb = std::getenv("HH_VERBOSE")) m->verbose = std::atoi(vb) != 0;
@@ -1737,16 +1758,24 @@ int hh_model_set_near_shape(hh_model *m, int shape) {
   });
 }
 
+int hh_model_set_near_workgroups(hh_model *m, int workgroups) {
+  return guarded([&]() {
+    if (!m) throw HHError(HH_ERR_ARG, "NULL model");
+    if (workgroups < 0) throw HHError(HH_ERR_ARG, "negative workgroup count");
+    m->near_cus = workgroups;
+  });
+}
+
 int hh_model_tune_near(hh_model *m, const float *f_dev, int64_t n, void *stream, double *table,
                        int max_rows, int *rows) {
   return guarded([&]() {
     if (!m || !f_dev || !rows) throw HHError(HH_ERR_ARG, "NULL argument");
     if (n != m->n) throw HHError(HH_ERR_ARG, "array length differs from the model's");
-    std::vector<std::array<double, 4>> out;
+    std::vector<std::array<double, 5>> out;
     m->tune_near(f_dev, stream ? (hipStream_t)stream : m->stream, out);
     const int nr = (int)std::min<size_t>(out.size(), (size_t)std::max(max_rows, 0));
     for (int r = 0; r < nr && table; r++)
-      for (int c = 0; c < 4; c++) table[4 * r + c] = out[(size_t)r][(size_t)c];
+      for (int c = 0; c < 5; c++) table[5 * r + c] = out[(size_t)r][(size_t)c];
     *rows = nr;
   });
 }

@@ -136,21 +136,23 @@ class Model {
     return out;
   }
 
-  // Tile-shape tuning of the 16-bit near kernel on device forces f; returns
-  // the rows (shape, tiles, threads, matvec ms) and keeps the fastest.
+  // Launch tuning of the 16-bit near kernel on device forces f; returns the
+  // rows (shape, tiles, threads, workgroups, matvec ms) and keeps the fastest.
   py::list tune_near(uintptr_t f, uintptr_t stream) {
-    double t[8 * 4];
+    double t[16 * 5];
     int rows = 0;
     {
       py::gil_scoped_release nogil;
-      check(hh_model_tune_near(m_, (const float *)f, n_, (void *)stream, t, 8, &rows));
+      check(hh_model_tune_near(m_, (const float *)f, n_, (void *)stream, t, 16, &rows));
     }
     py::list out;
     for (int r = 0; r < rows; r++)
-      out.append(py::make_tuple((int)t[4 * r], (int)t[4 * r + 1], (int)t[4 * r + 2], t[4 * r + 3]));
+      out.append(py::make_tuple((int)t[5 * r], (int)t[5 * r + 1], (int)t[5 * r + 2],
+                                (int)t[5 * r + 3], t[5 * r + 4]));
     return out;
   }
   void set_near_shape(int shape) { check(hh_model_set_near_shape(m_, shape)); }
+  void set_near_workgroups(int wg) { check(hh_model_set_near_workgroups(m_, wg)); }
 
   py::dict tree_export() {
     int64_t c[4];
@@ -763,6 +765,7 @@ PYBIND11_MODULE(hignn, m) {
       .def("tune_far_field", &Model::tune_far_field, py::arg("f_ptr"), py::arg("stream") = 0)
       .def("tune_near", &Model::tune_near, py::arg("f_ptr"), py::arg("stream") = 0)
       .def("set_near_shape", &Model::set_near_shape)
+      .def("set_near_workgroups", &Model::set_near_workgroups)
       .def("tree_export", &Model::tree_export)
       .def("far_ranks", &Model::far_ranks)
       .def("check_fp64", &Model::check_fp64, py::arg("f"), py::arg("targets"),

@@ -180,14 +180,27 @@ int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *st
 /* Tile shape of the 16-bit (BF16/FP16) near-field kernel: index 0..3 of
  * {16-pair tiles per MLP call, threads per persistent workgroup} =
  * {4, 512}, {2, 512}, {2, 1024}, {4, 256}; default 0 (HH_NEAR_SHAPE sets the
- * default of new models).  hh_model_tune_near times every shape on the
+ * default of new models).  hh_model_tune_near times launch choices on the
  * caller's forces (device array f_dev, (n,3) float32; one untimed and one
- * timed matvec each, since the near field shares the GPU with the far field)
- * and keeps the fastest; table receives up to max_rows rows of {shape,
- * tiles, threads, matvec ms}, *rows their number (0 in the FP32 mode, whose
- * kernel has one shape).  Every shape computes the same sums up to rounding.  No reference counterpart: the
- * reference's close-field launch is fixed (src/CloseDot.cpp:25-337). */
+ * timed matvec each, since the near field shares the GPU with the far field):
+ * every shape on one workgroup per compute unit and, when the far field runs
+ * beside the near field, shapes 0..2 on 5/8 and 1/2 of the compute units;
+ * it keeps the fastest shape and workgroup count.  table receives up to
+ * max_rows rows of {shape, tiles, threads, workgroups, matvec ms} (10 rows
+ * at most), *rows their number (0 in the FP32 mode, whose kernel has one
+ * shape).  Every shape computes the same sums up to rounding.  No reference
+ * counterpart: the reference's close-field launch is fixed
+ * (src/CloseDot.cpp:25-337). */
 int hh_model_set_near_shape(hh_model *m, int shape);
+
+/* Workgroups of the persistent 16-bit near kernel: 0 (the default;
+ * HH_NEAR_WORKGROUPS sets the default of new models) or any count at or above
+ * the number of compute units gives one per compute unit (its LDS holds one
+ * per compute unit); fewer leave whole compute units to the far-field
+ * kernels that run beside it.  The workgroups take their tasks from a shared
+ * queue, so every count computes the same sums up to rounding.  Negative
+ * counts are rejected.  No reference counterpart. */
+int hh_model_set_near_workgroups(hh_model *m, int workgroups);
 
 /* Near field beside the batched far field (1, the default; HH_NEAR_OVERLAP=0
  * in the environment makes new models start with 0): the near kernel runs on

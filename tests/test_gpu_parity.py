@@ -198,9 +198,11 @@ def test_tune_far_field(model_small, xavier_w):
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_near_shapes_and_tuning(model_small, xavier_w, prec):
     """Every tile shape of the 16-bit near kernel (16-pair tiles per MLP call
-    x threads per workgroup) computes the same matvec up to the FP64
-    summation order; hh_model_tune_near times all of them and keeps the
-    fastest; the FP32 mode has nothing to tune."""
+    x threads per workgroup) and every workgroup count computes the same
+    matvec up to the FP64 summation order; hh_model_tune_near times the
+    shapes on all compute units and, with a far field beside the near
+    field, shapes 0..2 on 5/8 and 1/2 of them, and keeps the fastest; the
+    FP32 mode has nothing to tune."""
     x, f = model_small["x"], model_small["f"]
     m = abi.Model(x, 64)
     m.load_weights(xavier_w)
@@ -214,12 +216,22 @@ def test_near_shapes_and_tuning(model_small, xavier_w, prec):
         assert O.rel_l2(us[shape], us[0]) <= 1e-12, shape
     with pytest.raises(RuntimeError):
         m.set_near_shape(4)
+    m.set_near_shape(0)
+    for wg in (1, 3, 100000):  # one workgroup drains the whole task queue
+        m.set_near_workgroups(wg)
+        assert O.rel_l2(m.dot(f), us[0]) <= 1e-12, wg
+    m.set_near_workgroups(0)
+    with pytest.raises(RuntimeError):
+        m.set_near_workgroups(-1)
     fd = abi.DeviceArray.of(f)
     rows = m.tune_near(fd.ptr)
-    print(f"near shapes ({prec}):", rows)
-    assert [r[0] for r in rows] == [0, 1, 2, 3]
-    assert [(r[1], r[2]) for r in rows] == [(4, 512), (2, 512), (2, 1024), (4, 256)]
-    assert all(r[3] > 0 for r in rows)
+    print(f"near launches ({prec}):", rows)
+    assert m.far_ranks().size > 0  # the capped candidates are timed
+    assert [r[0] for r in rows] == [0, 1, 2, 3, 0, 1, 2, 0, 1, 2]
+    assert [(r[1], r[2]) for r in rows[:4]] == [(4, 512), (2, 512), (2, 1024), (4, 256)]
+    cus = rows[0][3]
+    assert [r[3] for r in rows] == [cus] * 4 + [cus * 5 // 8] * 3 + [cus // 2] * 3
+    assert all(r[4] > 0 for r in rows)
     assert O.rel_l2(m.dot(f), us[0]) <= 1e-12
     m.set(precision="fp32")
     assert m.tune_near(fd.ptr) == []
