@@ -1560,40 +1560,55 @@ __global__ __launch_bounds__(64) void aca_row_resid_kernel(AcaStepArgs A, const 
 // holds mu2 of iteration k-1 without its cross terms, nu2[b] its nu2; it[b]
 // ends as the block's rank when it stops.
 __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, double *pr,
-                                              ArgMax *sh_am) {
+                                              ArgMax *sh_am, double *sh_g) {
+  // Every global load below depends only on the slot's block, so the loads of
+  // the pivot arg-max (the threads of the first row chunks), the Gram sums
+  // (lanes 32-43, one entry each) and the cross terms (threads 64-255) are in
+  // flight together before the first barrier.  Each sum keeps its order.
+  const int tid = threadIdx.x;
   const int b = A.active[slot];
-  const FarBlock fb = A.blk[b];
-  const int k = A.it[b];
   const int c0 = A.cfirst_c[slot], c1 = A.cfirst_c[slot + 1];
   const int r0 = A.cfirst_r[slot], r1 = A.cfirst_r[slot + 1];
-  ArgMax best{kNegHuge, -1};
-  for (int c = r0 + (int)threadIdx.x; c < r1; c += blockDim.x)
-    best = better(best, ArgMax{A.part_r[c].val, A.part_r[c].idx});
-  const ArgMax am = block_argmax(best, sh_am);
-  const int ncross = 9 * (k > 2 ? k - 2 : 0);
-  for (int t = threadIdx.x; t < ncross; t += blockDim.x) {
-    double sc = 0.0, sq = 0.0;
-    for (int c = c0; c < c1; c++) sc += A.x_c[(size_t)c * A.max_iter * 9 + t];
-    for (int c = r0; c < r1; c++) sq += A.x_r[(size_t)c * A.max_iter * 9 + t];
-    pr[t] = 2.0 * fabs(sc) * fabs(sq);
+  const int k = A.it[b];
+  double mu2 = 0.0, nu2_prev = 0.0;
+  int mn = 0;
+  if (tid == 0) {
+    mu2 = A.mu2[b];
+    nu2_prev = A.nu2[b];
+    const FarBlock fb = A.blk[b];
+    mn = fb.m < fb.n ? fb.m : fb.n;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mu2 = A.mu2[b];
+  ArgMax best{kNegHuge, -1};
+  for (int c = r0 + tid; c < r1; c += blockDim.x)
+    best = better(best, ArgMax{A.part_r[c].val, A.part_r[c].idx});
+  if (tid >= 32 && tid < 44) {  // column-pass Gram entry tid-32, row-pass entry tid-38
+    const int e = tid < 38 ? tid - 32 : tid - 38;
+    const AcaPart *P = tid < 38 ? A.part_c : A.part_r;
+    const int e0 = tid < 38 ? c0 : r0, e1 = tid < 38 ? c1 : r1;
+    double g = 0.0;
+    for (int c = e0; c < e1; c++) g += P[c].g[e];
+    sh_g[tid - 32] = g;
+  }
+  const int ncross = 9 * (k > 2 ? k - 2 : 0);
+  if (tid >= 64)
+    for (int t = tid - 64; t < ncross; t += (int)blockDim.x - 64) {
+      double sc = 0.0, sq = 0.0;
+      for (int c = c0; c < c1; c++) sc += A.x_c[(size_t)c * A.max_iter * 9 + t];
+      for (int c = r0; c < r1; c++) sq += A.x_r[(size_t)c * A.max_iter * 9 + t];
+      pr[t] = 2.0 * fabs(sc) * fabs(sq);
+    }
+  const ArgMax am = block_argmax(best, sh_am);  // its barriers publish pr and sh_g
+  if (tid == 0) {
     bool soft = false;
     if (k >= 1) {
       for (int t = 0; t < ncross; t++) mu2 += pr[t];  // the reference's order (l, d1, d2)
-      soft = A.nu2[b] < mu2 * A.eps2;
+      soft = nu2_prev < mu2 * A.eps2;
     }
     if (soft) {  // iteration k-1 met the tolerance: rank k
       A.mu2[b] = mu2;
       A.stop[slot] = 1;
     } else {
-      double GC[6] = {0, 0, 0, 0, 0, 0}, GQ[6] = {0, 0, 0, 0, 0, 0};
-      for (int c = c0; c < c1; c++)
-        for (int e = 0; e < 6; e++) GC[e] += A.part_c[c].g[e];
-      for (int c = r0; c < r1; c++)
-        for (int e = 0; e < 6; e++) GQ[e] += A.part_r[c].g[e];
+      const double *GC = sh_g, *GQ = sh_g + 6;
       const double nu2 = GC[0] * GQ[0] + GC[1] * GQ[1] + GC[2] * GQ[2] + 2.0 * GQ[3] * GC[3] +
                          2.0 * GQ[4] * GC[4] + 2.0 * GQ[5] * GC[5];
       A.jpiv[(size_t)b * (A.max_iter + 1) + k + 1] = am.idx < 0 ? 0 : am.idx;
@@ -1602,7 +1617,6 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
       A.nu2[b] = nu2;
       const int itn = k + 1;
       A.it[b] = itn;
-      const int mn = fb.m < fb.n ? fb.m : fb.n;
       // The tolerance test of this iteration waits for its cross terms (the
       // next residual passes), which add non-negative terms to m2: when
       // nu2 < m2 eps^2 already holds, that test holds too (rounding is
@@ -1612,7 +1626,7 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
       A.stop[slot] = (met || itn >= A.max_iter || itn >= mn) ? 1 : 0;
     }
   }
-  __syncthreads();  // pr / sh_am are reused by the next slot
+  __syncthreads();  // pr / sh_g / sh_am are reused by the next slot
 }
 
 // pr: 9 (max_iter - 2) doubles, sized at launch (fewer bytes of LDS than
@@ -1620,8 +1634,9 @@ __device__ __forceinline__ void finalize_slot(const AcaStepArgs &A, int slot, do
 __global__ __launch_bounds__(256) void aca_finalize_kernel(AcaStepArgs A) {
   extern __shared__ double pr[];
   __shared__ ArgMax sh_am[4];
+  __shared__ double sh_g[12];
   const int na = A.cnt->na;
-  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, pr, sh_am);
+  for (int slot = blockIdx.x; slot < na; slot += gridDim.x) finalize_slot(A, slot, pr, sh_am, sh_g);
 }
 
 // ---- chunked apply of stopped blocks (src/FarDot.cpp:1200-1388) -----------
