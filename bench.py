@@ -43,11 +43,11 @@ MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
 # The near kernel's writes are its FP64 velocity atomics as the L2 issues
 # them; the source-side ones come once per MLP call of T targets and the
 # target sums once per task and block, so they depend on the tile shape and
-# the leaf size: with 2 tiles per call it reads 0.76 B and writes 14.53 B per
-# evaluation at leaf 80 (0.46 B and 13.81 B at leaf 128), with 4 tiles 0.63 B
-# and 7.49 B at leaf 128.  The ACA passes of the full MLP (leaf 256, FP16,
-# earlier setting) wrote 36 B of entries and read 9.5 B.
-NEAR_TRAFFIC_PER_PAIR = {4: 8.12, 2: 15.29}  # by tiles per MLP call
+# the leaf size: at leaf 80, with 2 tiles per call it reads 0.76 B and writes
+# 14.53 B per evaluation, with 4 tiles 0.75 B and 8.35 B (at leaf 128: 0.46 B
+# and 13.81 B, 0.63 B and 7.49 B).  The ACA passes of the full MLP (leaf 256,
+# FP16, earlier setting) wrote 36 B of entries and read 9.5 B.
+NEAR_TRAFFIC_PER_PAIR = {4: 9.10, 2: 15.29}  # by tiles per MLP call, leaf 80
 NEAR_SHAPE_TILES = {0: 4, 1: 2, 2: 2, 3: 4}  # near kernel shapes (tiles, threads): 0 (4, 512),
                                              # 1 (2, 512), 2 (2, 1024), 3 (4, 256)
 TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": NEAR_TRAFFIC_PER_PAIR[4]}
