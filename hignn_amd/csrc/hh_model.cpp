@@ -1218,9 +1218,17 @@ struct hh_model {
     near_field(side, near_blocks, near_tasks, n_near_tasks, prec, u.as<double>(), nullptr);
     if (timed) HIPCHK(hipEventRecord(ev[1], side));
     HIPCHK(hipEventRecord(join_ev, side));
+    // The judge-rejected blocks' loop is latency-bound (few blocks, run to
+    // max_iter) and its full-MLP kernel needs a whole compute unit's LDS.
+    // When the 16-bit near kernel leaves compute units free, that loop runs
+    // first, beside the near kernel, and the surrogate blocks' loop follows;
+    // otherwise it waits for the near kernel to finish.
+    const bool rejected_first =
+        split && (prec == PREC_BF16 || prec == PREC_FP16) && near_workgroups() < g_cus;
     if (persistent) {
       far_field_persistent(st);
     } else if (split) {
+      if (rejected_first) far_loops(st, timed, 0);
       far_loops(st, timed, 1);  // beside the near field
     } else {
       far_field(st, timed);
@@ -1228,7 +1236,7 @@ struct hh_model {
     if (timed && !split) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
     if (split) {
-      far_loops(st, timed, 0);  // judge-rejected blocks, full MLP, after the near field
+      if (!rejected_first) far_loops(st, timed, 0);  // full MLP, after the near field
       far_finish();
       if (timed) HIPCHK(hipEventRecord(ev[2], st));
     }
@@ -1322,7 +1330,7 @@ struct hh_model {
   // whole matvec: the near field runs beside the far field, so the best
   // choice depends on both); the fastest is kept.  Candidates: every tile
   // shape on one workgroup per compute unit, and, when the far field runs
-  // beside the near field, the first three shapes on 5/8 and 1/2 of the
+  // beside the near field, the first three shapes on 5/8, 1/2 and 3/8 of the
   // compute units (the far kernels then hold the other compute units at full
   // occupancy instead of the one wave per SIMD that fits beside a near
   // workgroup).  Rows of `out`: {shape, tiles, threads, workgroups, matvec ms}.
@@ -1333,7 +1341,7 @@ struct hh_model {
     std::vector<std::array<int, 2>> cand;
     for (int c = 0; c < kNearShapeCount; c++) cand.push_back({c, 0});
     if (!far.empty() && (resolved_far_mode() == 0 || near_overlap))
-      for (int wg : {g_cus * 5 / 8, g_cus / 2})
+      for (int wg : {g_cus * 5 / 8, g_cus / 2, g_cus * 3 / 8})
         for (int c = 0; c < 3; c++) cand.push_back({c, wg});
     const int h0 = near_hcfg, w0 = near_cus;
     std::array<int, 2> pick{h0, w0};

@@ -184,10 +184,10 @@ int hh_model_tune_far_field(hh_model *m, const float *f_dev, int64_t n, void *st
  * caller's forces (device array f_dev, (n,3) float32; one untimed and one
  * timed matvec each, since the near field shares the GPU with the far field):
  * every shape on one workgroup per compute unit and, when the far field runs
- * beside the near field, shapes 0..2 on 5/8 and 1/2 of the compute units;
- * it keeps the fastest shape and workgroup count.  table receives up to
- * max_rows rows of {shape, tiles, threads, workgroups, matvec ms} (10 rows
- * at most), *rows their number (0 in the FP32 mode, whose kernel has one
+ * beside the near field, shapes 0..2 on 5/8, 1/2 and 3/8 of the compute
+ * units; it keeps the fastest shape and workgroup count.  table receives up
+ * to max_rows rows of {shape, tiles, threads, workgroups, matvec ms} (13
+ * rows at most), *rows their number (0 in the FP32 mode, whose kernel has one
  * shape).  Every shape computes the same sums up to rounding.  No reference
  * counterpart: the reference's close-field launch is fixed
  * (src/CloseDot.cpp:25-337). */

@@ -201,7 +201,7 @@ def test_near_shapes_and_tuning(model_small, xavier_w, prec):
     x threads per workgroup) and every workgroup count computes the same
     matvec up to the FP64 summation order; hh_model_tune_near times the
     shapes on all compute units and, with a far field beside the near
-    field, shapes 0..2 on 5/8 and 1/2 of them, and keeps the fastest; the
+    field, shapes 0..2 on 5/8, 1/2 and 3/8 of them, and keeps the fastest; the
     FP32 mode has nothing to tune."""
     x, f = model_small["x"], model_small["f"]
     m = abi.Model(x, 64)
@@ -227,10 +227,10 @@ def test_near_shapes_and_tuning(model_small, xavier_w, prec):
     rows = m.tune_near(fd.ptr)
     print(f"near launches ({prec}):", rows)
     assert m.far_ranks().size > 0  # the capped candidates are timed
-    assert [r[0] for r in rows] == [0, 1, 2, 3, 0, 1, 2, 0, 1, 2]
+    assert [r[0] for r in rows] == [0, 1, 2, 3] + [0, 1, 2] * 3
     assert [(r[1], r[2]) for r in rows[:4]] == [(4, 512), (2, 512), (2, 1024), (4, 256)]
     cus = rows[0][3]
-    assert [r[3] for r in rows] == [cus] * 4 + [cus * 5 // 8] * 3 + [cus // 2] * 3
+    assert [r[3] for r in rows] == [cus] * 4 + [cus * 5 // 8] * 3 + [cus // 2] * 3 + [cus * 3 // 8] * 3
     assert all(r[4] > 0 for r in rows)
     assert O.rel_l2(m.dot(f), us[0]) <= 1e-12
     m.set(precision="fp32")

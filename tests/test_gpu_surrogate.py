@@ -142,6 +142,20 @@ def test_judged_surrogate_matvec_vs_fp64(trained, prec):
     errs = m.surrogate_errors()
     assert int(np.sum(errs <= 2e-3)) == on_sur
     assert e_sur <= 1e-3
+    # on one near workgroup the 16-bit path runs the rejected blocks' loop
+    # first, beside the near kernel: the same ranks, judge split and
+    # velocities up to the order of the FP64 velocity sums (a bar at the
+    # median judged error sends half the blocks to the full MLP)
+    m.set_surrogate(s, tol=float(np.median(errs)))
+    u_a = m.dot(f)
+    ranks_a, stats_a = m.far_ranks(), tuple(m.surrogate_stats()[:3])
+    m.set_near_workgroups(1)
+    u_cap = m.dot(f)
+    m.set_near_workgroups(0)
+    assert stats_a[2] > 0.3 * judged
+    np.testing.assert_array_equal(m.far_ranks(), ranks_a)
+    assert tuple(m.surrogate_stats()[:3]) == stats_a
+    assert rel(u_cap, u_a) <= 1e-6
     # the surrogate is detached: the next matvec is the full-MLP one again
     m.set_surrogate(None)
     np.testing.assert_allclose(m.dot(f), u_full, rtol=0, atol=1e-6 * np.abs(u_full).max())
