@@ -676,24 +676,24 @@ __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo) {
 // The hidden layers carry scaled pre-activations z' = -z log2 e (W1, b1 and
 // b2 scaled on loading; W2 unchanged, as log2 e ln 2 = 1) and the scaled
 // activation s' = z' sigma(z) = z' / (1 + 2^z'), so that silu z = -ln 2 s'
-// costs one exponential, one add, one reciprocal and one multiply; the
-// factor -ln 2 is folded into W3.
+// costs one exponential, one add and one reciprocal for sigma; the product
+// z' sigma is formed inside the two mixed-precision FMAs that split it into
+// 16-bit operands.  The factor -ln 2 is folded into W3.
 constexpr float kSurScale = -1.4426950408889634f;  // -log2 e
 constexpr float kSurUnscale = -0.6931471805599453f;  // -ln 2
-__device__ __forceinline__ float silu_scaled(float zs) {
-  return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
+// sigma(z) of a scaled pre-activation z': s' = z' sig_scaled(z')
+__device__ __forceinline__ float sig_scaled(float zs) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
 }
 
 struct SurMfma {
   float w1[2];
   f16x8 w2h[2], w2l[2], w3h, w3l;
   f32x4 b2v[2], b3v;
-  float one;  // 1.0 the compiler does not fold: lo = fma(-hi, one, v) stays one v_fma_mix_f32
 
   // The lane's weight fragments and biases, gathered from the parameters.
   __device__ __forceinline__ void init(const float *params, int g, int p) {
     using namespace sur;
-    asm volatile("v_mov_b32 %0, 1.0" : "=v"(one));
 #pragma unroll
     for (int t = 0; t < 2; t++)
       w1[t] = kSurScale * (g < 3 ? params[kOW1 + (16 * t + p) * kIn + g] : params[kOB1 + 16 * t + p]);
@@ -721,46 +721,46 @@ struct SurMfma {
     for (int r = 0; r < 4; r++) b3v[r] = 4 * g + r < kOut ? params[kOB3 + 4 * g + r] : 0.0f;
   }
 
-  // The lane's eight activations (two output registers of four of the layer
-  // before) as split 16-bit B operands: K-slot 8g + j <- a[j / 4][j % 4].
-  // hi = fp16(v) in pairs; lo = v - hi is exact in FP32 (one mixed-precision
-  // FMA reading hi's half), then rounded to FP16.
-  __device__ __forceinline__ void operands(const f32x4 (&a)[2], f16x8 &hi, f16x8 &lo) const {
+  // The lane's eight activations v = z sigma (two output registers of four
+  // of the layer before, given as z and sigma) as split 16-bit B operands:
+  // K-slot 8g + j <- v[j / 4][j % 4].  hi = fp16(z sigma) and lo = fp16(z
+  // sigma - hi), each one mixed-precision FMA (z sigma - hi is exact in the
+  // FMA, hi read as a 16-bit operand), so hi + lo = z sigma to ~2^-22.
+  __device__ __forceinline__ void operands(const f32x4 (&z)[2], const f32x4 (&sg)[2], f16x8 &hi,
+                                           f16x8 &lo) const {
 #pragma unroll
-    for (int j = 0; j < 8; j += 2) {
-      const float v0 = a[j / 4][j % 4], v1 = a[j / 4][j % 4 + 1];
-      const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
-      hi[j] = h0;
-      hi[j + 1] = h1;
-      lo[j] = (_Float16)__builtin_fmaf(-(float)h0, one, v0);
-      lo[j + 1] = (_Float16)__builtin_fmaf(-(float)h1, one, v1);
+    for (int j = 0; j < 8; j++) {
+      const float zz = z[j / 4][j % 4], ss = sg[j / 4][j % 4];
+      const _Float16 h = (_Float16)__builtin_fmaf(zz, ss, 0.0f);
+      hi[j] = h;
+      lo[j] = (_Float16)__builtin_fmaf(zz, ss, -(float)h);
     }
   }
 
   // Outputs 4g + r (r = 0..3) of the tile's item p; bz is z_g of item p
   // (g < 3) or 1 (g = 3).  Every lane of the wave takes part.
   __device__ __forceinline__ f32x4 eval(float bz) const {
-    f32x4 a1[2];
+    f32x4 a1[2], s1[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
       a1[t] = mma(w1[t], bz, f32x4{0, 0, 0, 0});
 #pragma unroll
-      for (int r = 0; r < 4; r++) a1[t][r] = silu_scaled(a1[t][r]);
+      for (int r = 0; r < 4; r++) s1[t][r] = sig_scaled(a1[t][r]);
     }
     f16x8 h1, l1;
-    operands(a1, h1, l1);
-    f32x4 a2[2];
+    operands(a1, s1, h1, l1);
+    f32x4 a2[2], s2[2];
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) {
       f32x4 c = mma(w2h[mt], h1, b2v[mt]);
       c = mma(w2h[mt], l1, c);
       c = mma(w2l[mt], h1, c);
 #pragma unroll
-      for (int r = 0; r < 4; r++) c[r] = silu_scaled(c[r]);
+      for (int r = 0; r < 4; r++) s2[mt][r] = sig_scaled(c[r]);
       a2[mt] = c;
     }
     f16x8 h2, l2;
-    operands(a2, h2, l2);
+    operands(a2, s2, h2, l2);
     f32x4 y = mma(w3h, h2, b3v);
     y = mma(w3h, l2, y);
     y = mma(w3l, h2, y);
