@@ -514,6 +514,13 @@ def workload_bench(args):
         del r_s, t_s
         t_train = time.time() - t0
         log(f"[c4] surrogate trained in {t_train:.1f}s (loss {loss:.3e})")
+        near_tuning = []
+        if not args.no_tune_near:  # tile shape and workgroups of the near kernel, untimed
+            m.set(precision=args.prec)
+            m.set_surrogate(sur, tol=args.sur_tol)
+            near_tuning = m.tune_near(f_dev.ptr)
+            log(f"[c4] near-kernel launches (shape, tiles, threads, workgroups, matvec ms): "
+                f"{near_tuning}")
         times = {}
         errs = {}
         for form, prec, use_sur, k in (("bf16+surrogate", args.prec, True, args.steps),
@@ -535,6 +542,9 @@ def workload_bench(args):
                                          "surrogate_entries", "full_mlp_far_entries"], stats)),
             "mean_far_rank": float(ranks.mean()) if ranks.size else None,
             "matvec_s": {k: round(v, 4) for k, v in times.items()},
+            "near_launch": (dict(zip(["shape", "tiles", "threads", "workgroups", "ms"],
+                                     min(near_tuning, key=lambda r: r[4])))
+                            if near_tuning else None),
             "rel_err_fp64": errs,
             "speedup_vs_fp32": round(times["fp32"] / times["bf16+surrogate"], 3)})
         line = {"metric": "C4: BF16+surrogate matvec particle-updates/s at N=100k filaments",
@@ -593,6 +603,13 @@ def instances_bench(args):
         abi.synchronize()
         return time.perf_counter() - t
 
+    near_tuning = []
+    if not args.no_tune_near:  # tile shape and workgroups of the near kernel, untimed
+        m.set(precision=args.prec)
+        m.set_surrogate(sur, tol=args.sur_tol)
+        near_tuning = m.tune_near(f_dev.ptr)
+        log(f"[instances] near-kernel launches (shape, tiles, threads, workgroups, matvec ms): "
+            f"{near_tuning}")
     for i in range(args.warmup):
         log(f"[instances] warmup {i}: {run(args.prec, True):.3f}s")
     abi.synchronize()
@@ -643,6 +660,9 @@ def instances_bench(args):
             "per_instance_rel_err_fp64_median": float(np.median(errs)),
             "per_instance_rel_err_fp64": [round(e, 8) for e in errs],
             "check_targets_per_instance": 150,
+            "near_launch": (dict(zip(["shape", "tiles", "threads", "workgroups", "ms"],
+                                     min(near_tuning, key=lambda r: r[4])))
+                            if near_tuning else None),
             "fp32_value": round(n / t32, 2) if t32 else None,
             "speedup_vs_fp32": round(value / (n / t32), 3) if t32 else None,
         },
