@@ -160,6 +160,7 @@ struct hh_model {
   int aca_chunk = kAcaChunk;  // HH_ACA_CHUNK: items per ACA step chunk (1..kAcaChunk)
   int near_hcfg = 0;          // tile shape of the 16-bit near kernel (kNearShapes)
   int near_cus = 0;           // workgroups of the persistent 16-bit near kernel (0: one per CU)
+  int mlp_cus = 0;            // workgroups of the 16-bit full-MLP ACA kernel (0: one per CU)
   DBuf p_blk, p_order, p_rank, p_poolC, p_poolQ, p_Y;
   bool p_ready = false;
   int64_t p_slotC = 0, p_slotQ = 0, p_slotY = 0;
@@ -875,7 +876,7 @@ struct hh_model {
       ml.wf = devw_f(false);
       ml.wh = devw_h(prec);
       ml.max_blocks = 8 * g_cus;
-      ml.cus = g_cus;
+      ml.cus = mlp_cus > 0 ? mlp_cus : g_cus;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       const bool mlp = select_kind != 1;
       if (timed && mlp) {
@@ -1228,7 +1229,14 @@ struct hh_model {
     if (persistent) {
       far_field_persistent(st);
     } else if (split) {
-      if (rejected_first) far_loops(st, timed, 0);
+      if (rejected_first) {
+        // one full-MLP workgroup per compute unit the near kernel leaves
+        // free: the kernel's workgroups split its tiles evenly, so more than
+        // fit at once would run in two rounds
+        mlp_cus = g_cus - near_workgroups();
+        far_loops(st, timed, 0);
+        mlp_cus = 0;
+      }
       far_loops(st, timed, 1);  // beside the near field
     } else {
       far_field(st, timed);
