@@ -43,14 +43,16 @@ MLP_FLOP = 141_056  # 2*(3*128 + 128*512 + 512*9) per pair-function evaluation
 # The near kernel's writes are its FP64 velocity atomics as the L2 issues
 # them; the source-side ones come once per MLP call of T targets and the
 # target sums once per task and block, so they depend on the tile shape and
-# the leaf size: at leaf 80, with 2 tiles per call it reads 0.76 B and writes
-# 14.53 B per evaluation, with 4 tiles 0.75 B and 8.35 B (at leaf 128: 0.46 B
-# and 13.81 B, 0.63 B and 7.49 B).  The ACA passes of the full MLP (leaf 256,
-# FP16, earlier setting) wrote 36 B of entries and read 9.5 B.
-NEAR_TRAFFIC_PER_PAIR = {4: 9.10, 2: 15.29}  # by tiles per MLP call, leaf 80
+# the leaf size: at leaf 80, shape 0 (4 tiles per call, 128 workgroups) reads
+# 0.75 B and writes 8.35 B per evaluation, shape 1 (2 tiles, 256 workgroups)
+# 0.76 B and 14.53 B, shape 2 (2 tiles, 96 workgroups) 0.81 B and 14.88 B
+# (at leaf 128: 2 tiles 0.46 B and 13.81 B, 4 tiles 0.63 B and 7.49 B).
+# Shape 3 (4 tiles) is taken as shape 0.  The ACA passes of the full MLP
+# (leaf 256, FP16, earlier setting) wrote 36 B of entries and read 9.5 B.
+NEAR_TRAFFIC_PER_PAIR = {0: 9.10, 1: 15.29, 2: 15.69, 3: 9.10}  # by near shape, leaf 80
 NEAR_SHAPE_TILES = {0: 4, 1: 2, 2: 2, 3: 4}  # near kernel shapes (tiles, threads): 0 (4, 512),
                                              # 1 (2, 512), 2 (2, 1024), 3 (4, 256)
-TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": NEAR_TRAFFIC_PER_PAIR[4]}
+TRAFFIC_PER_PAIR = {"far_mlp": 45.6, "near": NEAR_TRAFFIC_PER_PAIR[0]}
 PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}  # dense MFMA TFLOP/s
 
 
@@ -789,7 +791,7 @@ def matvec_bench(args):
     near_wg = (int(near_best[3]) if near_tuning
                else int(os.environ.get("HH_NEAR_WORKGROUPS", "0")))
     n_cus = int(near_tuning[0][3]) if near_tuning else None
-    TRAFFIC_PER_PAIR["near"] = NEAR_TRAFFIC_PER_PAIR[NEAR_SHAPE_TILES[near_shape]]
+    TRAFFIC_PER_PAIR["near"] = NEAR_TRAFFIC_PER_PAIR[near_shape]
     far_tuning = model.tune_far_field(f_dev.ptr) if args.tune_far else []
     if far_tuning:
         log(f"[bench] far-field launch choices (mode, chunk, matvec ms): {far_tuning}")
